@@ -1,0 +1,308 @@
+"""ctypes view of the CPU oracle (oracle/phx_oracle.h).
+
+Test infrastructure only: the oracle is the parity checker of the HIP path, never part of
+the product.  The shared library is built by oracle/Makefile (``__graft_entry__.build()``
+runs it) into oracle/_build/.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIBPATH = os.path.join(ORACLE_DIR, "_build", "libphx_oracle.so")
+
+DBL_MAX = np.finfo(np.float64).max
+
+GEOM_MINKOWSKI, GEOM_SPHKS = 0, 1
+RECON_CONSTANT, RECON_LINEAR, RECON_WENO5, RECON_MP5 = 0, 1, 2, 3
+RIEMANN_LLF, RIEMANN_HLL = 0, 1
+FLOOR_CONSTANT, FLOOR_EXPX1_RHO_SIE, FLOOR_EXPX1_RHO_U, FLOOR_X1_RHO_SIE, FLOOR_R_RHO_SIE = range(5)
+
+
+class Geom(C.Structure):
+    _fields_ = [("kind", C.c_int), ("a", C.c_double)]
+
+
+class Bounds(C.Structure):
+    _fields_ = [("floor_type", C.c_int), ("r0", C.c_double), ("s0", C.c_double),
+                ("ralpha", C.c_double), ("salpha", C.c_double),
+                ("rho_min_eos", C.c_double), ("sie_min_eos", C.c_double),
+                ("gam_max", C.c_double), ("sie_max", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("gm1", C.c_double), ("cv", C.c_double), ("mhd", C.c_int),
+                ("recon", C.c_int), ("riemann", C.c_int), ("c2p_tol", C.c_double),
+                ("c2p_max_iter", C.c_int), ("c2p_floor_scale_fac", C.c_double),
+                ("fail_on_floors", C.c_int), ("fail_on_ceilings", C.c_int),
+                ("bounds", Bounds), ("geom", Geom)]
+
+
+class PointGeom(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta", C.c_double * 3),
+                ("gcov", (C.c_double * 4) * 4), ("gcon", (C.c_double * 3) * 3),
+                ("gam", (C.c_double * 3) * 3), ("detgamma", C.c_double), ("detg", C.c_double)]
+
+    def np(self, name):
+        return np.array(getattr(self, name)[:] if name == "beta" else
+                        [list(r) for r in getattr(self, name)])
+
+
+class Block(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nx", C.c_int * 3), ("ng", C.c_int),
+                ("xmin", C.c_double * 3), ("dx", C.c_double * 3)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        L = C.CDLL(LIBPATH)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.orc_ratio.restype = C.c_double
+        L.orc_face_flux.restype = C.c_double
+        L.orc_c2p.restype = C.c_double
+        L.orc_c2p_residual.restype = C.c_double
+        L.orc_estimate_dt.restype = C.c_double
+        L.orc_regula_falsi.restype = C.c_double
+        L.orc_regula_falsi.argtypes = [C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p),
+                                       C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_int, P(C.c_int), P(C.c_int)]
+        for name in ("orc_plm",):
+            getattr(L, name).argtypes = [C.c_double] * 3 + [dp, dp]
+        for name in ("orc_weno5z", "orc_mp5"):
+            getattr(L, name).argtypes = [C.c_double] * 5 + [dp, dp]
+        L.orc_geom_point.argtypes = [P(Geom), C.c_double, C.c_double, C.c_double, P(PointGeom)]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    """Pointer to the data of a contiguous numpy array (float64 or int32)."""
+    assert a.flags.c_contiguous
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(C.POINTER(C.c_int))
+    raise TypeError(a.dtype)
+
+
+def default_params(gamma=5.0 / 3.0, mhd=True, recon=RECON_WENO5, riemann=RIEMANN_HLL,
+                   geom=GEOM_MINKOWSKI, a=0.0, tol=1e-10, max_iter=100):
+    """Floors and ceilings disabled (fixup.hpp default Floors()/Ceilings(); EOS minima 0)."""
+    b = Bounds(FLOOR_CONSTANT, -DBL_MAX, -DBL_MAX, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX)
+    return Params(gamma - 1.0, 1.0, int(mhd), recon, riemann, tol, max_iter, 1.0, 0, 0, b,
+                  Geom(geom, a))
+
+
+def nprim(mhd):
+    return 10 if mhd else 7
+
+
+def ncons(mhd):
+    return 8 if mhd else 5
+
+
+def point_geom(params, X):
+    g = PointGeom()
+    lib().orc_geom_point(C.byref(params.geom), X[0], X[1], X[2], C.byref(g))
+    return g
+
+
+def recon(kind, q):
+    ql, qr = C.c_double(), C.c_double()
+    L = lib()
+    if kind == RECON_LINEAR:
+        L.orc_plm(q[1], q[2], q[3], C.byref(ql), C.byref(qr))
+    elif kind == RECON_WENO5:
+        L.orc_weno5z(*[float(x) for x in q], C.byref(ql), C.byref(qr))
+    elif kind == RECON_MP5:
+        L.orc_mp5(*[float(x) for x in q], C.byref(ql), C.byref(qr))
+    else:
+        return float(q[2]), float(q[2])
+    return ql.value, qr.value
+
+
+def face_flux(params, d, X, ql, qr):
+    f = np.zeros(8)
+    Xa = (C.c_double * 3)(*X)
+    s = lib().orc_face_flux(C.byref(params), d, Xa, ptr(np.ascontiguousarray(ql, np.float64)),
+                            ptr(np.ascontiguousarray(qr, np.float64)), ptr(f))
+    return f[:ncons(params.mhd)], s
+
+
+def p2c(params, X, prim):
+    g = point_geom(params, X)
+    cons = np.zeros(8)
+    sig = np.zeros(3)
+    lib().orc_p2c(C.byref(params), C.byref(g), ptr(np.ascontiguousarray(prim, np.float64)),
+                  ptr(cons), ptr(sig))
+    return cons[:ncons(params.mhd)], sig
+
+
+def c2p(params, X, cons, mu_guess=0.0):
+    g = point_geom(params, X)
+    cs = np.zeros(8)
+    cs[:len(cons)] = cons
+    prim = np.zeros(10)
+    t = C.c_double(0.0)
+    mu = C.c_double(mu_guess)
+    sig = np.zeros(3)
+    it = C.c_int(0)
+    Xa = (C.c_double * 3)(*X)
+    st = lib().orc_c2p(C.byref(params), C.byref(g), Xa, ptr(cs), ptr(prim), C.byref(t),
+                       C.byref(mu), ptr(sig), C.byref(it))
+    return dict(fail=st, prim=prim[:nprim(params.mhd)], cons=cs[:ncons(params.mhd)],
+                temp=t.value, mu=mu.value, sig=sig, iters=it.value)
+
+
+def make_block(ndim, nx, ng=4, xmin=(0.0, 0.0, 0.0), dx=(1.0, 1.0, 1.0)):
+    nxx = list(nx) + [1] * (3 - len(nx))
+    return Block(ndim, (C.c_int * 3)(*nxx), ng, (C.c_double * 3)(*xmin), (C.c_double * 3)(*dx))
+
+
+def block_shape(b):
+    """(nk, nj, ni) including ghosts."""
+    n = [b.nx[d] + (2 * b.ng if d < b.ndim else 0) for d in range(3)]
+    return (n[2], n[1], n[0])
+
+
+def interior(b):
+    """slices (k, j, i) of the interior cells."""
+    sl = []
+    for d in (2, 1, 0):
+        s = b.ng if d < b.ndim else 0
+        sl.append(slice(s, s + b.nx[d]))
+    return tuple(sl)
+
+
+def calculate_fluxes(params, b, prim):
+    sh = block_shape(b)
+    nc = ncons(params.mhd)
+    flux = np.zeros((b.ndim, nc) + sh)
+    fsig = np.zeros((b.ndim,) + sh)
+    lib().orc_calculate_fluxes(C.byref(params), C.byref(b), ptr(np.ascontiguousarray(prim)),
+                               ptr(flux), ptr(fsig))
+    return flux, fsig
+
+
+def flux_ct(params, b, flux):
+    emf = np.zeros((3,) + block_shape(b))
+    lib().orc_flux_ct(C.byref(params), C.byref(b), ptr(flux), ptr(emf))
+    return emf
+
+
+def flux_divergence_update(params, b, flux, src, u_stage, u_base, beta, dt):
+    u_out = u_stage.copy()
+    lib().orc_flux_divergence_update(
+        C.byref(params), C.byref(b), ptr(flux), ptr(src) if src is not None else None,
+        ptr(u_stage), ptr(u_base), C.c_double(beta), C.c_double(dt), ptr(u_out))
+    return u_out
+
+
+def con2prim_block(params, b, cons, prim, mu, lo=None, hi=None):
+    sh = block_shape(b)
+    if lo is None:
+        lo = (0, 0, 0)
+        hi = (sh[2] - 1, sh[1] - 1, sh[0] - 1)
+    temp = np.zeros(sh)
+    sig = np.zeros((b.ndim,) + sh)
+    fail = np.zeros(sh)
+    iters = np.zeros(sh, np.int32)
+    lib().orc_con2prim_block(C.byref(params), C.byref(b), (C.c_int * 3)(*lo),
+                             (C.c_int * 3)(*hi), ptr(cons), ptr(prim), ptr(temp), ptr(mu),
+                             ptr(sig), ptr(fail), ptr(iters))
+    return dict(temp=temp, sig=sig, fail=fail, iters=iters)
+
+
+def prim_to_con_block(params, b, prim, lo=None, hi=None):
+    sh = block_shape(b)
+    if lo is None:
+        lo = (0, 0, 0)
+        hi = (sh[2] - 1, sh[1] - 1, sh[0] - 1)
+    cons = np.zeros((ncons(params.mhd),) + sh)
+    sig = np.zeros((b.ndim,) + sh)
+    lib().orc_prim_to_con_block(C.byref(params), C.byref(b), (C.c_int * 3)(*lo),
+                                (C.c_int * 3)(*hi), ptr(np.ascontiguousarray(prim)),
+                                ptr(cons), ptr(sig))
+    return cons, sig
+
+
+def source_terms_block(params, b, prim):
+    src = np.zeros((ncons(params.mhd),) + block_shape(b))
+    lib().orc_source_terms_block(C.byref(params), C.byref(b), ptr(np.ascontiguousarray(prim)),
+                                 ptr(src))
+    return src
+
+
+def estimate_dt(params, b, cell_sig, face_sig, cfl):
+    return lib().orc_estimate_dt(C.byref(params), C.byref(b), ptr(cell_sig),
+                                 ptr(face_sig) if face_sig is not None else None,
+                                 C.c_double(cfl))
+
+
+def c2p_batch(params, coords, cons, mu=None):
+    n = coords.shape[1]
+    cons = np.ascontiguousarray(cons).copy()
+    prim = np.zeros((nprim(params.mhd), n))
+    mu = np.zeros(n) if mu is None else np.ascontiguousarray(mu, np.float64).copy()
+    fail = np.zeros(n)
+    iters = np.zeros(n, np.int32)
+    lib().orc_c2p_batch(C.byref(params), C.c_long(n), ptr(np.ascontiguousarray(coords)),
+                        ptr(cons), ptr(prim), ptr(mu), ptr(fail), ptr(iters))
+    return dict(cons=cons, prim=prim, mu=mu, fail=fail, iters=iters)
+
+
+def p2c_batch(params, coords, prim):
+    n = coords.shape[1]
+    cons = np.zeros((ncons(params.mhd), n))
+    lib().orc_p2c_batch(C.byref(params), C.c_long(n), ptr(np.ascontiguousarray(coords)),
+                        ptr(np.ascontiguousarray(prim)), ptr(cons))
+    return cons
+
+
+def tmunu(params, X, prim):
+    T = np.zeros((4, 4))
+    lib().orc_tmunu(C.byref(params), C.c_double(X[0]), C.c_double(X[1]), C.c_double(X[2]),
+                    ptr(np.ascontiguousarray(prim, np.float64)), ptr(T))
+    return T
+
+
+def sources(params, X, prim):
+    s = np.zeros(5)
+    lib().orc_sources(C.byref(params), C.c_double(X[0]), C.c_double(X[1]), C.c_double(X[2]),
+                      ptr(np.ascontiguousarray(prim, np.float64)), ptr(s))
+    return s
+
+
+def geom_dg(params, X):
+    dg = np.zeros((4, 4, 4))
+    lib().orc_geom_dg(C.byref(params.geom), C.c_double(X[0]), C.c_double(X[1]),
+                      C.c_double(X[2]), ptr(dg))
+    return dg
+
+
+def geom_gcon4(params, X):
+    gi = np.zeros((4, 4))
+    lib().orc_geom_gcon4(C.byref(params.geom), C.c_double(X[0]), C.c_double(X[1]),
+                         C.c_double(X[2]), ptr(gi))
+    return gi
+
+
+def geom_gradlnalpha(params, X):
+    da = np.zeros(4)
+    lib().orc_geom_gradlnalpha(C.byref(params.geom), C.c_double(X[0]), C.c_double(X[1]),
+                               C.c_double(X[2]), ptr(da))
+    return da
