@@ -1,0 +1,188 @@
+/*
+ * phx.h -- C ABI of the MI355X-native phoebus fluid-stage library (libphx.so).
+ *
+ * This is the drop-in boundary for phoebus's per-RK-stage fluid path.  Each entry point
+ * replaces one task of the reference (file:line cited at each declaration).  The ABI
+ * contains plain pointers and sizes only.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers, caller-owned, FP64, structure-of-arrays
+ *    with the cell index i fastest: a field with nv components over a block is laid out
+ *    as [nv][nk][nj][ni], where (ni, nj, nk) include ng ghost layers in every active
+ *    dimension (ng = 4 for WENO5/MP5, as in the reference).
+ *  - Primitive ordering is the reference's reconstruction pack (fluid.cpp:272-286):
+ *      MHD   : rho, v1, v2, v3, u, B1, B2, B3, P, gamma1     (nprim = 10)
+ *      hydro : rho, v1, v2, v3, u, P, gamma1                 (nprim = 7)
+ *    where v^i is the phoebus primitive velocity (W times the normal-observer 3-velocity)
+ *    and u = rho * eps.  Conserved ordering: D, S1, S2, S3, tau, [B1, B2, B3]
+ *    (ncons = 8 or 5), densitized by sqrt(gamma); tau is the Valencia energy.
+ *  - Face fluxes: flux[d][m][k][j][i] is the flux through the LOWER face of cell (k,j,i)
+ *    in direction d, as in the reference (riemann.hpp:296, fluid.cpp:753-770).
+ *  - Every call is asynchronous on the given stream (NULL = the context's stream) and
+ *    returns PHX_OK or a negative error code.  No call aborts or throws; per-zone
+ *    failures are data (the fail field, 1.0 = success, 0.0 = failure, as
+ *    con2prim_robust.hpp:41-44).
+ *  - No entry point allocates device memory after phx_create; scratch lives in the
+ *    context.  A context is bound to one device and is not thread-safe.
+ */
+#ifndef PHX_H_
+#define PHX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phx_ctx phx_ctx;
+typedef void *phx_stream; /* hipStream_t */
+
+enum {
+  PHX_OK = 0,
+  PHX_ERR_INVALID = -1,     /* bad argument or configuration (reference: PARTHENON_THROW) */
+  PHX_ERR_UNSUPPORTED = -2, /* configuration the library does not implement */
+  PHX_ERR_DEVICE = -3,      /* HIP runtime error */
+  PHX_ERR_NOMEM = -4
+};
+
+enum { PHX_GEOM_MINKOWSKI = 0, PHX_GEOM_SPHERICAL_KS = 1 };
+enum { PHX_RECON_CONSTANT = 0, PHX_RECON_LINEAR = 1, PHX_RECON_WENO5Z = 2, PHX_RECON_MP5 = 3 };
+enum { PHX_RIEMANN_LLF = 0, PHX_RIEMANN_HLL = 1 };
+enum {
+  PHX_FLOOR_CONSTANT_RHO_SIE = 0,
+  PHX_FLOOR_EXPX1_RHO_SIE = 1,
+  PHX_FLOOR_EXPX1_RHO_U = 2,
+  PHX_FLOOR_X1_RHO_SIE = 3,
+  PHX_FLOOR_R_RHO_SIE = 4
+};
+
+/* Analytic metric selection (reference: compile-time Geometry::CoordSysMeshBlock,
+ * cmake/Geometry.cmake:17-41).  Minkowski uses Cartesian X; spherical Kerr-Schild uses
+ * X = (r, theta, phi) and spin a (spherical_kerr_schild.hpp:42-243). */
+typedef struct {
+  int kind;
+  double a;
+} phx_geometry;
+
+/* fixup::Bounds (fixup.hpp:72-327): floors of floor_type with (rho0, sie0, rho_exp,
+ * sie_exp), clamped below by the EOS minima; constant gamma / sie ceilings.  Disabled
+ * floors are rho0 = sie0 = -DBL_MAX; disabled ceilings are DBL_MAX. */
+typedef struct {
+  int floor_type;
+  double rho0, sie0, rho_exp, sie_exp;
+  double rho_min_eos, sie_min_eos;
+  double gam_max, sie_max;
+} phx_bounds;
+
+/* The fluid package parameters the path reads (fluid.cpp:44-136) plus the ideal-gas EOS
+ * (eos_phoebus.cpp:73-95).  Ye is not supported (PHX_ERR_UNSUPPORTED). */
+typedef struct {
+  double gamma, cv;
+  int mhd;
+  int recon, riemann;
+  double c2p_tol;
+  int c2p_max_iter;
+  double c2p_floor_scale_fac;
+  int c2p_fail_on_floors, c2p_fail_on_ceilings;
+  phx_bounds bounds;
+  phx_geometry geom;
+} phx_fluid_params;
+
+/* One meshblock: interior cells nx[3] (nx = 1 in inactive dimensions), ng ghost layers
+ * in active dimensions, uniform logical coordinates Xf_d(idx) = xmin_d + (idx - is_d)
+ * * dx_d with is_d = ng (active) or 0 (inactive), as parthenon's uniform coordinates. */
+typedef struct {
+  int ndim, nx[3], ng;
+  double xmin[3], dx[3];
+} phx_block;
+
+/* Device fields of one block (any may be NULL where the call does not use it). */
+typedef struct {
+  double *prim;     /* [nprim][cells]                                   */
+  double *cons;     /* [ncons][cells]                                   */
+  double *temp;     /* [cells] temperature                              */
+  double *c2p_mu;   /* [cells] con2prim root (warm start, fluid.cpp:224) */
+  double *fail;     /* [cells] 1.0 success / 0.0 failure                */
+  double *cell_sig; /* [ndim][cells] cell signal speeds                  */
+  double *face_sig; /* [ndim][cells] face signal speeds                  */
+  double *flux;     /* [ndim][ncons][cells]                              */
+  double *emf;      /* [3][cells] corner EMFs (MHD, ndim > 1)            */
+  double *src;      /* [ncons][cells] geometric sources (S_i, tau slots) */
+} phx_block_fields;
+
+/* ---- context ---------------------------------------------------------------------- */
+int phx_create(phx_ctx **ctx, int device);
+int phx_destroy(phx_ctx *ctx);
+const char *phx_last_error(const phx_ctx *ctx);
+
+/* ---- per-block task entry points --------------------------------------------------- */
+
+/* fluid::PrimitiveToConservedRegion (fluid.cpp:359-449) over cells lo..hi (inclusive,
+ * (i, j, k) order): cons and cell_sig from prim. */
+int phx_prim_to_con(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                    const phx_block_fields *f, const int lo[3], const int hi[3],
+                    phx_stream stream);
+
+/* fluid::CalculateFluxes (fluid.cpp:659-773): reconstruction (ReconLoop,
+ * reconstruction.hpp:246-271) fused with the HLL/LLF flux (riemann.hpp:257-301).  Writes
+ * f->flux and f->face_sig on faces k in [ks-dk, ke+dk], j in [js-dj, je+dj],
+ * i in [is-1, ie+1] for every direction (the reference's ranges).  The face states are
+ * never materialized (no ql/qr arrays). */
+int phx_calculate_fluxes(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                         const phx_block_fields *f, phx_stream stream);
+
+/* fluid::FluxCT (fluid.cpp:775-841): corner EMFs into f->emf, then B fluxes.  No-op for
+ * hydro or 1-D. */
+int phx_flux_ct(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                const phx_block_fields *f, phx_stream stream);
+
+/* fluid::CalculateFluidSourceTerms (fluid.cpp:566-657): geometric sources into f->src
+ * (S_i and tau slots, interior cells).  Identically zero for Minkowski. */
+int phx_fluid_source_terms(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                           const phx_block_fields *f, phx_stream stream);
+
+/* parthenon Update::FluxDivergence + SumData(src) + AverageIndependentData(beta) +
+ * UpdateIndependentData(beta*dt) as sequenced at phoebus_driver.cpp:560-576, interior
+ * cells: u_stage <- beta*u_stage + (1-beta)*u_base; u_out <- u_stage + beta*dt*dU/dt.
+ * src may be NULL.  u_out may alias u_base (the rk2 second stage does). */
+int phx_flux_divergence_update(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                               const double *flux, const double *src, double *u_stage,
+                               const double *u_base, double beta, double dt, double *u_out,
+                               phx_stream stream);
+
+/* fluid::ConservedToPrimitiveRobust (fluid.cpp:478-517) with con2prim_robust::ConToPrim
+ * (con2prim_robust.hpp:224-453) over cells lo..hi.  Reads f->cons and f->c2p_mu (guess);
+ * writes f->prim, f->temp, f->c2p_mu, f->cell_sig, f->fail and, for zones that hit a
+ * bound, f->cons.  iters (nullable, int32 per cell) receives RootFind::iteration_count;
+ * an unbracketed root (the reference aborts there, root_find.hpp:115-121) is reported
+ * as a failure with iters = -1. */
+int phx_con2prim(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                 const phx_block_fields *f, const int lo[3], const int hi[3], int32_t *iters,
+                 phx_stream stream);
+
+/* fluid::EstimateTimestepBlock (fluid.cpp:899-944): dt_out[0] (device) = cfl * min over
+ * interior cells.  use_face_speeds = 0 selects the reference's first-call form
+ * (cell speeds only, :930-939). */
+int phx_estimate_dt(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                    const phx_block_fields *f, int use_face_speeds, double cfl,
+                    double *dt_out, phx_stream stream);
+
+/* ---- batched con2prim (independent states) ---------------------------------------
+ * The per-zone solve of con2prim_robust.hpp:299-453 over n states at spherical
+ * Kerr-Schild points coords = [r[n], theta[n]] (phi = 0), or Minkowski when
+ * p->geom.kind == PHX_GEOM_MINKOWSKI (coords may then be NULL).  cons [ncons][n] (in,
+ * rewritten where a bound was hit), prim [nprim][n] out, mu [n] in (guess) / out,
+ * fail [n] out, iters [n] out (nullable). */
+int phx_con2prim_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
+                       const double *coords, double *cons, double *prim, double *mu,
+                       double *fail, int32_t *iters, phx_stream stream);
+
+/* prim2con::p2c (prim2con.hpp:49-102) over the same batched layout. */
+int phx_prim_to_con_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
+                          const double *coords, const double *prim, double *cons,
+                          phx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHX_H_ */

@@ -1,0 +1,110 @@
+"""ctypes binding of libphx.so (include/phx.h, include/phx_runtime.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading fails loudly
+when it has not been built; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libphx.so")
+
+PHX_OK, PHX_ERR_INVALID, PHX_ERR_UNSUPPORTED, PHX_ERR_DEVICE, PHX_ERR_NOMEM = 0, -1, -2, -3, -4
+GEOM_MINKOWSKI, GEOM_SPHERICAL_KS = 0, 1
+RECON_CONSTANT, RECON_LINEAR, RECON_WENO5Z, RECON_MP5 = 0, 1, 2, 3
+RIEMANN_LLF, RIEMANN_HLL = 0, 1
+(FLOOR_CONSTANT_RHO_SIE, FLOOR_EXPX1_RHO_SIE, FLOOR_EXPX1_RHO_U, FLOOR_X1_RHO_SIE,
+ FLOOR_R_RHO_SIE) = range(5)
+
+DBL_MAX = 1.7976931348623157e308
+
+
+class Geometry(C.Structure):
+    _fields_ = [("kind", C.c_int), ("a", C.c_double)]
+
+
+class Bounds(C.Structure):
+    _fields_ = [("floor_type", C.c_int), ("rho0", C.c_double), ("sie0", C.c_double),
+                ("rho_exp", C.c_double), ("sie_exp", C.c_double),
+                ("rho_min_eos", C.c_double), ("sie_min_eos", C.c_double),
+                ("gam_max", C.c_double), ("sie_max", C.c_double)]
+
+
+class FluidParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("cv", C.c_double), ("mhd", C.c_int),
+                ("recon", C.c_int), ("riemann", C.c_int), ("c2p_tol", C.c_double),
+                ("c2p_max_iter", C.c_int), ("c2p_floor_scale_fac", C.c_double),
+                ("c2p_fail_on_floors", C.c_int), ("c2p_fail_on_ceilings", C.c_int),
+                ("bounds", Bounds), ("geom", Geometry)]
+
+
+class Block(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nx", C.c_int * 3), ("ng", C.c_int),
+                ("xmin", C.c_double * 3), ("dx", C.c_double * 3)]
+
+
+class BlockFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("prim", "cons", "temp", "c2p_mu", "fail", "cell_sig",
+                                           "face_sig", "flux", "emf", "src")]
+
+
+_lib = None
+
+
+class PhxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libphx.so once.  Raises if it is missing: the HIP path has no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise PhxError(f"{LIBPATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIBPATH)
+    vp, i32, dbl = C.c_void_p, C.c_int, C.c_double
+    P = C.POINTER
+    sig = {
+        "phx_create": (i32, [P(vp), i32]),
+        "phx_destroy": (i32, [vp]),
+        "phx_last_error": (C.c_char_p, [vp]),
+        "phx_prim_to_con": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(i32), P(i32), vp]),
+        "phx_calculate_fluxes": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), vp]),
+        "phx_flux_ct": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), vp]),
+        "phx_fluid_source_terms": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), vp]),
+        "phx_flux_divergence_update": (i32, [vp, P(FluidParams), P(Block), vp, vp, vp, vp, dbl,
+                                             dbl, vp, vp]),
+        "phx_con2prim": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(i32), P(i32), vp, vp]),
+        "phx_estimate_dt": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), i32, dbl, vp, vp]),
+        "phx_con2prim_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
+        "phx_prim_to_con_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp]),
+        "phx_rt_device_count": (i32, [P(i32)]),
+        "phx_rt_set_device": (i32, [i32]),
+        "phx_rt_malloc": (i32, [P(vp), C.c_size_t]),
+        "phx_rt_free": (i32, [vp]),
+        "phx_rt_memcpy_htod": (i32, [vp, vp, C.c_size_t]),
+        "phx_rt_memcpy_dtoh": (i32, [vp, vp, C.c_size_t]),
+        "phx_rt_memcpy_dtod": (i32, [vp, vp, C.c_size_t]),
+        "phx_rt_memset": (i32, [vp, i32, C.c_size_t]),
+        "phx_rt_device_synchronize": (i32, []),
+        "phx_rt_stream_create": (i32, [P(vp)]),
+        "phx_rt_stream_destroy": (i32, [vp]),
+        "phx_rt_stream_synchronize": (i32, [vp]),
+        "phx_rt_event_create": (i32, [P(vp)]),
+        "phx_rt_event_destroy": (i32, [vp]),
+        "phx_rt_event_record": (i32, [vp, vp]),
+        "phx_rt_event_elapsed_ms": (i32, [vp, vp, P(C.c_float)]),
+        "phx_rt_device_info": (i32, [C.c_char_p, i32, P(i32), P(dbl)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names every header in include/ declares (checked by the CPU tests)."""
+    return sorted(n for n in dir(lib()) if n.startswith("phx_"))

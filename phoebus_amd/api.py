@@ -1,0 +1,160 @@
+"""Python view of the C ABI for tests and the benchmark.
+
+Each method forwards to one entry point of include/phx.h and raises PhxError with the
+library's message on a nonzero status.  Arrays are DeviceArray objects (device memory).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from .device import DeviceArray
+
+NPRIM = {True: 10, False: 7}
+NCONS = {True: 8, False: 5}
+
+
+def fluid_params(gamma=5.0 / 3.0, cv=1.0, mhd=True, recon=A.RECON_WENO5Z,
+                 riemann=A.RIEMANN_HLL, c2p_tol=1e-10, c2p_max_iter=100,
+                 geom=A.GEOM_MINKOWSKI, a=0.0, floor_scale_fac=1.0, fail_on_floors=False,
+                 fail_on_ceilings=False, floors=None, ceilings=None):
+    """FluidParams with floors and ceilings disabled unless given (fixup.hpp defaults)."""
+    b = A.Bounds(A.FLOOR_CONSTANT_RHO_SIE, -A.DBL_MAX, -A.DBL_MAX, 0.0, 0.0, 0.0, 0.0,
+                 A.DBL_MAX, A.DBL_MAX)
+    if floors:
+        for k, v in floors.items():
+            setattr(b, k, v)
+    if ceilings:
+        b.gam_max, b.sie_max = ceilings
+    return A.FluidParams(gamma, cv, int(mhd), recon, riemann, c2p_tol, c2p_max_iter,
+                         floor_scale_fac, int(fail_on_floors), int(fail_on_ceilings), b,
+                         A.Geometry(geom, a))
+
+
+def block(ndim, nx, ng=4, xmin=(0.0, 0.0, 0.0), dx=(1.0, 1.0, 1.0)):
+    nxx = list(nx) + [1] * (3 - len(nx))
+    return A.Block(ndim, (C.c_int * 3)(*nxx), ng, (C.c_double * 3)(*xmin),
+                   (C.c_double * 3)(*dx))
+
+
+def block_shape(b):
+    """(nk, nj, ni) including ghost zones."""
+    n = [b.nx[d] + (2 * b.ng if d < b.ndim else 0) for d in range(3)]
+    return (n[2], n[1], n[0])
+
+
+def _p(d):
+    return None if d is None else d.ptr
+
+
+class Phx:
+    """One library context (phx_ctx) on one device."""
+
+    def __init__(self, device=0):
+        self.L = A.lib()
+        h = C.c_void_p()
+        rc = self.L.phx_create(C.byref(h), device)
+        if rc != A.PHX_OK:
+            raise A.PhxError(f"phx_create failed ({rc}): no usable HIP device")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            self.L.phx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != A.PHX_OK:
+            msg = self.L.phx_last_error(self.h)
+            raise A.PhxError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+        return rc
+
+    @staticmethod
+    def fields(**kw):
+        f = A.BlockFields()
+        for k, v in kw.items():
+            setattr(f, k, _p(v))
+        return f
+
+    # ---- per-block tasks -------------------------------------------------------------------
+    def prim_to_con(self, p, b, f, lo=None, hi=None, stream=None):
+        lo_ = (C.c_int * 3)(*lo) if lo is not None else None
+        hi_ = (C.c_int * 3)(*hi) if hi is not None else None
+        return self._chk(self.L.phx_prim_to_con(self.h, C.byref(p), C.byref(b), C.byref(f),
+                                                lo_, hi_, stream), "phx_prim_to_con")
+
+    def calculate_fluxes(self, p, b, f, stream=None):
+        return self._chk(self.L.phx_calculate_fluxes(self.h, C.byref(p), C.byref(b), C.byref(f),
+                                                     stream), "phx_calculate_fluxes")
+
+    def flux_ct(self, p, b, f, stream=None):
+        return self._chk(self.L.phx_flux_ct(self.h, C.byref(p), C.byref(b), C.byref(f), stream),
+                         "phx_flux_ct")
+
+    def fluid_source_terms(self, p, b, f, stream=None):
+        return self._chk(self.L.phx_fluid_source_terms(self.h, C.byref(p), C.byref(b),
+                                                       C.byref(f), stream),
+                         "phx_fluid_source_terms")
+
+    def flux_divergence_update(self, p, b, flux, src, u_stage, u_base, beta, dt, u_out,
+                               stream=None):
+        return self._chk(self.L.phx_flux_divergence_update(
+            self.h, C.byref(p), C.byref(b), _p(flux), _p(src), _p(u_stage), _p(u_base),
+            beta, dt, _p(u_out), stream), "phx_flux_divergence_update")
+
+    def con2prim(self, p, b, f, lo=None, hi=None, iters=None, stream=None):
+        lo_ = (C.c_int * 3)(*lo) if lo is not None else None
+        hi_ = (C.c_int * 3)(*hi) if hi is not None else None
+        return self._chk(self.L.phx_con2prim(self.h, C.byref(p), C.byref(b), C.byref(f), lo_,
+                                             hi_, _p(iters), stream), "phx_con2prim")
+
+    def estimate_dt(self, p, b, f, use_face_speeds, cfl, dt_out, stream=None):
+        return self._chk(self.L.phx_estimate_dt(self.h, C.byref(p), C.byref(b), C.byref(f),
+                                                int(use_face_speeds), cfl, _p(dt_out), stream),
+                         "phx_estimate_dt")
+
+    # ---- batched con2prim ----------------------------------------------------------------
+    def con2prim_batch(self, p, n, coords, cons, prim, mu, fail, iters=None, stream=None):
+        return self._chk(self.L.phx_con2prim_batch(self.h, C.byref(p), n, _p(coords), _p(cons),
+                                                   _p(prim), _p(mu), _p(fail), _p(iters),
+                                                   stream), "phx_con2prim_batch")
+
+    def prim_to_con_batch(self, p, n, coords, prim, cons, stream=None):
+        return self._chk(self.L.phx_prim_to_con_batch(self.h, C.byref(p), n, _p(coords),
+                                                      _p(prim), _p(cons), stream),
+                         "phx_prim_to_con_batch")
+
+
+class BlockState:
+    """Device fields of one meshblock (the MeshBlockData analogue for tests and bench)."""
+
+    def __init__(self, p, b):
+        self.p, self.b = p, b
+        mhd = bool(p.mhd)
+        sh = block_shape(b)
+        nd = b.ndim
+        self.shape = sh
+        self.prim = DeviceArray((NPRIM[mhd],) + sh)
+        self.cons = DeviceArray((NCONS[mhd],) + sh)
+        self.temp = DeviceArray(sh)
+        self.c2p_mu = DeviceArray(sh)
+        self.fail = DeviceArray(sh)
+        self.cell_sig = DeviceArray((nd,) + sh)
+        self.face_sig = DeviceArray((nd,) + sh)
+        self.flux = DeviceArray((nd, NCONS[mhd]) + sh)
+        self.emf = DeviceArray((3,) + sh)
+        self.src = DeviceArray((NCONS[mhd],) + sh)
+        for a in (self.prim, self.cons, self.temp, self.c2p_mu, self.fail, self.cell_sig,
+                  self.face_sig, self.flux, self.emf, self.src):
+            a.zero()
+
+    def fields(self, with_src=False):
+        return Phx.fields(prim=self.prim, cons=self.cons, temp=self.temp, c2p_mu=self.c2p_mu,
+                          fail=self.fail, cell_sig=self.cell_sig, face_sig=self.face_sig,
+                          flux=self.flux, emf=self.emf, src=self.src if with_src else None)

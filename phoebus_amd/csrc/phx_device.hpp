@@ -1,0 +1,764 @@
+// phx_device.hpp -- device-side physics of the fluid stage for gfx950 (HIP, FP64).
+//
+// Point functions used by the kernels in phx_kernels.hip.  Each follows the reference
+// function cited beside it in operation order (the library is built with
+// -ffp-contract=off so that +,-,*,/,sqrt round exactly as the reference's would), but
+// is written for the GPU: metrics are compile-time policies whose structural zeros are
+// removed at compile time, nothing is heap- or pack-indexed, and every per-zone state
+// lives in registers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "phx.h"
+
+namespace phx {
+
+// ---- parameters as the kernels see them ------------------------------------------------
+struct DevBounds {
+  int floor_type;
+  double r0, s0, ralpha, salpha, rho_min_eos, sie_min_eos, gam_max, sie_max;
+};
+
+struct DevParams {
+  double gm1, cv;          // ideal gas: Gamma - 1 (computed on the host), C_v
+  double tol;              // c2p bracket half-width tolerance
+  double fsf;              // c2p_floor_scale_fac
+  int max_iter;
+  int fail_on_floors, fail_on_ceilings;
+  double a;                // Kerr-Schild spin
+  DevBounds bnd;
+};
+
+// ---- robust.hpp:10-40 -------------------------------------------------------------------
+constexpr double kSmall = 10.0 * DBL_MIN;
+constexpr double kEps = 10.0 * DBL_EPSILON;
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double ratio(double a, double b) {
+  const double sgn = b >= 0 ? 1.0 : -1.0;
+  return a / (b + sgn * kSmall);
+}
+__device__ __forceinline__ double make_positive(double v) { return smax(v, kEps); }
+__device__ __forceinline__ double make_bounded(double v, double lo, double hi) {
+  return smin(smax(v, lo + kEps), hi * (1.0 - kEps));
+}
+
+// ---- reconstruction.hpp:101-244 -------------------------------------------------------
+__device__ __forceinline__ double mc_limiter(double dm, double dp, double alpha) {
+  const double dc = (dm * dp > 0.0) * 0.5 * (dm + dp);
+  return copysign(smin(fabs(dc), alpha * smin(fabs(dm), fabs(dp))), dc);
+}
+
+__device__ __forceinline__ void recon_plm(double qm, double q0, double qp, double &ql,
+                                          double &qr) {
+  double dq = qp - q0;
+  dq = mc_limiter(q0 - qm, dq, 1.5);
+  ql = q0 + 0.5 * dq;
+  qr = q0 - 0.5 * dq;
+}
+
+// One WENO-Z edge: weights from smoothness indicators (b_near, b_mid, b_far), candidate
+// stencils evaluated from the far side toward the centre.
+__device__ __forceinline__ void recon_weno5z(double q0, double q1, double q2, double q3,
+                                             double q4, double &ql, double &qr) {
+  constexpr double c00 = 1.0 / 3.0, c01 = -7.0 / 6.0, c02 = 11.0 / 6.0;
+  constexpr double c10 = -1.0 / 6.0, c11 = 5.0 / 6.0, c12 = 1.0 / 3.0;
+  constexpr double c20 = 1.0 / 3.0, c21 = 5.0 / 6.0, c22 = -1.0 / 6.0;
+  constexpr double g0 = 0.1, g1 = 0.6, g2 = 0.3;
+  constexpr double eps = 1e-100;
+  constexpr double tt = 13.0 / 3.0;
+  double a = q0 - 2 * q1 + q2;
+  double b = q0 - 4.0 * q1 + 3.0 * q2;
+  double beta0 = tt * a * a + b * b + eps;
+  a = q1 - 2.0 * q2 + q3;
+  b = q3 - q1;
+  double beta1 = tt * a * a + b * b + eps;
+  a = q2 - 2.0 * q3 + q4;
+  b = q4 - 4.0 * q3 + 3.0 * q2;
+  double beta2 = tt * a * a + b * b + eps;
+  const double tau5 = fabs(beta2 - beta0);
+  beta0 = (beta0 + tau5) / beta0;
+  beta1 = (beta1 + tau5) / beta1;
+  beta2 = (beta2 + tau5) / beta2;
+
+  double w0 = g0 * beta0 + eps, w1 = g1 * beta1 + eps, w2 = g2 * beta2 + eps;
+  double wsum = 1.0 / (w0 + w1 + w2);
+  double l = w0 * (c00 * q0 + c01 * q1 + c02 * q2);
+  l += w1 * (c10 * q1 + c11 * q2 + c12 * q3);
+  l += w2 * (c20 * q2 + c21 * q3 + c22 * q4);
+  l *= wsum;
+  const double al = 3.0 * wsum * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
+
+  w0 = g0 * beta2 + eps;
+  w1 = g1 * beta1 + eps;
+  w2 = g2 * beta0 + eps;
+  wsum = 1.0 / (w0 + w1 + w2);
+  double r = w0 * (c00 * q4 + c01 * q3 + c02 * q2);
+  r += w1 * (c10 * q3 + c11 * q2 + c12 * q1);
+  r += w2 * (c20 * q2 + c21 * q1 + c22 * q0);
+  r *= wsum;
+  const double ar = 3.0 * wsum * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
+
+  double dq = q3 - q2;
+  dq = mc_limiter(q2 - q1, dq, 2.0);
+  const double alin = 2.0 * al * ar / (al + ar);
+  ql = alin * l + (1.0 - alin) * (q2 + 0.5 * dq);
+  qr = alin * r + (1.0 - alin) * (q2 - 0.5 * dq);
+}
+
+__device__ __forceinline__ double minmod2(double a, double b) {
+  return a * b > 0.0 ? (fabs(a) < fabs(b) ? a : b) : 0.0;
+}
+
+__device__ __forceinline__ double mp5_edge(double fm2, double fm1, double f0, double fp1,
+                                           double fp2) {
+  constexpr double alpha = 4.0, epsm = 1.e-12;
+  double f = 2.0 * fm2 - 13.0 * fm1 + 47.0 * f0 + 27.0 * fp1 - 3.0 * fp2;
+  f /= 60.0;
+  const double fmp = f0 + minmod2(fp1 - f0, alpha * (f0 - fm1));
+  if ((f - f0) * (f - fmp) <= epsm) return f;
+  const double d2m = fm2 + f0 - 2.0 * fm1;
+  const double d2 = fm1 + fp1 - 2.0 * f0;
+  const double d2p = f0 + fp2 - 2.0 * fp1;
+  const double dmmp = minmod2(minmod2(4.0 * d2 - d2p, 4.0 * d2p - d2), minmod2(d2, d2p));
+  const double dmmm = minmod2(minmod2(4.0 * d2m - d2, 4.0 * d2 - d2m), minmod2(d2, d2m));
+  const double ful = f0 + alpha * (f0 - fm1);
+  const double fav = 0.5 * (f0 + fp1);
+  const double fmd = fav - 0.5 * dmmp;
+  const double flc = 0.5 * (3.0 * f0 - fm1) + 4.0 / 3.0 * dmmm;
+  const double fmin_ = fmax(fmin(fmin(f0, fp1), fmd), fmin(fmin(f0, ful), flc));
+  const double fmax_ = fmin(fmax(fmax(f0, fp1), fmd), fmax(fmax(f0, ful), flc));
+  return f + minmod2(fmin_ - f, fmax_ - f);
+}
+
+// upper edge (ql) and lower edge (qr) of the centre cell of a 5-point stencil
+template <int RECON>
+__device__ __forceinline__ void reconstruct(double q0, double q1, double q2, double q3,
+                                            double q4, double &ql, double &qr) {
+  if constexpr (RECON == PHX_RECON_CONSTANT) {
+    ql = q2;
+    qr = q2;
+  } else if constexpr (RECON == PHX_RECON_LINEAR) {
+    recon_plm(q1, q2, q3, ql, qr);
+  } else if constexpr (RECON == PHX_RECON_WENO5Z) {
+    recon_weno5z(q0, q1, q2, q3, q4, ql, qr);
+  } else {
+    ql = mp5_edge(q0, q1, q2, q3, q4);
+    qr = mp5_edge(q4, q3, q2, q1, q0);
+  }
+}
+
+// edges of the centre cell of q[0 .. 2H] (H = 0, 1, 2 for constant, linear, WENO5/MP5)
+template <int RECON>
+__device__ __forceinline__ void reconstruct_edges(const double *q, double &ql, double &qr) {
+  if constexpr (RECON == PHX_RECON_CONSTANT) {
+    ql = q[0];
+    qr = q[0];
+  } else if constexpr (RECON == PHX_RECON_LINEAR) {
+    recon_plm(q[0], q[1], q[2], ql, qr);
+  } else {
+    reconstruct<RECON>(q[0], q[1], q[2], q[3], q[4], ql, qr);
+  }
+}
+
+// ---- metrics -----------------------------------------------------------------------------
+// A metric policy evaluates alpha, beta^i, g_{mu nu}, gamma^{ij}, sqrt(gamma), sqrt(-g) at
+// one point.  nz_cov / nz_con / nz_beta mark entries that can be nonzero; loops below
+// skip the others at compile time.
+
+struct MinkowskiMetric {  // minkowski.hpp:35-115
+  static constexpr __host__ __device__ bool nz_cov(int m, int n) { return m == n; }
+  static constexpr __host__ __device__ bool nz_con(int m, int n) { return m == n; }
+  static constexpr __host__ __device__ bool nz_beta(int) { return false; }
+  __device__ MinkowskiMetric(double, double, double, double) {}
+  __device__ double alpha() const { return 1.0; }
+  __device__ double beta(int) const { return 0.0; }
+  __device__ double gcov(int m, int n) const { return m == n ? (m == 0 ? -1.0 : 1.0) : 0.0; }
+  __device__ double gcon(int m, int n) const { return m == n ? 1.0 : 0.0; }
+  __device__ double detgamma() const { return 1.0; }
+  __device__ double detg() const { return 1.0; }
+};
+
+struct SphericalKSMetric {  // spherical_kerr_schild.hpp:42-243, X = (r, theta, phi)
+  static constexpr __host__ __device__ bool nz_cov(int m, int n) {
+    return (m == n) || (m + n == 1) || (m + n == 3 && m != 1 && m != 2) ||
+           (m + n == 4 && m != 2);
+  }
+  static constexpr __host__ __device__ bool nz_con(int m, int n) {
+    return (m == n) || (m + n == 2 && m != 1);
+  }
+  static constexpr __host__ __device__ bool nz_beta(int i) { return i == 0; }
+  double alpha_, beta0_, g00_, g01_, g03_, g11_, g13_, g22_, g33_;
+  double c00_, c02_, c11_, c22_, detgamma_, detg_;
+  __device__ SphericalKSMetric(double a, double X1, double X2, double) {
+    const double a2 = a * a;
+    const double r = fabs(X1), cth = cos(X2), sth = sin(X2);
+    const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
+    alpha_ = sqrt(ratio(rho2, 2 * r + rho2));
+    beta0_ = ratio(2 * r, 2 * r + rho2);
+    g01_ = ratio(2 * r, rho2);
+    g00_ = -1 + g01_;
+    g03_ = -ratio(2 * a * r * sth2, rho2);
+    g11_ = 1 + g01_;
+    g13_ = -a * (1 + g01_) * sth2;
+    g22_ = rho2;
+    g33_ = (rho2 + a2 * (1 + g01_) * sth2) * sth2;
+    c00_ = ratio(rho2, 2 * r + rho2) + ratio(a2 * sth2, rho2);
+    c02_ = ratio(a, rho2);
+    c11_ = ratio(1, rho2);
+    c22_ = ratio(1, rho2 * sth2);
+    detgamma_ = sqrt(fabs(rho2 * (2 * r + rho2) * sth2));
+    detg_ = sqrt(fabs(rho2 * rho2 * sth2));
+  }
+  __device__ double alpha() const { return alpha_; }
+  __device__ double beta(int i) const { return i == 0 ? beta0_ : 0.0; }
+  __device__ double gcov(int m, int n) const {
+    const int lo = m < n ? m : n, hi = m < n ? n : m;
+    if (lo == 0) return hi == 0 ? g00_ : hi == 1 ? g01_ : hi == 3 ? g03_ : 0.0;
+    if (lo == 1) return hi == 1 ? g11_ : hi == 3 ? g13_ : 0.0;
+    if (lo == 2) return hi == 2 ? g22_ : 0.0;
+    return g33_;
+  }
+  __device__ double gcon(int m, int n) const {
+    const int lo = m < n ? m : n, hi = m < n ? n : m;
+    if (lo == 0) return hi == 0 ? c00_ : hi == 2 ? c02_ : 0.0;
+    if (lo == 1) return hi == 1 ? c11_ : 0.0;
+    return hi == 2 ? c22_ : 0.0;
+  }
+  __device__ double detgamma() const { return detgamma_; }
+  __device__ double detg() const { return detg_; }
+};
+
+template <int GEOM>
+struct MetricOf;
+template <>
+struct MetricOf<PHX_GEOM_MINKOWSKI> { using type = MinkowskiMetric; };
+template <>
+struct MetricOf<PHX_GEOM_SPHERICAL_KS> { using type = SphericalKSMetric; };
+
+// ---- bounds: fixup.hpp:91-134, 188-213 ------------------------------------------------
+__device__ __forceinline__ void get_floors(const DevBounds &b, double x1, double x2,
+                                           double x3, double &rflr, double &sflr) {
+  switch (b.floor_type) {
+  case PHX_FLOOR_CONSTANT_RHO_SIE:
+    rflr = b.r0;
+    sflr = b.s0;
+    break;
+  case PHX_FLOOR_EXPX1_RHO_SIE:
+    rflr = b.r0 * exp(b.ralpha * x1);
+    sflr = b.s0 * exp(b.salpha * x1);
+    break;
+  case PHX_FLOOR_EXPX1_RHO_U: {
+    const double scratch = b.r0 * exp(b.ralpha * x1);
+    sflr = b.s0 * exp(b.salpha * x1) / smax(rflr, scratch);
+    rflr = scratch;
+  } break;
+  case PHX_FLOOR_X1_RHO_SIE:
+    rflr = b.r0 * smin(1.0, pow(x1, b.ralpha));
+    sflr = b.s0 * smin(1.0, pow(x1, b.salpha));
+    break;
+  default: {
+    const double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+    rflr = b.r0 * smin(1.0, pow(r, b.ralpha));
+    sflr = b.s0 * smin(1.0, pow(r, b.salpha));
+  } break;
+  }
+  rflr = smax(rflr, b.rho_min_eos);
+  sflr = smax(sflr, b.sie_min_eos);
+}
+
+// ---- ideal-gas EOS (singularity-eos IdealGas closed forms, unit scales 1) -------------
+__device__ __forceinline__ double eos_p_rho_sie(const DevParams &p, double rho, double sie) {
+  return p.gm1 * rho * sie;
+}
+__device__ __forceinline__ double eos_t_rho_sie(const DevParams &p, double sie) {
+  return sie / p.cv;
+}
+__device__ __forceinline__ double eos_p_rho_t(const DevParams &p, double rho, double t) {
+  return p.gm1 * rho * p.cv * t;
+}
+__device__ __forceinline__ double eos_bmod_rho_t(const DevParams &p, double rho, double t) {
+  return (p.gm1 + 1.0) * p.gm1 * rho * p.cv * t;
+}
+
+template <bool MHD>
+struct Layout {
+  static constexpr int nprim = MHD ? 10 : 7;
+  static constexpr int ncons = MHD ? 8 : 5;
+  static constexpr int prs = MHD ? 8 : 5;
+  static constexpr int gm1 = MHD ? 9 : 6;
+};
+
+// ---- riemann.hpp:88-209 FluxState::prim_to_flux (Valencia energy) ---------------------
+template <bool MHD, class M>
+__device__ __forceinline__ void prim_to_flux(const DevParams &p, const int dir, const M &g,
+                                             const double X[3], const double *q,
+                                             double &vm, double &vp, double *U, double *F) {
+  using L = Layout<MHD>;
+  double rho_floor = q[0], sie_floor;
+  get_floors(p.bnd, X[0], X[1], X[2], rho_floor, sie_floor);
+  const double gam_max = p.bnd.gam_max, sie_max = p.bnd.sie_max;
+  const double rho = smax(q[0], rho_floor);
+  double vpcon[3] = {q[1], q[2], q[3]};
+  double vsq0 = 0.;
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+      if (M::nz_cov(ii + 1, jj + 1)) vsq0 += g.gcov(ii + 1, jj + 1) * vpcon[ii] * vpcon[jj];
+  double W = sqrt(1. + vsq0);
+  if (W > gam_max) {
+    const double rescale = sqrt((gam_max * gam_max - 1.) / (W * W - 1.));
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) vpcon[ii] *= rescale;
+  }
+  double vcon[3] = {vpcon[0] / W, vpcon[1] / W, vpcon[2] / W};
+  const double vel = vcon[dir];
+  double Bcon[3] = {0.0, 0.0, 0.0};
+  const double u = (q[4] / rho > sie_floor ? (q[4] / rho > sie_max ? rho * sie_max : q[4])
+                                            : rho * sie_floor);
+  const double P = smax(q[L::prs], 0.0);
+  const double gamma1 = q[L::gm1];
+  if constexpr (MHD) {
+    Bcon[0] = q[5];
+    Bcon[1] = q[6];
+    Bcon[2] = q[7];
+  }
+  double BdotB = 0.0, Bdotv = 0.0, vsq = 0.0;
+#pragma unroll
+  for (int m = 0; m < 3; m++)
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      if (!M::nz_cov(m + 1, n + 1)) continue;
+      vsq += g.gcov(m + 1, n + 1) * vcon[m] * vcon[n];
+      if constexpr (MHD) {
+        Bdotv += g.gcov(m + 1, n + 1) * Bcon[m] * vcon[n];
+        BdotB += g.gcov(m + 1, n + 1) * Bcon[m] * Bcon[n];
+      }
+    }
+  const double vsq_max = 1.0 - 1.0 / (gam_max * gam_max);
+  if (vsq > vsq_max) {
+    double scale = vsq_max / vsq;
+    vsq *= scale;
+    scale = sqrt(scale);
+    Bdotv *= scale;
+#pragma unroll
+    for (int m = 0; m < 3; m++) vcon[m] *= scale;
+  }
+  W = 1.0 / sqrt(1 - vsq);
+  const double ialpha_beta_dir = M::nz_beta(0) || M::nz_beta(1) || M::nz_beta(2)
+                                     ? g.beta(dir) / g.alpha() : 0.0;
+  const double vtil = vel - ialpha_beta_dir;
+
+  U[0] = rho * W;
+  F[0] = U[0] * vtil;
+  const double b0 = W * Bdotv;
+  const double bsq = (BdotB + b0 * b0) / (W * W);
+  const double rhohWsq = (rho + u + P) * W * W;
+  const double bsqWsq = bsq * W * W;
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    double bcovm = 0.0;
+    if (MHD && M::nz_cov(m + 1, 0)) bcovm = g.gcov(m + 1, 0) * b0 / g.alpha();
+    double vcovm = 0.0;
+#pragma unroll
+    for (int n = 1; n < 4; n++) {
+      if (!M::nz_cov(m + 1, n)) continue;
+      if constexpr (MHD) {
+        const double sh = M::nz_beta(n - 1) ? g.beta(n - 1) / g.alpha() : 0.0;
+        bcovm += g.gcov(m + 1, n) * (Bcon[n - 1] / W + b0 * (vcon[n - 1] - sh));
+      }
+      vcovm += g.gcov(m + 1, n) * vcon[n - 1];
+    }
+    U[1 + m] = (rhohWsq + bsqWsq) * vcovm - b0 * bcovm;
+    F[1 + m] = U[1 + m] * vtil + (P + 0.5 * bsq) * (double)(dir == m) - bcovm * Bcon[dir] / W;
+  }
+  U[4] = rhohWsq + bsqWsq - (P + 0.5 * bsq) - b0 * b0 - U[0];
+  F[4] = U[4] * vtil + (P + 0.5 * bsq) * vel - Bdotv * Bcon[dir];
+  if constexpr (MHD) {
+#pragma unroll
+    for (int m = 5; m <= 7; m++) {
+      const double sh = M::nz_beta(m - 5) ? g.beta(m - 5) / g.alpha() : 0.0;
+      U[m] = Bcon[m - 5];
+      F[m] = U[m] * vtil - Bcon[dir] * (vcon[m - 5] - sh);
+    }
+  }
+  const double vasq = bsqWsq / make_positive(rhohWsq + bsqWsq);
+  const double cssq = make_bounded(gamma1 * P * W * W / make_positive(rhohWsq), 0.0, 1.0);
+  const double cmsq = make_bounded(cssq + vasq * (1.0 - cssq), 0.0, 1.0);
+  const double vcoff = g.alpha() / (1.0 - vsq * cmsq);
+  const double v0 = vel * (1.0 - cmsq);
+  const double vpm =
+      sqrt(make_positive(cmsq * (1.0 - vsq) * (g.gcon(dir, dir) * (1.0 - vsq * cmsq) - vel * v0)));
+  const double bd = M::nz_beta(dir) ? g.beta(dir) : 0.0;
+  vp = vcoff * (v0 + vpm) - bd;
+  vm = vcoff * (v0 - vpm) - bd;
+}
+
+// riemann.hpp:257-301.  Writes ncons fluxes (times sqrt(-g) / sqrt(gamma) as the reference)
+// and returns the face signal speed.
+template <bool MHD, int RIEMANN, class M>
+__device__ __forceinline__ double riemann_flux(const DevParams &p, int dir, const M &g,
+                                               const double X[3], const double *ql,
+                                               const double *qr, double *flux) {
+  constexpr int NC = Layout<MHD>::ncons;
+  double Ul[NC], Ur[NC], Fl[NC], Fr[NC], vml, vpl, vmr, vpr;
+  prim_to_flux<MHD>(p, dir, g, X, ql, vml, vpl, Ul, Fl);
+  prim_to_flux<MHD>(p, dir, g, X, qr, vmr, vpr, Ur, Fr);
+  if constexpr (RIEMANN == PHX_RIEMANN_LLF) {
+    const double cmax = smax(smax(-vml, vpl), smax(-vmr, vpr));
+#pragma unroll
+    for (int m = 0; m < NC; m++)
+      flux[m] = 0.5 * ((Fl[m] + Fr[m]) * g.detg() - cmax * ((Ur[m] - Ul[m]) * g.detgamma()));
+    return cmax;
+  } else {
+    const double cl = smin(smin(vml, vmr), 0.0);
+    const double cr = smax(smax(vpl, vpr), 0.0);
+    const double crcl = cr * cl;
+    const double inv = ratio(1.0, cr - cl);
+#pragma unroll
+    for (int m = 0; m < NC; m++)
+      flux[m] = ((cr * Fl[m] - cl * Fr[m]) * g.detg() + crcl * (Ur[m] - Ul[m]) * g.detgamma()) * inv;
+    return smax(-cl, cr);
+  }
+}
+
+// ---- prim2con.hpp:25-102 ---------------------------------------------------------------
+template <class M>
+__device__ __forceinline__ void signal_speeds(double rho, double u, double pr, double bsq,
+                                              const double v[3], double vsq, double gam1,
+                                              const M &g, double sig[3]) {
+  const double rho_rel = rho + u + pr;
+  const double vasq = bsq / (rho_rel + bsq);
+  double cssq = gam1 * pr / rho_rel;
+  cssq += vasq - cssq * vasq;
+  const double vcoff = g.alpha() / (1.0 - vsq * cssq);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double vpm = sqrt(cssq * (1.0 - vsq) *
+                            (g.gcon(m, m) * (1.0 - vsq * cssq) - v[m] * v[m] * (1.0 - cssq)));
+    const double bm = M::nz_beta(m) ? g.beta(m) : 0.0;
+    const double vp = vcoff * (v[m] * (1.0 - cssq) + vpm) - bm;
+    const double vmn = vcoff * (v[m] * (1.0 - cssq) - vpm) - bm;
+    sig[m] = smax(fabs(vp), fabs(vmn));
+  }
+}
+
+template <bool MHD, class M>
+__device__ __forceinline__ void p2c(const double rho, const double vp[3], const double b[3],
+                                    const double u, const double pr, const double gam1,
+                                    const M &g, double &D, double S[3], double bcons[3],
+                                    double &tau, double sig[3]) {
+  double vsq = 0.0, Bsq = 0.0, Bdotv = 0.0, vsq0 = 0.0;
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+      if (M::nz_cov(ii + 1, jj + 1)) vsq0 += g.gcov(ii + 1, jj + 1) * vp[ii] * vp[jj];
+  const double W = sqrt(1. + vsq0);
+  const double iW = 1.0 / W;
+  const double v[3] = {vp[0] * iW, vp[1] * iW, vp[2] * iW};
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      if (!M::nz_cov(ii + 1, jj + 1)) continue;
+      vsq += g.gcov(ii + 1, jj + 1) * v[ii] * v[jj];
+      if constexpr (MHD) {
+        Bsq += g.gcov(ii + 1, jj + 1) * b[ii] * b[jj];
+        Bdotv += g.gcov(ii + 1, jj + 1) * b[ii] * v[jj];
+      }
+    }
+  const double b0 = W * Bdotv;
+  double bcon[4] = {0.0, 0.0, 0.0, 0.0};
+  double bsq = 0.0;
+  double bcov[3] = {0.0, 0.0, 0.0};
+  if constexpr (MHD) {
+    bcon[0] = W * Bdotv / g.alpha();
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double sh = M::nz_beta(m) ? g.beta(m) / g.alpha() : 0.0;
+      bcon[m + 1] = b[m] * iW + Bdotv * W * (v[m] - sh);
+    }
+    bsq = Bsq * iW * iW + Bdotv * Bdotv;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+        if (M::nz_cov(m + 1, n)) bcov[m] += g.gcov(m + 1, n) * bcon[n];
+  }
+  D = g.detgamma() * rho * W;
+  const double rho_rel = (rho + u + pr + bsq) * W * W;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double vcov = 0.0;
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+      if (M::nz_cov(m + 1, n + 1)) vcov += g.gcov(m + 1, n + 1) * v[n];
+    S[m] = g.detgamma() * (rho_rel * vcov - b0 * bcov[m]);
+    bcons[m] = g.detgamma() * b[m];
+  }
+  tau = g.detgamma() * (rho_rel - (pr + 0.5 * bsq) - b0 * b0) - D;
+  signal_speeds(rho, u, pr, bsq, v, vsq, gam1, g, sig);
+}
+
+// ---- con2prim_robust.hpp:46-180 Residual ----------------------------------------------
+struct Residual {
+  const DevParams *p;
+  double D, q, bsq, bsq_rpsq, rsq, rbsq, v0sq;
+  double x1, x2, x3;
+  double rho_floor, e_floor, e_max;
+  bool used_density_floor, used_energy_floor, used_energy_max, used_gamma_max;
+
+  __device__ __forceinline__ double x_mu(double mu) const { return ratio(1.0, 1.0 + mu * bsq); }
+  __device__ __forceinline__ double rbarsq_mu(double mu, double x) const {
+    return x * (x * rsq + mu * (1.0 + x) * rbsq);
+  }
+  __device__ __forceinline__ double qbar_mu(double mu, double x) const {
+    const double mux = mu * x;
+    return q - 0.5 * (bsq + mux * mux * bsq_rpsq);
+  }
+  __device__ __forceinline__ double vhatsq_mu(double mu, double rbarsq) {
+    const double vsq_trial = mu * mu * rbarsq;
+    used_gamma_max = vsq_trial > v0sq;
+    return used_gamma_max ? v0sq : vsq_trial;
+  }
+  __device__ __forceinline__ double rhohat_mu(double iWhat) {
+    const double rho_trial = D * iWhat;
+    used_density_floor = rho_trial <= rho_floor;
+    return used_density_floor ? rho_floor : rho_trial;
+  }
+  __device__ __forceinline__ double ehat_mu(double mu, double qbar, double rbarsq,
+                                            double vhatsq, double What) {
+    double rho = rhohat_mu(ratio(1.0, What));
+    get_floors(p->bnd, x1, x2, x3, rho, e_floor);
+    e_floor *= p->fsf;
+    const double ehat_trial =
+        What * (qbar - mu * rbarsq) + vhatsq * What * ratio(What, 1.0 + What);
+    used_energy_floor = false;
+    used_energy_max = false;
+    if (ehat_trial <= e_floor) {
+      used_energy_floor = true;
+      return e_floor;
+    } else if (ehat_trial > e_max) {
+      used_energy_max = true;
+      return e_max;
+    }
+    return ehat_trial;
+  }
+  __device__ __forceinline__ double operator()(double mu) {  // :121-141
+    const double x = x_mu(mu);
+    const double rbarsq = rbarsq_mu(mu, x);
+    const double qbar = qbar_mu(mu, x);
+    const double vhatsq = vhatsq_mu(mu, rbarsq);
+    const double iWhat = sqrt(1.0 - vhatsq);
+    const double What = ratio(1.0, iWhat);
+    const double rhohat = rhohat_mu(iWhat);
+    const double ehat = ehat_mu(mu, qbar, rbarsq, vhatsq, What);
+    const double Phat = eos_p_rho_sie(*p, rhohat, ehat);
+    const double hhat = rhohat * (1.0 + ehat) + Phat;
+    const double ahat = Phat / make_positive(hhat - Phat);
+    const double nua = (1.0 + ahat) * (1.0 + ehat) * iWhat;
+    const double nub = (1.0 + ahat) * (1.0 + qbar - mu * rbarsq);
+    const double nuhat = smax(nua, nub);
+    const double muhat = 1.0 / (nuhat + mu * rbarsq);
+    return mu - muhat;
+  }
+};
+
+// root_find.hpp:41-58 (refine_bracket) and :109-167 (regula_falsi, Illinois variant).
+// Returns false when the root is not bracketed (the reference aborts there).
+template <class F>
+__device__ __forceinline__ bool regula_falsi(F &f, double a, double b, double tol,
+                                             double guess, int max_iter, double &root,
+                                             int &iters) {
+  double ya, yb;
+  int it = 0;
+  if (a <= guess && b >= guess) {
+    ya = f(guess);
+    yb = f(b);
+    if (ya * yb > 0.0) {
+      yb = ya;
+      b = guess;
+      ya = f(a);
+    } else {
+      a = guess;
+    }
+  } else {
+    ya = f(a);
+    yb = f(b);
+  }
+  if (ya * yb > 0.0) {
+    iters = 0;
+    root = guess;
+    return false;
+  }
+  const double sign = (ya < 0 ? 1.0 : -1.0);
+  ya *= sign;
+  yb *= sign;
+  int b1 = 0, b2 = 0;
+  while (b - a > 2.0 * tol && it < max_iter) {
+    const double c = (a * yb - b * ya) / (yb - ya);
+    if (c == a) {
+      b = a;
+      it++;
+      continue;
+    } else if (c == b) {
+      a = b;
+      it++;
+      continue;
+    }
+    const double yc = sign * f(c);
+    if (yc > 0.0) {
+      b = c;
+      yb = yc;
+      b1++;
+      ya *= (b1 > 1 ? 0.5 : 1.0);
+      b2 = 0;
+    } else if (yc < 0.0) {
+      a = c;
+      ya = yc;
+      b2++;
+      yb *= (b2 > 1 ? 0.5 : 1.0);
+      b1 = 0;
+    } else {
+      a = c;
+      b = c;
+    }
+    it++;
+  }
+  iters = it;
+  root = 0.5 * (a + b);
+  return true;
+}
+
+// con2prim_robust.hpp:299-453 for one zone.  cons/prim in registers; returns the fail
+// flag value (1.0 success, 0.0 failure).  iters = -1 marks an unbracketed root.
+template <bool MHD, class M>
+__device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
+                                                const double X[3], double *cons,
+                                                double *prim, double &temp, double &mu_io,
+                                                double sig[3], int &iters) {
+  using L = Layout<MHD>;
+  iters = 0;
+  if (isnan(cons[0]) + isnan(cons[1]) + isnan(cons[4]) > 0) return 0.0;
+  const double igdet = 1.0 / g.detgamma();
+  double rhoflr = 0.0, epsflr;
+  get_floors(p.bnd, X[0], X[1], X[2], rhoflr, epsflr);
+  rhoflr *= p.fsf;
+  const double gam_max = p.bnd.gam_max;
+  const double D = cons[0] * igdet;
+  const double tau = cons[4] * igdet;
+  const double q = ratio(tau, D);
+  double bsq = 0.0, bsq_rpsq = 0.0, rbsq = 0.0, rsq = 0.0, bdotr = 0.0;
+  double rcov[3], rcon[3], bu[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) rcov[i] = ratio(cons[1 + i], cons[0]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    rcon[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (M::nz_con(i, j)) rcon[i] += g.gcon(i, j) * rcov[j];
+    rsq += rcon[i] * rcov[i];
+  }
+  if constexpr (MHD) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) prim[5 + i] = cons[5 + i] * igdet;
+    const double sD = 1.0 / sqrt(smax(D, rhoflr));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      bu[i] = prim[5 + i] * sD;
+      bdotr += bu[i] * rcov[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (M::nz_cov(i + 1, j + 1)) bsq += g.gcov(i + 1, j + 1) * bu[i] * bu[j];
+    bsq = smax(0.0, bsq);
+    rbsq = bdotr * bdotr;
+    bsq_rpsq = bsq * rsq - rbsq;
+  }
+  const double zsq = rsq / 1.0;
+  const double v0sq = smin(zsq / (1.0 + zsq), 1.0 - 1.0 / (gam_max * gam_max));
+
+  Residual res;
+  res.p = &p;
+  res.D = D; res.q = q; res.bsq = bsq; res.bsq_rpsq = bsq_rpsq; res.rsq = rsq;
+  res.rbsq = rbsq; res.v0sq = v0sq; res.x1 = X[0]; res.x2 = X[1]; res.x3 = X[2];
+  {
+    double garbage = 0.0;
+    res.rho_floor = 0.0;
+    get_floors(p.bnd, X[0], X[1], X[2], res.rho_floor, garbage);
+    res.e_max = p.bnd.sie_max;
+    res.rho_floor *= p.fsf;
+  }
+  res.e_floor = 0.0;
+  res.used_density_floor = res.used_energy_floor = res.used_energy_max = res.used_gamma_max = false;
+
+  double mu;
+  if (!regula_falsi(res, 0.0, 1.0, p.tol, mu_io, p.max_iter, mu, iters)) {
+    iters = -1;
+    return 0.0;
+  }
+  mu_io = mu;
+
+  const double x = res.x_mu(mu);
+  const double rbarsq = res.rbarsq_mu(mu, x);
+  const double vsq = res.vhatsq_mu(mu, rbarsq);
+  const double iW = sqrt(1.0 - vsq);
+  prim[0] = res.rhohat_mu(iW);
+  const double qbar = res.qbar_mu(mu, x);
+  const double W = 1.0 / iW;
+  prim[4] = res.ehat_mu(mu, qbar, rbarsq, vsq, W);
+  temp = eos_t_rho_sie(p, prim[4]);
+  prim[4] *= prim[0];
+  prim[L::prs] = eos_p_rho_t(p, prim[0], temp);
+  prim[L::gm1] = eos_bmod_rho_t(p, prim[0], temp) / prim[L::prs];
+  double vel[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vel[i] = W * mu * x * (rcon[i] + mu * bdotr * bu[i]);
+    prim[1 + i] = vel[i];
+  }
+  if constexpr (MHD) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bu[i] = prim[5 + i];
+  }
+  if (res.used_density_floor || res.used_energy_floor || res.used_energy_max ||
+      res.used_gamma_max) {
+    double S[3], bcons[3];
+    p2c<MHD>(prim[0], vel, bu, prim[4], prim[L::prs], prim[L::gm1], g, cons[0], S, bcons,
+             cons[4], sig);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) cons[1 + i] = S[i];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) vel[m] *= iW;
+    double bsq2 = 0.0;
+    if constexpr (MHD) {
+      double bdotv = 0.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          if (!M::nz_cov(m + 1, n + 1)) continue;
+          bsq2 += bu[m] * bu[n] * g.gcov(m + 1, n + 1);
+          bdotv += bu[m] * vel[n] * g.gcov(m + 1, n + 1);
+        }
+      bsq2 = iW * iW * bsq2 + bdotv * bdotv;
+    }
+    signal_speeds(prim[0], prim[4], prim[L::prs], bsq2, vel, vsq, prim[L::gm1], g, sig);
+  }
+  if (isnan(cons[0]) + isnan(cons[1]) + isnan(cons[4]) > 0) return 0.0;
+  if (p.fail_on_floors && (res.used_density_floor || res.used_energy_floor)) return 0.0;
+  if (p.fail_on_ceilings && res.used_gamma_max) return 0.0;
+  return 1.0;
+}
+
+}  // namespace phx

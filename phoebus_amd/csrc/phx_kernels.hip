@@ -1,0 +1,903 @@
+// phx_kernels.hip -- HIP kernels of the fluid stage for gfx950 and the C ABI (include/phx.h).
+//
+// Zone-per-lane kernels over SoA fields with the cell index i fastest, so that a
+// wavefront's 64 lanes read 64 consecutive doubles of each variable.  Reconstruction is
+// fused into the Riemann kernel (face states never reach HBM); the CFL reduction is a
+// wavefront shuffle reduction followed by one small finishing kernel.  All FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "phx.h"
+#include "phx_device.hpp"
+#include "phx_internal.hpp"
+
+namespace phx {
+
+// ---- block geometry as the kernels see it ---------------------------------------------
+struct BlockGeom {
+  int ni, nj, nk;
+  int ndim, ng;
+  int is[3], ie[3];  // interior index range per dimension (i, j, k)
+  long ncell;
+  double xmin[3], dx[3];
+  int istart[3];     // parthenon's istart: ng in active dimensions, 0 otherwise
+};
+
+static BlockGeom make_block_geom(const phx_block *b) {
+  BlockGeom g;
+  g.ndim = b->ndim;
+  g.ng = b->ng;
+  int n[3];
+  for (int d = 0; d < 3; ++d) {
+    const bool act = d < b->ndim;
+    n[d] = b->nx[d] + (act ? 2 * b->ng : 0);
+    g.istart[d] = act ? b->ng : 0;
+    g.is[d] = g.istart[d];
+    g.ie[d] = g.istart[d] + b->nx[d] - 1;
+    g.xmin[d] = b->xmin[d];
+    g.dx[d] = b->dx[d];
+  }
+  g.ni = n[0];
+  g.nj = n[1];
+  g.nk = n[2];
+  g.ncell = (long)n[0] * n[1] * n[2];
+  return g;
+}
+
+__device__ __forceinline__ double xc(const BlockGeom &b, int d, int idx) {
+  return b.xmin[d] + (idx - b.istart[d] + 0.5) * b.dx[d];
+}
+__device__ __forceinline__ double xf(const BlockGeom &b, int d, int idx) {
+  return b.xmin[d] + (idx - b.istart[d]) * b.dx[d];
+}
+
+// ---- fused reconstruction + Riemann flux (fluid.cpp:676-770) ---------------------------
+// One lane per face.  Face (k,j,i) of direction DIR: left state = upper edge of the cell
+// below, right state = lower edge of cell (k,j,i); six cells of each primitive along DIR.
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_fluxes(DevParams p, BlockGeom b,
+                                                const double *__restrict__ prim,
+                                                double *__restrict__ flux,
+                                                double *__restrict__ fsig) {
+  using M = typename MetricOf<GEOM>::type;
+  constexpr int NP = Layout<MHD>::nprim, NC = Layout<MHD>::ncons;
+  constexpr int H = RECON == PHX_RECON_CONSTANT ? 0 : RECON == PHX_RECON_LINEAR ? 1 : 2;
+  const int dj = b.ndim > 1, dk = b.ndim > 2;
+  const int fi = b.ie[0] - b.is[0] + 3;
+  const int fj = b.ie[1] - b.is[1] + 1 + 2 * dj;
+  const int fk = b.ie[2] - b.is[2] + 1 + 2 * dk;
+  const long nface = (long)fi * fj * fk;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nface) return;
+  const int i = (int)(t % fi) + b.is[0] - 1;
+  const long r = t / fi;
+  const int j = (int)(r % fj) + b.is[1] - dj;
+  const int k = (int)(r / fj) + b.is[2] - dk;
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long st = DIR == 0 ? 1 : DIR == 1 ? (long)b.ni : (long)b.ni * b.nj;
+  const int own = DIR == 0 ? i : DIR == 1 ? j : k;
+  // the lowest face's left state is never consumed (the reference reads unwritten ql
+  // there); use the right state when its stencil would leave the block
+  const bool left_ok = own - 1 - H >= 0;
+
+  // cells c-(H+1)*st .. c+H*st: the left edge state uses the first 2H+1 of them, the
+  // right edge state the last 2H+1; nothing outside the block is read
+  double ql[NP], qr[NP];
+#pragma unroll
+  for (int v = 0; v < NP; ++v) {
+    const double *pv = prim + v * b.ncell + c;
+    double q[2 * H + 2];
+#pragma unroll
+    for (int o = 1; o < 2 * H + 2; ++o) q[o] = pv[(o - H - 1) * st];
+    q[0] = left_ok ? pv[-(H + 1) * st] : q[1];
+    double lo, hi;
+    reconstruct_edges<RECON>(q + 1, lo, hi);
+    qr[v] = hi;  // lower edge of cell (k,j,i)
+    reconstruct_edges<RECON>(q, lo, hi);
+    ql[v] = left_ok ? lo : qr[v];  // upper edge of the cell below
+  }
+  double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+  X[DIR] = xf(b, DIR, own);
+  const M g(p.a, X[0], X[1], X[2]);
+  double f[NC];
+  const double s = riemann_flux<MHD, RIEMANN>(p, DIR, g, X, ql, qr, f);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) flux[((long)DIR * NC + m) * b.ncell + c] = f[m];
+  fsig[(long)DIR * b.ncell + c] = s;
+}
+
+// ---- FluxCT (fluid.cpp:775-841) --------------------------------------------------------
+template <int NC>
+__global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, const double *__restrict__ flux,
+                                                   double *__restrict__ emf) {
+  const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 2;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni) + b.is[0];
+  const int j = (int)((t / ni) % nj) + b.is[1];
+  const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long sj = b.ni, sk = (long)b.ni * b.nj, n = b.ncell;
+  auto F = [&](int d, int comp, long cc) { return flux[((long)d * NC + 5 + comp) * n + cc]; };
+  emf[c] = 0.25 * (F(1, 2, c) + F(1, 2, c - sk) - F(2, 1, c) - F(2, 1, c - sj));
+  emf[n + c] = -0.25 * (F(0, 2, c) + F(0, 2, c - sk) - F(2, 0, c) - F(2, 0, c - 1));
+  emf[2 * n + c] = 0.25 * (F(0, 1, c) + F(0, 1, c - sj) - F(1, 0, c) - F(1, 0, c - 1));
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_ct_flux_3d(BlockGeom b, double *__restrict__ flux,
+                                                    const double *__restrict__ emf) {
+  const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 2;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni) + b.is[0];
+  const int j = (int)((t / ni) % nj) + b.is[1];
+  const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long sj = b.ni, sk = (long)b.ni * b.nj, n = b.ncell;
+  const double *e0 = emf, *e1 = emf + n, *e2 = emf + 2 * n;
+  auto F = [&](int d, int comp) -> double & { return flux[((long)d * NC + 5 + comp) * n + c]; };
+  F(0, 0) = 0.0;
+  F(0, 1) = 0.5 * (e2[c] + e2[c + sj]);
+  F(0, 2) = -0.5 * (e1[c] + e1[c + sk]);
+  F(1, 0) = -0.5 * (e2[c] + e2[c + 1]);
+  F(1, 1) = 0.0;
+  F(1, 2) = 0.5 * (e0[c] + e0[c + sk]);
+  F(2, 0) = 0.5 * (e1[c] + e1[c + 1]);
+  F(2, 1) = -0.5 * (e0[c] + e0[c + sj]);
+  F(2, 2) = 0.0;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_ct_emf_2d(BlockGeom b, const double *__restrict__ flux,
+                                                   double *__restrict__ emf) {
+  const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni) + b.is[0];
+  const int j = (int)((t / ni) % nj) + b.is[1];
+  const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell, sj = b.ni;
+  auto F = [&](int d, int comp, long cc) { return flux[((long)d * NC + 5 + comp) * n + cc]; };
+  emf[c] = 0.25 * (F(0, 1, c) + F(0, 1, c - sj) - F(1, 0, c) - F(1, 0, c - 1));
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_ct_flux_2d(BlockGeom b, double *__restrict__ flux,
+                                                    const double *__restrict__ emf) {
+  const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni) + b.is[0];
+  const int j = (int)((t / ni) % nj) + b.is[1];
+  const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell, sj = b.ni;
+  flux[((long)0 * NC + 5) * n + c] = 0.0;
+  flux[((long)0 * NC + 6) * n + c] = 0.5 * (emf[c] + emf[c + sj]);
+  flux[((long)1 * NC + 5) * n + c] = -0.5 * (emf[c] + emf[c + 1]);
+  flux[((long)1 * NC + 6) * n + c] = 0.0;
+}
+
+// ---- flux divergence + sources + RK average/update (phoebus_driver.cpp:560-576) -------
+template <int NC>
+__global__ __launch_bounds__(256) void k_update(BlockGeom b, const double *__restrict__ flux,
+                                                const double *__restrict__ src,
+                                                double *__restrict__ u_stage,
+                                                const double *u_base, double beta, double bdt,
+                                                double *u_out) {
+  const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni) + b.is[0];
+  const int j = (int)((t / ni) % nj) + b.is[1];
+  const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell;
+  const double a1 = b.dx[1] * b.dx[2], a2 = b.dx[0] * b.dx[2], a3 = b.dx[0] * b.dx[1];
+  const double vol = b.dx[0] * b.dx[1] * b.dx[2];
+#pragma unroll
+  for (int l = 0; l < NC; ++l) {
+    const double *F1 = flux + (long)l * n;
+    double du = a1 * F1[c + 1] - a1 * F1[c];
+    if (b.ndim >= 2) {
+      const double *F2 = flux + ((long)NC + l) * n;
+      du += a2 * F2[c + b.ni] - a2 * F2[c];
+    }
+    if (b.ndim == 3) {
+      const double *F3 = flux + ((long)2 * NC + l) * n;
+      du += a3 * F3[c + (long)b.ni * b.nj] - a3 * F3[c];
+    }
+    double dudt = -du / vol;
+    if (src != nullptr && l >= 1 && l <= 4) dudt = dudt + src[(long)l * n + c];
+    const double us = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
+    u_stage[(long)l * n + c] = us;
+    u_out[(long)l * n + c] = us + bdt * dudt;
+  }
+}
+
+// ---- con2prim over a cell range (fluid.cpp:478-517) ------------------------------------
+struct Range3 {
+  int lo[3], n[3];
+};
+
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_con2prim(DevParams p, BlockGeom b, Range3 rg,
+                                                  double *__restrict__ cons,
+                                                  double *__restrict__ prim,
+                                                  double *__restrict__ temp,
+                                                  double *__restrict__ mu,
+                                                  double *__restrict__ csig,
+                                                  double *__restrict__ fail,
+                                                  int32_t *__restrict__ iters) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)rg.n[0] * rg.n[1] * rg.n[2]) return;
+  const int i = (int)(t % rg.n[0]) + rg.lo[0];
+  const int j = (int)((t / rg.n[0]) % rg.n[1]) + rg.lo[1];
+  const int k = (int)(t / ((long)rg.n[0] * rg.n[1])) + rg.lo[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell;
+  const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+  const M g(p.a, X[0], X[1], X[2]);
+  double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sig[d] = d < b.ndim ? csig[d * n + c] : 0.0;
+  // zones rejected up front by the NaN test keep every field (con2prim_robust.hpp:303-304)
+  if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
+    fail[c] = 0.0;
+    if (iters) iters[c] = 0;
+    return;
+  }
+  double tt = temp ? temp[c] : 0.0, m = mu[c];
+  int it = 0;
+  const double st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) cons[v * n + c] = cs[v];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) prim[v * n + c] = pr[v];
+  for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
+  if (temp) temp[c] = tt;
+  mu[c] = m;
+  fail[c] = st;
+  if (iters) iters[c] = it;
+}
+
+// ---- prim -> cons over a cell range (fluid.cpp:359-449) --------------------------------
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_prim_to_con(DevParams p, BlockGeom b, Range3 rg,
+                                                     const double *__restrict__ prim,
+                                                     double *__restrict__ cons,
+                                                     double *__restrict__ csig) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)rg.n[0] * rg.n[1] * rg.n[2]) return;
+  const int i = (int)(t % rg.n[0]) + rg.lo[0];
+  const int j = (int)((t / rg.n[0]) % rg.n[1]) + rg.lo[1];
+  const int k = (int)(t / ((long)rg.n[0] * rg.n[1])) + rg.lo[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell;
+  const M g(p.a, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
+  double pr[L::nprim];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  const double vp[3] = {pr[1], pr[2], pr[3]};
+  const double bp[3] = {MHD ? pr[5] : 0.0, MHD ? pr[6] : 0.0, MHD ? pr[7] : 0.0};
+  double D, S[3], bc[3], tau, sig[3];
+  p2c<MHD>(pr[0], vp, bp, pr[4], pr[L::prs], pr[L::gm1], g, D, S, bc, tau, sig);
+  cons[c] = D;
+  cons[n + c] = S[0];
+  cons[2 * n + c] = S[1];
+  cons[3 * n + c] = S[2];
+  cons[4 * n + c] = tau;
+  if constexpr (MHD) {
+    cons[5 * n + c] = bc[0];
+    cons[6 * n + c] = bc[1];
+    cons[7 * n + c] = bc[2];
+  }
+  if (csig)
+    for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
+}
+
+// ---- CFL timestep (fluid.cpp:899-944): per-block minimum, then one finishing block ----
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double block_min(double v) {
+  __shared__ double part[16];
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  v = threadIdx.x < nw ? part[threadIdx.x] : DBL_MAX;
+  if (w == 0) v = wave_min(v);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_dt_partial(BlockGeom b, const double *__restrict__ csig,
+                                                    const double *__restrict__ fsig,
+                                                    double *__restrict__ partial) {
+  const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
+  const long total = (long)ni * nj * nk;
+  const long n = b.ncell;
+  double lmin = DBL_MAX;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % ni) + b.is[0];
+    const int j = (int)((t / ni) % nj) + b.is[1];
+    const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+    const long c = ((long)k * b.nj + j) * b.ni + i;
+    double ldt = 0.0;
+    for (int d = 0; d < b.ndim; ++d) {
+      double s = csig[d * n + c];
+      if (fsig) {
+        const long off = d == 0 ? 1 : d == 1 ? (long)b.ni : (long)b.ni * b.nj;
+        s = smax(s, smax(fsig[d * n + c], fsig[d * n + c + off]));
+      }
+      ldt += s / b.dx[d];
+    }
+    lmin = smin(lmin, 1.0 / (ldt + 1.e-50));
+  }
+  lmin = block_min(lmin);
+  if (threadIdx.x == 0) partial[blockIdx.x] = lmin;
+}
+
+__global__ __launch_bounds__(1024) void k_dt_finish(const double *__restrict__ partial, int np,
+                                                    double cfl, double *__restrict__ dt) {
+  double v = DBL_MAX;
+  for (int t = threadIdx.x; t < np; t += blockDim.x) v = smin(v, partial[t]);
+  v = block_min(v);
+  if (threadIdx.x == 0) dt[0] = cfl * v;
+}
+
+// ---- geometric sources for spherical Kerr-Schild (fluid.cpp:566-657, tmunu.hpp) -------
+struct KSDerivs {
+  double dg[4][4][4];
+  double da[4];
+  double gi[4][4];
+};
+
+__device__ void ks_derivs(double a, double X1, double X2, KSDerivs &o) {
+  const double a2 = a * a;
+  const double r = fabs(X1), th = X2, cth = cos(th), sth = sin(th);
+  const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
+  const double r2 = r * r, r3 = r2 * r, r5 = r2 * r3, a4 = a2 * a2;
+  const double rho4 = rho2 * rho2, cth2 = cth * cth;
+  const double s2th = sin(2 * th), c2th = cos(2 * th);
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y) {
+      o.gi[x][y] = 0.0;
+      for (int z = 0; z < 4; ++z) o.dg[x][y][z] = 0.0;
+    }
+  o.dg[0][0][1] = 2 * ratio(rho2 - 2 * r2, rho4);
+  o.dg[0][1][1] = o.dg[1][0][1] = o.dg[0][0][1];
+  o.dg[0][3][1] = o.dg[3][0][1] = ratio(2 * a * (r2 - a2 * cth2) * sth2, rho4);
+  o.dg[1][1][1] = o.dg[0][0][1];
+  o.dg[1][3][1] = o.dg[3][1][1] = o.dg[0][3][1];
+  o.dg[2][2][1] = 2 * r;
+  o.dg[3][3][1] = ratio(2 * sth2 * (r5 + a4 * r * cth2 * cth2 - a2 * r2 * sth2 + cth2 * (2 * a2 * r3 + a4 * sth2)), rho4);
+  o.dg[0][0][2] = ratio(4 * r * (rho2 - r2) * sth, rho4 * cth);
+  o.dg[0][1][2] = o.dg[1][0][2] = o.dg[0][0][2];
+  o.dg[0][3][2] = o.dg[3][0][2] = -ratio(2 * a * r * (a2 + r2) * s2th, rho4);
+  o.dg[1][1][2] = o.dg[0][0][2];
+  o.dg[1][3][2] = o.dg[3][1][2] = -ratio(2 * a * cth * (r3 * (2 + r) + a4 * cth2 * cth2 + a2 * r * (2 + r + r * c2th)) * sth, rho4);
+  o.dg[2][2][2] = -a2 * s2th;
+  o.dg[3][3][2] = (a2 + r * (r - 2) + ratio(2 * r * (a2 + r2) * (a2 + r2), rho4)) * s2th;
+  o.da[0] = o.da[3] = 0.0;
+  o.da[1] = ratio(r, rho2) - ratio(1 + r, 2 * r + rho2);
+  o.da[2] = -ratio(2 * a2 * r * cth * sth, 2 * r * rho2 + rho4);
+  o.gi[0][1] = o.gi[1][0] = ratio(2 * r, rho2);
+  o.gi[0][0] = -1 - o.gi[0][1];
+  o.gi[1][1] = ratio(-2 * r + rho2 + a2 * sth2, rho2);
+  o.gi[1][3] = o.gi[3][1] = ratio(a, rho2);
+  o.gi[2][2] = ratio(1, rho2);
+  o.gi[3][3] = ratio(1, rho2 * sth2);
+}
+
+template <bool MHD>
+__global__ __launch_bounds__(128) void k_sources_ks(DevParams p, BlockGeom b,
+                                                    const double *__restrict__ prim,
+                                                    double *__restrict__ src) {
+  using L = Layout<MHD>;
+  const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni) + b.is[0];
+  const int j = (int)((t / ni) % nj) + b.is[1];
+  const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell;
+  const double X1 = xc(b, 0, i), X2 = xc(b, 1, j), X3 = xc(b, 2, k);
+  const SphericalKSMetric g(p.a, X1, X2, X3);
+  KSDerivs dv;
+  ks_derivs(p.a, X1, X2, dv);
+  double pr[L::nprim];
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  // tmunu.hpp:94-126
+  double gam[3][3];
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) gam[x][y] = g.gcov(x + 1, y + 1);
+  const double vp[3] = {pr[1], pr[2], pr[3]};
+  const double bp[3] = {MHD ? pr[5] : 0.0, MHD ? pr[6] : 0.0, MHD ? pr[7] : 0.0};
+  double vsq = 0.;
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = 0; jj < 3; ++jj) vsq += gam[ii][jj] * vp[ii] * vp[jj];
+  const double W = sqrt(1. + vsq);
+  double Bdotv = 0.0, Bsq = 0.0;
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = 0; jj < 3; ++jj) {
+      Bdotv += bp[ii] * (vp[jj] / W) * gam[ii][jj];
+      Bsq += bp[ii] * bp[jj] * gam[ii][jj];
+    }
+  const double iW = ratio(1., W);
+  const double alpha = g.alpha();
+  double u[4], bb[4];
+  u[0] = ratio(W, fabs(alpha));
+  bb[0] = u[0] * Bdotv;
+  for (int l = 1; l < 4; ++l) {
+    u[l] = vp[l - 1] - u[0] * g.beta(l - 1);
+    bb[l] = iW * (bp[l - 1] + alpha * bb[0] * u[l]);
+  }
+  const double bsq = Bsq * iW * iW + Bdotv * Bdotv;
+  const double A = pr[0] + pr[4] + pr[L::prs] + bsq;
+  const double B = pr[L::prs] + 0.5 * bsq;
+  double T[4][4];
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu) T[mu][nu] = A * u[mu] * u[nu] + B * dv.gi[mu][nu] - bb[mu] * bb[nu];
+  // connection Gamma_{abc} (geometry_utils.hpp:72-78) and the source terms (fluid.cpp:604-640)
+  const double gdet = g.detg();
+  for (int l = 0; l < 3; ++l) {
+    double s = 0.0;
+    for (int m = 0; m < 4; ++m)
+      for (int nn = 0; nn < 4; ++nn) {
+        const double gl = 0.5 * (dv.dg[nn][l + 1][m] + dv.dg[m][l + 1][nn] - dv.dg[nn][m][l + 1]);
+        s += T[m][nn] * (dv.dg[nn][l + 1][m] - gl);
+      }
+    src[(long)(1 + l) * n + c] = gdet * s;
+  }
+  const double ia2 = ratio(1, alpha * alpha);
+  const double gcon0[4] = {-ia2, ia2 * g.beta(0), ia2 * g.beta(1), ia2 * g.beta(2)};
+  double TGam = 0.0;
+  for (int m = 0; m < 4; m++)
+    for (int nn = 0; nn < 4; nn++) {
+      double gam0 = 0;
+      for (int rr = 0; rr < 4; rr++)
+        gam0 += gcon0[rr] * (0.5 * (dv.dg[m][rr][nn] + dv.dg[nn][rr][m] - dv.dg[m][nn][rr]));
+      TGam += T[m][nn] * gam0;
+    }
+  double Ta = 0.0;
+  for (int m = 0; m < 4; m++) Ta += T[m][0] * dv.da[m];
+  src[4L * n + c] = gdet * alpha * (Ta - TGam);
+}
+
+template <int NC>
+__global__ void k_zero_sources(BlockGeom b, double *__restrict__ src) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.ncell) return;
+  for (int l = 1; l <= 4; ++l) src[(long)l * b.ncell + t] = 0.0;
+}
+
+// ---- batched con2prim / prim2con over independent states --------------------------------
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_con2prim_batch(DevParams p, long n,
+                                                        const double *__restrict__ coords,
+                                                        double *__restrict__ cons,
+                                                        double *__restrict__ prim,
+                                                        double *__restrict__ mu,
+                                                        double *__restrict__ fail,
+                                                        int32_t *__restrict__ iters) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double X[3] = {0.0, 0.0, 0.0};
+  if (coords) {
+    X[0] = coords[s];
+    X[1] = coords[n + s];
+  }
+  const M g(p.a, X[0], X[1], X[2]);
+  double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + s];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = 0.0;
+  double tt = 0.0, m = mu[s];
+  int it = 0;
+  const double st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) cons[v * n + s] = cs[v];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) prim[v * n + s] = pr[v];
+  mu[s] = m;
+  fail[s] = st;
+  if (iters) iters[s] = it;
+}
+
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_prim_to_con_batch(DevParams p, long n,
+                                                           const double *__restrict__ coords,
+                                                           const double *__restrict__ prim,
+                                                           double *__restrict__ cons) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const M g(p.a, coords ? coords[s] : 0.0, coords ? coords[n + s] : 0.0, 0.0);
+  double pr[L::nprim];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + s];
+  const double vp[3] = {pr[1], pr[2], pr[3]};
+  const double bp[3] = {MHD ? pr[5] : 0.0, MHD ? pr[6] : 0.0, MHD ? pr[7] : 0.0};
+  double D, S[3], bc[3], tau, sig[3];
+  p2c<MHD>(pr[0], vp, bp, pr[4], pr[L::prs], pr[L::gm1], g, D, S, bc, tau, sig);
+  cons[s] = D;
+  cons[n + s] = S[0];
+  cons[2 * n + s] = S[1];
+  cons[3 * n + s] = S[2];
+  cons[4 * n + s] = tau;
+  if constexpr (MHD) {
+    cons[5 * n + s] = bc[0];
+    cons[6 * n + s] = bc[1];
+    cons[7 * n + s] = bc[2];
+  }
+}
+
+// ================================ host side ==============================================
+static inline unsigned nblocks(long work, int bs) { return (unsigned)((work + bs - 1) / bs); }
+
+DevParams make_dev_params(const phx_fluid_params *p) {
+  DevParams d;
+  d.gm1 = p->gamma - 1.0;
+  d.cv = p->cv;
+  d.tol = p->c2p_tol;
+  d.fsf = p->c2p_floor_scale_fac;
+  d.max_iter = p->c2p_max_iter;
+  d.fail_on_floors = p->c2p_fail_on_floors;
+  d.fail_on_ceilings = p->c2p_fail_on_ceilings;
+  d.a = p->geom.a;
+  d.bnd.floor_type = p->bounds.floor_type;
+  d.bnd.r0 = p->bounds.rho0;
+  d.bnd.s0 = p->bounds.sie0;
+  d.bnd.ralpha = p->bounds.rho_exp;
+  d.bnd.salpha = p->bounds.sie_exp;
+  d.bnd.rho_min_eos = p->bounds.rho_min_eos;
+  d.bnd.sie_min_eos = p->bounds.sie_min_eos;
+  d.bnd.gam_max = p->bounds.gam_max;
+  d.bnd.sie_max = p->bounds.sie_max;
+  return d;
+}
+
+int check_params(phx_ctx *ctx, const phx_fluid_params *p) {
+  if (!p) return ctx->fail(PHX_ERR_INVALID, "null parameters");
+  if (p->mhd != 0 && p->mhd != 1) return ctx->fail(PHX_ERR_INVALID, "mhd must be 0 or 1");
+  if (p->recon < PHX_RECON_CONSTANT || p->recon > PHX_RECON_MP5)
+    return ctx->fail(PHX_ERR_INVALID, "invalid reconstruction");
+  if (p->riemann != PHX_RIEMANN_LLF && p->riemann != PHX_RIEMANN_HLL)
+    return ctx->fail(PHX_ERR_INVALID, "invalid Riemann solver");
+  if (p->geom.kind != PHX_GEOM_MINKOWSKI && p->geom.kind != PHX_GEOM_SPHERICAL_KS)
+    return ctx->fail(PHX_ERR_UNSUPPORTED, "unsupported geometry");
+  if (!(p->c2p_floor_scale_fac <= 1. && p->c2p_floor_scale_fac > 0.))
+    return ctx->fail(PHX_ERR_INVALID, "c2p_floor_scale_fac must be in (0, 1]");
+  if (p->bounds.floor_type < 0 || p->bounds.floor_type > PHX_FLOOR_R_RHO_SIE)
+    return ctx->fail(PHX_ERR_INVALID, "invalid floor type");
+  if (p->c2p_max_iter < 0) return ctx->fail(PHX_ERR_INVALID, "c2p_max_iter < 0");
+  return PHX_OK;
+}
+
+int check_block(phx_ctx *ctx, const phx_block *b, int recon) {
+  if (!b) return ctx->fail(PHX_ERR_INVALID, "null block");
+  if (b->ndim < 1 || b->ndim > 3) return ctx->fail(PHX_ERR_INVALID, "ndim must be 1..3");
+  for (int d = 0; d < 3; ++d) {
+    if (b->nx[d] < 1) return ctx->fail(PHX_ERR_INVALID, "nx must be >= 1");
+    if (d >= b->ndim && b->nx[d] != 1)
+      return ctx->fail(PHX_ERR_INVALID, "nx must be 1 in inactive dimensions");
+  }
+  const int need = recon >= PHX_RECON_WENO5Z ? 4 : recon == PHX_RECON_LINEAR ? 2 : 1;
+  if (b->ng < need) return ctx->fail(PHX_ERR_INVALID, "not enough ghost zones for the reconstruction");
+  long cells = 1;
+  for (int d = 0; d < 3; ++d) cells *= b->nx[d] + (d < b->ndim ? 2 * b->ng : 0);
+  if (cells > (1L << 31)) return ctx->fail(PHX_ERR_INVALID, "block too large");
+  return PHX_OK;
+}
+
+// dispatch helpers ----------------------------------------------------------------------
+template <int DIR, int RECON, int RIEMANN, bool MHD>
+static void launch_flux_geom(int geom, const DevParams &dp, const BlockGeom &bg,
+                             const double *prim, double *flux, double *fsig, long nface,
+                             hipStream_t s) {
+  if (geom == PHX_GEOM_MINKOWSKI)
+    hipLaunchKernelGGL((k_fluxes<DIR, RECON, RIEMANN, MHD, PHX_GEOM_MINKOWSKI>),
+                       dim3(nblocks(nface, 256)), dim3(256), 0, s, dp, bg, prim, flux, fsig);
+  else
+    hipLaunchKernelGGL((k_fluxes<DIR, RECON, RIEMANN, MHD, PHX_GEOM_SPHERICAL_KS>),
+                       dim3(nblocks(nface, 256)), dim3(256), 0, s, dp, bg, prim, flux, fsig);
+}
+
+template <int DIR, int RECON, int RIEMANN>
+static void launch_flux_mhd(bool mhd, int geom, const DevParams &dp, const BlockGeom &bg,
+                            const double *prim, double *flux, double *fsig, long nface,
+                            hipStream_t s) {
+  if (mhd)
+    launch_flux_geom<DIR, RECON, RIEMANN, true>(geom, dp, bg, prim, flux, fsig, nface, s);
+  else
+    launch_flux_geom<DIR, RECON, RIEMANN, false>(geom, dp, bg, prim, flux, fsig, nface, s);
+}
+
+template <int DIR, int RECON>
+static void launch_flux_riemann(int riemann, bool mhd, int geom, const DevParams &dp,
+                                const BlockGeom &bg, const double *prim, double *flux,
+                                double *fsig, long nface, hipStream_t s) {
+  if (riemann == PHX_RIEMANN_HLL)
+    launch_flux_mhd<DIR, RECON, PHX_RIEMANN_HLL>(mhd, geom, dp, bg, prim, flux, fsig, nface, s);
+  else
+    launch_flux_mhd<DIR, RECON, PHX_RIEMANN_LLF>(mhd, geom, dp, bg, prim, flux, fsig, nface, s);
+}
+
+template <int DIR>
+static void launch_flux_dir(int recon, int riemann, bool mhd, int geom, const DevParams &dp,
+                            const BlockGeom &bg, const double *prim, double *flux, double *fsig,
+                            long nface, hipStream_t s) {
+  switch (recon) {
+  case PHX_RECON_CONSTANT:
+    launch_flux_riemann<DIR, PHX_RECON_CONSTANT>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
+    break;
+  case PHX_RECON_LINEAR:
+    launch_flux_riemann<DIR, PHX_RECON_LINEAR>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
+    break;
+  case PHX_RECON_WENO5Z:
+    launch_flux_riemann<DIR, PHX_RECON_WENO5Z>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
+    break;
+  default:
+    launch_flux_riemann<DIR, PHX_RECON_MP5>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
+    break;
+  }
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+// ================================ C ABI ==================================================
+extern "C" {
+
+int phx_calculate_fluxes(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                         const phx_block_fields *f, phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, p->recon))) return rc;
+  if (!f || !f->prim || !f->flux || !f->face_sig) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const DevParams dp = make_dev_params(p);
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  const long nface = (long)(b->nx[0] + 2) * (b->nx[1] + (b->ndim > 1 ? 2 : 0)) *
+                     (b->nx[2] + (b->ndim > 2 ? 2 : 0));
+  for (int d = 0; d < b->ndim; ++d) {
+    if (d == 0)
+      launch_flux_dir<0>(p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux, f->face_sig, nface, s);
+    else if (d == 1)
+      launch_flux_dir<1>(p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux, f->face_sig, nface, s);
+    else
+      launch_flux_dir<2>(p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux, f->face_sig, nface, s);
+  }
+  return ctx->check_launch();
+}
+
+int phx_flux_ct(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                const phx_block_fields *f, phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, PHX_RECON_LINEAR))) return rc;  // CT reads 2 ghost layers
+  if (!p->mhd || b->ndim == 1) return PHX_OK;
+  if (!f || !f->flux || !f->emf) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  if (b->ndim == 3) {
+    const long n = (long)(b->nx[0] + 1) * (b->nx[1] + 1) * (b->nx[2] + 1);
+    hipLaunchKernelGGL(k_ct_emf_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
+    hipLaunchKernelGGL(k_ct_flux_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
+  } else {
+    const long n = (long)(b->nx[0] + 1) * (b->nx[1] + 1) * b->nx[2];
+    hipLaunchKernelGGL(k_ct_emf_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
+    hipLaunchKernelGGL(k_ct_flux_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
+  }
+  return ctx->check_launch();
+}
+
+int phx_fluid_source_terms(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                           const phx_block_fields *f, phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
+  if (!f || !f->src || !f->prim) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const DevParams dp = make_dev_params(p);
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd)
+      hipLaunchKernelGGL(k_zero_sources<8>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, f->src);
+    else
+      hipLaunchKernelGGL(k_zero_sources<5>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, f->src);
+  } else {
+    const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
+    if (p->mhd)
+      hipLaunchKernelGGL(k_sources_ks<true>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, f->prim, f->src);
+    else
+      hipLaunchKernelGGL(k_sources_ks<false>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, f->prim, f->src);
+  }
+  return ctx->check_launch();
+}
+
+int phx_flux_divergence_update(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                               const double *flux, const double *src, double *u_stage,
+                               const double *u_base, double beta, double dt, double *u_out,
+                               phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
+  if (!flux || !u_stage || !u_base || !u_out) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
+  const double bdt = beta * dt;
+  if (p->mhd)
+    hipLaunchKernelGGL(k_update<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, src, u_stage, u_base, beta, bdt, u_out);
+  else
+    hipLaunchKernelGGL(k_update<5>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, src, u_stage, u_base, beta, bdt, u_out);
+  return ctx->check_launch();
+}
+
+static int make_range(phx_ctx *ctx, const phx_block *b, const int lo[3], const int hi[3],
+                      Range3 &rg) {
+  const BlockGeom bg = make_block_geom(b);
+  const int n[3] = {bg.ni, bg.nj, bg.nk};
+  for (int d = 0; d < 3; ++d) {
+    const int l = lo ? lo[d] : 0, h = hi ? hi[d] : n[d] - 1;
+    if (l < 0 || h >= n[d] || h < l) return ctx->fail(PHX_ERR_INVALID, "cell range outside the block");
+    rg.lo[d] = l;
+    rg.n[d] = h - l + 1;
+  }
+  return PHX_OK;
+}
+
+int phx_con2prim(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                 const phx_block_fields *f, const int lo[3], const int hi[3], int32_t *iters,
+                 phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
+  if (!f || !f->cons || !f->prim || !f->c2p_mu || !f->cell_sig || !f->fail)
+    return ctx->fail(PHX_ERR_INVALID, "null field");
+  Range3 rg;
+  if ((rc = make_range(ctx, b, lo, hi, rg))) return rc;
+  const DevParams dp = make_dev_params(p);
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
+  const dim3 grid(nblocks(n, 256)), blk(256);
+#define C2P_ARGS dp, bg, rg, f->cons, f->prim, f->temp, f->c2p_mu, f->cell_sig, f->fail, iters
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_con2prim<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, C2P_ARGS);
+    else hipLaunchKernelGGL((k_con2prim<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, C2P_ARGS);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_con2prim<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, C2P_ARGS);
+    else hipLaunchKernelGGL((k_con2prim<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, C2P_ARGS);
+  }
+#undef C2P_ARGS
+  return ctx->check_launch();
+}
+
+int phx_prim_to_con(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                    const phx_block_fields *f, const int lo[3], const int hi[3],
+                    phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
+  if (!f || !f->cons || !f->prim) return ctx->fail(PHX_ERR_INVALID, "null field");
+  Range3 rg;
+  if ((rc = make_range(ctx, b, lo, hi, rg))) return rc;
+  const DevParams dp = make_dev_params(p);
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
+  const dim3 grid(nblocks(n, 256)), blk(256);
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
+    else hipLaunchKernelGGL((k_prim_to_con<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
+    else hipLaunchKernelGGL((k_prim_to_con<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
+  }
+  return ctx->check_launch();
+}
+
+int phx_estimate_dt(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                    const phx_block_fields *f, int use_face_speeds, double cfl,
+                    double *dt_out, phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
+  if (!f || !f->cell_sig || !dt_out || (use_face_speeds && !f->face_sig))
+    return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
+  const int np = (int)std::min<long>(nblocks(n, 256), kDtPartials);
+  hipLaunchKernelGGL(k_dt_partial, dim3(np), dim3(256), 0, s, bg, f->cell_sig,
+                     use_face_speeds ? f->face_sig : nullptr, ctx->dt_partial);
+  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1024), 0, s, ctx->dt_partial, np, cfl, dt_out);
+  return ctx->check_launch();
+}
+
+int phx_con2prim_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
+                       const double *coords, double *cons, double *prim, double *mu,
+                       double *fail, int32_t *iters, phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (n < 0) return ctx->fail(PHX_ERR_INVALID, "negative batch size");
+  if (n == 0) return PHX_OK;
+  if (!cons || !prim || !mu || !fail) return ctx->fail(PHX_ERR_INVALID, "null field");
+  if (p->geom.kind != PHX_GEOM_MINKOWSKI && !coords)
+    return ctx->fail(PHX_ERR_INVALID, "coordinates required for curved geometry");
+  const DevParams dp = make_dev_params(p);
+  hipStream_t s = ctx->stream_of(stream);
+  const dim3 grid(nblocks(n, 256)), blk(256);
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
+    else hipLaunchKernelGGL((k_con2prim_batch<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
+    else hipLaunchKernelGGL((k_con2prim_batch<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
+  }
+  return ctx->check_launch();
+}
+
+int phx_prim_to_con_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
+                          const double *coords, const double *prim, double *cons,
+                          phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (n < 0) return ctx->fail(PHX_ERR_INVALID, "negative batch size");
+  if (n == 0) return PHX_OK;
+  if (!cons || !prim) return ctx->fail(PHX_ERR_INVALID, "null field");
+  if (p->geom.kind != PHX_GEOM_MINKOWSKI && !coords)
+    return ctx->fail(PHX_ERR_INVALID, "coordinates required for curved geometry");
+  const DevParams dp = make_dev_params(p);
+  hipStream_t s = ctx->stream_of(stream);
+  const dim3 grid(nblocks(n, 256)), blk(256);
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con_batch<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
+    else hipLaunchKernelGGL((k_prim_to_con_batch<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con_batch<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
+    else hipLaunchKernelGGL((k_prim_to_con_batch<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
+  }
+  return ctx->check_launch();
+}
+
+}  // extern "C"

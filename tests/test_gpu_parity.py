@@ -1,0 +1,328 @@
+"""HIP path vs the CPU oracle on identical seeded inputs (through the C ABI).
+
+Bars: fail flags and iteration counts bit-exact; Minkowski results bit-exact (the
+library and the oracle round +,-,*,/,sqrt identically with contraction off); curved
+metrics within 1e-12 relative (sin/cos differ by an ulp between the device math library
+and glibc).  Flux comparisons cover the faces the stage consumes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import _physics as PH
+from phoebus_amd import _abi as A
+from phoebus_amd.api import BlockState, Phx, block, fluid_params
+from phoebus_amd.device import DeviceArray, synchronize
+
+pytestmark = pytest.mark.gpu
+
+RTOL_CURVED = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def phx():
+    ctx = Phx(0)
+    yield ctx
+    ctx.close()
+
+
+def oparams(p):
+    """The oracle's parameter struct for a library FluidParams."""
+    op = O.default_params(gamma=p.gamma, mhd=bool(p.mhd), recon=p.recon, riemann=p.riemann,
+                          geom=p.geom.kind, a=p.geom.a, tol=p.c2p_tol, max_iter=p.c2p_max_iter)
+    op.gm1 = p.gamma - 1.0
+    op.cv = p.cv
+    op.c2p_floor_scale_fac = p.c2p_floor_scale_fac
+    op.fail_on_floors = p.c2p_fail_on_floors
+    op.fail_on_ceilings = p.c2p_fail_on_ceilings
+    b = p.bounds
+    op.bounds = O.Bounds(b.floor_type, b.rho0, b.sie0, b.rho_exp, b.sie_exp, b.rho_min_eos,
+                         b.sie_min_eos, b.gam_max, b.sie_max)
+    return op
+
+
+def oblock(b):
+    return O.make_block(b.ndim, [b.nx[d] for d in range(b.ndim)], b.ng,
+                        tuple(b.xmin), tuple(b.dx))
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+
+
+def assert_match(gpu, ref, exact, what, rtol=RTOL_CURVED, atol_scale=None):
+    if exact:
+        bad = ~((gpu == ref) | (np.isnan(gpu) & np.isnan(ref)))
+        assert not bad.any(), f"{what}: {bad.sum()} mismatches, max rel {relerr(gpu, ref)}"
+    else:
+        scale = atol_scale if atol_scale is not None else 0.0
+        err = np.abs(gpu - ref)
+        lim = rtol * np.maximum(np.abs(ref), scale)
+        assert np.all(err <= lim), f"{what}: max rel {relerr(gpu, ref)}"
+
+
+# ---------------------------------------------------------------- batched states -------
+def ks_states(rng, n, mhd, gamma):
+    """States in spherical KS at r ~ U(1.5, 50), theta ~ acos U(-1,1) (away from the axis)."""
+    r = rng.uniform(1.5, 50.0, n)
+    th = np.arccos(rng.uniform(-0.98, 0.98, n))
+    coords = np.ascontiguousarray(np.stack([r, th]))
+    prims = PH.random_prims(rng, n, mhd, gamma, wmax=5.0)
+    # scale vector components into the coordinate basis at each point
+    for s in range(n):
+        g = O.point_geom(O.default_params(geom=O.GEOM_SPHKS, a=0.9375), (r[s], th[s], 0.0))
+        prims[:, s] = PH.to_coordinate_frame(prims[:, s:s + 1], g.np("gam"), mhd)[:, 0]
+    return coords, np.ascontiguousarray(prims)
+
+
+@pytest.mark.parametrize("geom", [A.GEOM_MINKOWSKI, A.GEOM_SPHERICAL_KS])
+@pytest.mark.parametrize("mhd", [False, True])
+def test_batch_p2c_c2p_parity(phx, geom, mhd):
+    say(f"[gpu] batch p2c/c2p geom={geom} mhd={mhd}")
+    rng = np.random.default_rng(100 + geom * 2 + mhd)
+    n = 4096
+    gamma = 4.0 / 3.0
+    p = fluid_params(gamma=gamma, mhd=mhd, geom=geom, a=0.9375, c2p_tol=1e-10)
+    op = oparams(p)
+    if geom == A.GEOM_MINKOWSKI:
+        coords = np.zeros((2, n))
+        prims = np.ascontiguousarray(PH.random_prims(rng, n, mhd, gamma, wmax=5.0))
+    else:
+        coords, prims = ks_states(rng, n, mhd, gamma)
+    exact = geom == A.GEOM_MINKOWSKI
+    # prim -> cons
+    ref_cons = O.p2c_batch(op, coords, prims)
+    d_coords = DeviceArray.from_host(coords)
+    d_prim = DeviceArray.from_host(prims)
+    d_cons = DeviceArray(ref_cons.shape)
+    phx.prim_to_con_batch(p, n, d_coords if geom else None, d_prim, d_cons)
+    synchronize()
+    assert_match(d_cons.get(), ref_cons, exact, "p2c cons",
+                 atol_scale=np.max(np.abs(ref_cons), axis=0))
+    # cons -> prim, cold start, from the same conserved state
+    ref = O.c2p_batch(op, coords, ref_cons)
+    d_cons.upload(ref_cons)
+    d_out = DeviceArray(ref["prim"].shape)
+    d_mu = DeviceArray.from_host(np.zeros(n))
+    d_fail = DeviceArray((n,))
+    d_it = DeviceArray((n,), np.int32)
+    phx.con2prim_batch(p, n, d_coords if geom else None, d_cons, d_out, d_mu, d_fail, d_it)
+    synchronize()
+    fail, it = d_fail.get(), d_it.get()
+    assert np.array_equal(fail, ref["fail"]), "fail flags differ"
+    if exact:
+        assert np.array_equal(it, ref["iters"]), "iteration counts differ"
+    else:
+        assert np.mean(it == ref["iters"]) > 0.999, "iteration counts differ"
+        same = it == ref["iters"]
+    prim_g = d_out.get()
+    if exact:
+        assert_match(prim_g, ref["prim"], True, "c2p prims")
+        assert_match(d_mu.get(), ref["mu"], True, "c2p mu")
+    else:
+        err = np.abs(prim_g[:, same] - ref["prim"][:, same])
+        scale = np.maximum(np.abs(ref["prim"][:, same]), np.max(np.abs(ref["prim"][1:4, same]), axis=0))
+        assert np.all(err <= 1e-10 * scale), f"c2p prims max rel {relerr(prim_g, ref['prim'])}"
+    # recovered state is the generating one
+    assert relerr(prim_g[0], prims[0]) < 1e-6
+
+
+# ---------------------------------------------------------------- block tasks ----------
+def random_block(rng, b, mhd, gamma, geom):
+    sh = O.block_shape(oblock(b))
+    n = int(np.prod(sh))
+    pr = PH.random_prims(rng, n, mhd, gamma, wmax=3.0)
+    if geom == A.GEOM_SPHERICAL_KS:
+        pr[1:4] *= 0.2  # keep curvilinear components moderate
+        if mhd:
+            pr[5:8] *= 0.2
+    return np.ascontiguousarray(pr.reshape((-1,) + sh))
+
+
+def consumed_face_slices(b, d):
+    """Faces whose flux the divergence reads: own-dim [s, e+1], others interior."""
+    sl = []
+    for dd in (2, 1, 0):
+        s = b.ng if dd < b.ndim else 0
+        n = b.nx[dd]
+        sl.append(slice(s, s + n + (1 if dd == d else 0)))
+    return tuple(sl)
+
+
+CASES = [
+    # ndim, nx, mhd, recon, riemann, geom
+    (1, (64,), False, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_MINKOWSKI),
+    (2, (16, 12), True, A.RECON_MP5, A.RIEMANN_LLF, A.GEOM_MINKOWSKI),
+    (3, (12, 10, 8), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_MINKOWSKI),
+    (3, (12, 10, 8), True, A.RECON_LINEAR, A.RIEMANN_HLL, A.GEOM_MINKOWSKI),
+    (3, (8, 8, 8), False, A.RECON_CONSTANT, A.RIEMANN_LLF, A.GEOM_MINKOWSKI),
+    (3, (10, 8, 6), True, A.RECON_MP5, A.RIEMANN_HLL, A.GEOM_MINKOWSKI),
+    (3, (10, 8, 6), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_SPHERICAL_KS),
+    (2, (12, 10), False, A.RECON_LINEAR, A.RIEMANN_LLF, A.GEOM_SPHERICAL_KS),
+]
+
+
+def case_id(c):
+    return f"{c[0]}d-mhd{int(c[2])}-r{c[3]}-s{c[4]}-g{c[5]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[case_id(c) for c in CASES])
+def test_block_stage_parity(phx, case):
+    ndim, nx, mhd, recon, riemann, geom = case
+    say(f"[gpu] block stage {case_id(case)}")
+    rng = np.random.default_rng(hash(case_id(case)) % 2 ** 32)
+    gamma = 13.0 / 9.0
+    if geom == A.GEOM_SPHERICAL_KS:
+        b = block(ndim, nx, 4, xmin=(4.0, 0.6, 0.0), dx=(0.25, 0.05, 0.05))
+    else:
+        b = block(ndim, nx, 4, xmin=(0.0, 0.0, 0.0), dx=(0.1, 0.2, 0.3))
+    p = fluid_params(gamma=gamma, mhd=mhd, recon=recon, riemann=riemann, geom=geom, a=0.9375,
+                     c2p_tol=1e-10, c2p_max_iter=100)
+    op, ob = oparams(p), oblock(b)
+    exact = geom == A.GEOM_MINKOWSKI
+    prim = random_block(rng, b, mhd, gamma, geom)
+    st = BlockState(p, b)
+    st.prim.upload(prim)
+    f = st.fields(with_src=True)
+
+    # PrimitiveToConserved over the entire block
+    ref_cons, ref_csig = O.prim_to_con_block(op, ob, prim)
+    phx.prim_to_con(p, b, f)
+    synchronize()
+    assert_match(st.cons.get(), ref_cons, exact, "p2c", atol_scale=np.max(np.abs(ref_cons), axis=0))
+    st.cons.upload(ref_cons)
+
+    # CalculateFluxes
+    ref_flux, ref_fsig = O.calculate_fluxes(op, ob, prim)
+    phx.calculate_fluxes(p, b, f)
+    synchronize()
+    gflux, gfsig = st.flux.get(), st.face_sig.get()
+    for d in range(ndim):
+        sl = consumed_face_slices(b, d)
+        for m in range(ref_flux.shape[1]):
+            assert_match(gflux[d, m][sl], ref_flux[d, m][sl], exact, f"flux d={d} m={m}",
+                         atol_scale=np.max(np.abs(ref_flux[d][:, sl[0], sl[1], sl[2]])))
+        assert_match(gfsig[d][sl], ref_fsig[d][sl], exact, f"face_sig d={d}")
+
+    # FluxCT
+    st.flux.upload(ref_flux)
+    emf_ref = O.flux_ct(op, ob, ref_flux)
+    phx.flux_ct(p, b, f)
+    synchronize()
+    gflux = st.flux.get()
+    for d in range(ndim):
+        sl = consumed_face_slices(b, d)
+        assert_match(gflux[d][:, sl[0], sl[1], sl[2]], ref_flux[d][:, sl[0], sl[1], sl[2]],
+                     exact, f"CT flux d={d}")
+    st.flux.upload(ref_flux)
+
+    # sources (curved only)
+    src = None
+    if geom != A.GEOM_MINKOWSKI:
+        src_ref = O.source_terms_block(op, ob, prim)
+        phx.fluid_source_terms(p, b, f)
+        synchronize()
+        I = O.interior(ob)
+        gsrc = st.src.get()
+        for l in range(1, 5):
+            assert_match(gsrc[l][I], src_ref[l][I], False, f"src {l}", rtol=1e-11,
+                         atol_scale=np.max(np.abs(src_ref[1:5][(slice(None),) + I])))
+        st.src.upload(src_ref)
+        src = src_ref
+
+    # FluxDivergence + average + update (stage 2 weights, output into a fresh array)
+    u_base = ref_cons * (1.0 + 0.01 * rng.uniform(-1, 1, ref_cons.shape))
+    d_base = DeviceArray.from_host(u_base)
+    d_out = DeviceArray.from_host(ref_cons)
+    stage = ref_cons.copy()
+    out_ref = O.flux_divergence_update(op, ob, ref_flux, src, stage, u_base, 0.5, 1e-3)
+    phx.flux_divergence_update(p, b, st.flux, st.src if src is not None else None, st.cons,
+                               d_base, 0.5, 1e-3, d_out)
+    synchronize()
+    I = O.interior(ob)
+    gout = d_out.get()
+    for m in range(ref_cons.shape[0]):
+        assert_match(gout[m][I], out_ref[m][I], exact, f"update m={m}",
+                     atol_scale=np.max(np.abs(out_ref[m][I])))
+        assert_match(st.cons.get()[m][I], stage[m][I], exact, f"averaged stage m={m}")
+
+    # ConservedToPrimitive over the entire block from the updated state (warm start mu)
+    mu0 = np.ascontiguousarray(rng.uniform(0.0, 1.0, O.block_shape(ob)) * 0.0)
+    cons_in = out_ref.copy()
+    st.cons.upload(cons_in)
+    st.c2p_mu.upload(mu0)
+    st.prim.upload(prim)
+    d_it = DeviceArray(O.block_shape(ob), np.int32)
+    ref_prim = prim.copy()
+    ref_mu = mu0.copy()
+    ref_cons2 = cons_in.copy()
+    r = O.con2prim_block(op, ob, ref_cons2, ref_prim, ref_mu)
+    phx.con2prim(p, b, f, iters=d_it)
+    synchronize()
+    assert np.array_equal(st.fail.get(), r["fail"]), "fail flags differ"
+    git = d_it.get()
+    if exact:
+        assert np.array_equal(git, r["iters"]), "iterations differ"
+        assert_match(st.prim.get(), ref_prim, True, "c2p prim")
+        assert_match(st.c2p_mu.get(), ref_mu, True, "c2p mu")
+        assert_match(st.cell_sig.get(), r["sig"], True, "cell_sig")
+        assert_match(st.cons.get(), ref_cons2, True, "c2p cons")
+    else:
+        assert np.mean(git == r["iters"]) > 0.99
+
+    # EstimateTimestep, both forms
+    st.cell_sig.upload(r["sig"])
+    st.face_sig.upload(np.ascontiguousarray(ref_fsig))
+    d_dt = DeviceArray((1,))
+    for use_face in (0, 1):
+        dt_ref = O.estimate_dt(op, ob, r["sig"], ref_fsig if use_face else None, 0.4)
+        phx.estimate_dt(p, b, f, use_face, 0.4, d_dt)
+        synchronize()
+        assert d_dt.get()[0] == dt_ref
+
+
+def test_con2prim_failure_paths(phx):
+    """NaN inputs and fail_on_floors/ceilings reproduce the reference's flags."""
+    say("[gpu] c2p failure paths")
+    n = 256
+    rng = np.random.default_rng(5)
+    p = fluid_params(gamma=5 / 3, mhd=False, floors={"rho0": 0.5, "sie0": 0.0},
+                     fail_on_floors=True)
+    op = oparams(p)
+    prims = np.ascontiguousarray(PH.random_prims(rng, n, False, 5 / 3, wmax=2.0))
+    coords = np.zeros((2, n))
+    cons = O.p2c_batch(op, coords, prims)
+    cons[0, :16] = np.nan
+    cons[4, 16:32] = np.nan
+    ref = O.c2p_batch(op, coords, cons)
+    d_cons = DeviceArray.from_host(cons)
+    d_prim = DeviceArray((7, n))
+    d_mu = DeviceArray.from_host(np.zeros(n))
+    d_fail = DeviceArray((n,))
+    d_it = DeviceArray((n,), np.int32)
+    phx.con2prim_batch(p, n, None, d_cons, d_prim, d_mu, d_fail, d_it)
+    synchronize()
+    assert np.array_equal(d_fail.get(), ref["fail"])
+    assert np.array_equal(d_it.get(), ref["iters"])
+    assert (ref["fail"][:32] == 0).all() and (ref["fail"] == 0).sum() > 32
+
+
+def test_abi_rejects_bad_configs(phx):
+    say("[gpu] ABI error paths")
+    p = fluid_params(recon=A.RECON_WENO5Z)
+    b = block(1, (16,), ng=2)
+    st = BlockState(fluid_params(), block(1, (16,), ng=4))
+    with pytest.raises(A.PhxError):
+        phx.calculate_fluxes(p, b, st.fields())
+    p.geom.kind = 7
+    with pytest.raises(A.PhxError):
+        phx.calculate_fluxes(p, block(1, (16,), ng=4), st.fields())
