@@ -1,0 +1,256 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of one GRMHD RK stage on a 128^3 meshblock (MI355X, FP64).
+
+Workload (single 3-D GRMHD meshblock, Minkowski, ideal gas Gamma = 13/9, 128^3 interior
++ 4 ghost zones, SoA FP64): one RK stage = CalculateFluxes (WENO5-Z + HLL, x/y/z) ->
+FluxCT -> FluxDivergence + RK update -> ConservedToPrimitive (robust, tol 1e-10) on the
+interior -> EstimateTimestep.  Synthetic seeded input: rho = exp N(0,1), P = rho U(0.01,1),
+W ~ U(1,5) isotropic, B = centred-difference curl of a random vector potential scaled to a
+median magnetization b^2/rho = 0.1.  Every step repeats the stage on the same input, with
+the con2prim guess taken from the input state's own root (the warm start a time-stepping
+run sees), so the K steps do identical work.
+
+Prints ONE JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX = 128
+NG = 4
+GAMMA = 13.0 / 9.0
+SEED = 20261202
+HBM_PEAK_GBS = 8000.0
+# compulsory FP64 state traffic of a fused stage per zone-update (reads U_s, U^n, 10 prims,
+# mu, T guess = 28 doubles; writes U_{s+1}, 11 prims incl. T, mu, fail, 3 cell speeds = 24)
+B_ALG_STAGE = 52 * 8
+
+
+def make_input(rng, n_int, ng):
+    n = n_int + 2 * ng
+    sh = (n, n, n)
+    rho = np.exp(rng.normal(0.0, 1.0, sh))
+    P = rho * rng.uniform(0.01, 1.0, sh)
+    W = rng.uniform(1.0, 5.0, sh)
+    nv = rng.normal(size=(3,) + sh)
+    nv /= np.linalg.norm(nv, axis=0)
+    vp = np.sqrt(W * W - 1.0) * nv
+    A = rng.uniform(-1.0, 1.0, (3,) + sh)
+    d = lambda f, ax: 0.5 * (np.roll(f, -1, axis=ax) - np.roll(f, 1, axis=ax))
+    # axes of (k, j, i): x -> 2, y -> 1, z -> 0
+    Bx = d(A[2], 1) - d(A[1], 0)
+    By = d(A[0], 0) - d(A[2], 2)
+    Bz = d(A[1], 2) - d(A[0], 1)
+    B = np.stack([Bx, By, Bz])
+    sig = np.sum(B * B, axis=0) / rho
+    B *= np.sqrt(0.1 / np.median(sig))
+    prim = np.empty((10,) + sh)
+    prim[0] = rho
+    prim[1:4] = vp
+    prim[4] = P / (GAMMA - 1.0)
+    prim[5:8] = B
+    prim[8] = P
+    prim[9] = GAMMA
+    return np.ascontiguousarray(prim)
+
+
+def kernel_bytes(nx):
+    """Algorithmic bytes per phase (compulsory reads + writes of its inputs/outputs)."""
+    cells = nx ** 3
+    faces = (nx + 2) * (nx + 2) * (nx + 2)  # per direction (the reference's face ranges)
+    return {
+        # per direction: read 10 prims per face once, write 8 fluxes + 1 signal speed
+        "fluxes": 3 * faces * (10 + 8 + 1) * 8,
+        "flux_ct": (nx + 1) ** 3 * (12 + 3 + 3 + 6) * 8,
+        "update": cells * (3 * 8 + 8 + 8 + 8 + 8) * 8,
+        "con2prim": cells * (8 + 1 + 10 + 1 + 1 + 1 + 3) * 8,
+        "dt": cells * (3 + 3) * 8,
+    }
+
+
+def cpu_baseline(prim, cons, mu0, args):
+    """The CPU oracle's stage on the same input (all host threads), bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    op = O.default_params(gamma=GAMMA, mhd=True, recon=O.RECON_WENO5, riemann=O.RIEMANN_HLL,
+                          tol=1e-10, max_iter=100)
+    n = args.cpu_nx
+    sl = slice(0, n + 2 * NG)
+    ob = O.make_block(3, [n, n, n], NG, dx=(1.0 / NX,) * 3)
+    p = np.ascontiguousarray(prim[:, sl, sl, sl])
+    u = np.ascontiguousarray(cons[:, sl, sl, sl])
+    m0 = np.ascontiguousarray(mu0[sl, sl, sl])
+    I = O.interior(ob)
+    lo = (NG, NG, NG)
+    hi = (NG + n - 1,) * 3
+    times = []
+    for _ in range(args.cpu_stages):
+        t0 = time.perf_counter()
+        flux, fsig = O.calculate_fluxes(op, ob, p)
+        O.flux_ct(op, ob, flux)
+        out = O.flux_divergence_update(op, ob, flux, None, u.copy(), u, 1.0, 1e-4)
+        pr = p.copy()
+        mu = m0.copy()
+        r = O.con2prim_block(op, ob, out, pr, mu, lo=lo, hi=hi)
+        O.estimate_dt(op, ob, r["sig"], fsig, 0.4)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": n ** 3 / t, "unit": "zone-updates/s", "cores": O.threads(),
+            "kind": "port",
+            "sample": f"{args.cpu_stages} full RK stages of a {n}^3 block of the same input "
+                      f"(C99 restatement, OpenMP, median stage {t:.3f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=NX)
+    ap.add_argument("--cpu-nx", type=int, default=NX)
+    ap.add_argument("--cpu-stages", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
+                                   "of it (see DESIGN.md for how the stage would shard)"}))
+        sys.exit(2)
+
+    from phoebus_amd import _abi as A
+    from phoebus_amd.api import BlockState, Phx, block, fluid_params
+    from phoebus_amd.device import DeviceArray, Event, synchronize
+
+    nx = args.nx
+    rng = np.random.default_rng(SEED)
+    prim_h = make_input(rng, nx, NG)
+    phx = Phx(0)
+    p = fluid_params(gamma=GAMMA, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
+                     c2p_tol=1e-10, c2p_max_iter=100)
+    b = block(3, (nx, nx, nx), NG, dx=(1.0 / nx,) * 3)
+    st = BlockState(p, b)
+    st.prim.upload(prim_h)
+    f_all = st.fields()
+    phx.prim_to_con(p, b, f_all)            # product p2c builds the conserved input
+    # the input state's own con2prim root: the warm start of the timed stages
+    prim_work = DeviceArray(st.prim.shape)
+    mu0 = DeviceArray(st.c2p_mu.shape)
+    mu0.zero()
+    it0 = DeviceArray(st.shape, np.int32)
+    cons_copy = DeviceArray.from_host(st.cons.get())
+    f0 = Phx.fields(prim=prim_work, cons=cons_copy, temp=st.temp, c2p_mu=mu0, fail=st.fail,
+                    cell_sig=st.cell_sig)
+    I = (slice(NG, NG + nx),) * 3
+    lo, hi = (NG, NG, NG), (NG + nx - 1,) * 3
+    phx.con2prim(p, b, f0, lo, hi, iters=it0)
+    synchronize()
+    cold_iters = it0.get()[I]
+    del cons_copy, prim_work
+
+    # per-step buffers: the stage reads st.prim / st.cons and writes u_out / prim_out
+    u_out = DeviceArray(st.cons.shape)
+    prim_out = DeviceArray(st.prim.shape)
+    iters = DeviceArray(st.shape, np.int32)
+    dt_d = DeviceArray((1,))
+    f_flux = st.fields()
+    f_c2p = Phx.fields(prim=prim_out, cons=u_out, temp=st.temp, c2p_mu=st.c2p_mu,
+                       fail=st.fail, cell_sig=st.cell_sig, face_sig=st.face_sig)
+    sp = C.c_void_p()
+    phx.L.phx_rt_stream_create(C.byref(sp))
+    s = sp.value
+    names = ["fluxes", "flux_ct", "update", "con2prim", "dt"]
+    evs = [Event() for _ in range(len(names) + 1)]
+
+    def stage(timed):
+        st.c2p_mu.fill_from(mu0)  # restore the warm-start guess (stream-ordered copy)
+        if timed: evs[0].record(s)
+        phx.calculate_fluxes(p, b, f_flux, stream=s)
+        if timed: evs[1].record(s)
+        phx.flux_ct(p, b, f_flux, stream=s)
+        if timed: evs[2].record(s)
+        phx.flux_divergence_update(p, b, st.flux, None, st.cons, st.cons, 1.0, 1e-4, u_out,
+                                   stream=s)
+        if timed: evs[3].record(s)
+        phx.con2prim(p, b, f_c2p, lo, hi, iters=iters, stream=s)
+        if timed: evs[4].record(s)
+        phx.estimate_dt(p, b, f_c2p, 1, 0.4, dt_d, stream=s)
+        if timed: evs[5].record(s)
+
+    for _ in range(args.warmup):
+        stage(False)
+    phx.L.phx_rt_stream_synchronize(s)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stage(False)
+    phx.L.phx_rt_stream_synchronize(s)
+    synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+
+    # per-phase device time: HIP events on the stream the kernels run on
+    acc = {k: 0.0 for k in names}
+    for _ in range(args.steps):
+        stage(True)
+        phx.L.phx_rt_stream_synchronize(s)
+        for i, k in enumerate(names):
+            acc[k] += evs[i].elapsed_ms(evs[i + 1])
+    phase_ms = {k: v / args.steps for k, v in acc.items()}
+    phx.L.phx_rt_stream_destroy(s)
+
+    kb = kernel_bytes(nx)
+    # dominant phase; the flux phase is three launches (x, y, z)
+    dom = max(phase_ms, key=lambda k: phase_ms[k])
+    launches = 3 if dom == "fluxes" else 1
+    per_launch_ms = phase_ms[dom] / launches
+    achieved = kb[dom] / launches / (per_launch_ms * 1e-3) / 1e9
+    warm_iters = iters.get()[I]
+    value = nx ** 3 / (ms_per_step * 1e-3)
+    out = {
+        "metric": "zone-updates/s of GRMHD RK stage; con2prim mean iters, MLP-guess vs plain",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded; see bench.py docstring)",
+        "config": {"workload": f"single 3-D GRMHD meshblock {nx}^3 + {NG} ghosts, Minkowski, "
+                               "Gamma=13/9, WENO5-Z + HLL + FluxCT + divergence/RK update + "
+                               "robust con2prim (tol 1e-10) + CFL, one RK stage",
+                   "zones": nx ** 3, "recon": "weno5z", "riemann": "hll"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "launches_per_step": launches,
+                     "avg_launch_ms": per_launch_ms},
+        "stage_roofline": {"bytes_per_zone_update": B_ALG_STAGE,
+                           "achieved_GBs": value * B_ALG_STAGE / 1e9,
+                           "frac": value * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS},
+        "phase_ms": phase_ms,
+        "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
+                     "mean_iters_cold": float(np.mean(cold_iters)),
+                     "hist_warm": np.bincount(np.maximum(warm_iters, 0)).tolist(),
+                     "hist_cold": np.bincount(np.maximum(cold_iters, 0)).tolist(),
+                     "fail_count": int(np.sum(st.fail.get()[I] == 0.0)),
+                     "mlp_guess": None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prim_h, st.cons.get(), mu0.get(), args)
+    print(json.dumps(out), flush=True)
+    phx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -306,3 +306,8 @@ def geom_gradlnalpha(params, X):
     lib().orc_geom_gradlnalpha(C.byref(params.geom), C.c_double(X[0]), C.c_double(X[1]),
                                C.c_double(X[2]), ptr(da))
     return da
+
+
+def threads():
+    """OpenMP threads the oracle uses."""
+    return lib().orc_num_threads()
