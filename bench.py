@@ -241,8 +241,8 @@ def main():
         "phase_ms": phase_ms,
         "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
                      "mean_iters_cold": float(np.mean(cold_iters)),
-                     "hist_warm": np.bincount(np.maximum(warm_iters, 0)).tolist(),
-                     "hist_cold": np.bincount(np.maximum(cold_iters, 0)).tolist(),
+                     "hist_warm": np.bincount(np.maximum(warm_iters, 0).ravel()).tolist(),
+                     "hist_cold": np.bincount(np.maximum(cold_iters, 0).ravel()).tolist(),
                      "fail_count": int(np.sum(st.fail.get()[I] == 0.0)),
                      "mlp_guess": None},
     }
