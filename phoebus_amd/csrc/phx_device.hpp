@@ -33,6 +33,23 @@ struct DevParams {
   DevBounds bnd;
 };
 
+// ---- block geometry as the kernels see it ---------------------------------------------
+struct BlockGeom {
+  int ni, nj, nk;
+  int ndim, ng;
+  int is[3], ie[3];  // interior index range per dimension (i, j, k)
+  long ncell;
+  double xmin[3], dx[3];
+  int istart[3];     // parthenon's istart: ng in active dimensions, 0 otherwise
+};
+
+__device__ __forceinline__ double xc(const BlockGeom &b, int d, int idx) {
+  return b.xmin[d] + (idx - b.istart[d] + 0.5) * b.dx[d];
+}
+__device__ __forceinline__ double xf(const BlockGeom &b, int d, int idx) {
+  return b.xmin[d] + (idx - b.istart[d]) * b.dx[d];
+}
+
 // ---- robust.hpp:10-40 -------------------------------------------------------------------
 constexpr double kSmall = 10.0 * DBL_MIN;
 constexpr double kEps = 10.0 * DBL_EPSILON;

@@ -19,99 +19,6 @@
 
 namespace phx {
 
-// ---- block geometry as the kernels see it ---------------------------------------------
-struct BlockGeom {
-  int ni, nj, nk;
-  int ndim, ng;
-  int is[3], ie[3];  // interior index range per dimension (i, j, k)
-  long ncell;
-  double xmin[3], dx[3];
-  int istart[3];     // parthenon's istart: ng in active dimensions, 0 otherwise
-};
-
-static BlockGeom make_block_geom(const phx_block *b) {
-  BlockGeom g;
-  g.ndim = b->ndim;
-  g.ng = b->ng;
-  int n[3];
-  for (int d = 0; d < 3; ++d) {
-    const bool act = d < b->ndim;
-    n[d] = b->nx[d] + (act ? 2 * b->ng : 0);
-    g.istart[d] = act ? b->ng : 0;
-    g.is[d] = g.istart[d];
-    g.ie[d] = g.istart[d] + b->nx[d] - 1;
-    g.xmin[d] = b->xmin[d];
-    g.dx[d] = b->dx[d];
-  }
-  g.ni = n[0];
-  g.nj = n[1];
-  g.nk = n[2];
-  g.ncell = (long)n[0] * n[1] * n[2];
-  return g;
-}
-
-__device__ __forceinline__ double xc(const BlockGeom &b, int d, int idx) {
-  return b.xmin[d] + (idx - b.istart[d] + 0.5) * b.dx[d];
-}
-__device__ __forceinline__ double xf(const BlockGeom &b, int d, int idx) {
-  return b.xmin[d] + (idx - b.istart[d]) * b.dx[d];
-}
-
-// ---- fused reconstruction + Riemann flux (fluid.cpp:676-770) ---------------------------
-// One lane per face.  Face (k,j,i) of direction DIR: left state = upper edge of the cell
-// below, right state = lower edge of cell (k,j,i); six cells of each primitive along DIR.
-template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(256) void k_fluxes(DevParams p, BlockGeom b,
-                                                const double *__restrict__ prim,
-                                                double *__restrict__ flux,
-                                                double *__restrict__ fsig) {
-  using M = typename MetricOf<GEOM>::type;
-  constexpr int NP = Layout<MHD>::nprim, NC = Layout<MHD>::ncons;
-  constexpr int H = RECON == PHX_RECON_CONSTANT ? 0 : RECON == PHX_RECON_LINEAR ? 1 : 2;
-  const int dj = b.ndim > 1, dk = b.ndim > 2;
-  const int fi = b.ie[0] - b.is[0] + 3;
-  const int fj = b.ie[1] - b.is[1] + 1 + 2 * dj;
-  const int fk = b.ie[2] - b.is[2] + 1 + 2 * dk;
-  const long nface = (long)fi * fj * fk;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nface) return;
-  const int i = (int)(t % fi) + b.is[0] - 1;
-  const long r = t / fi;
-  const int j = (int)(r % fj) + b.is[1] - dj;
-  const int k = (int)(r / fj) + b.is[2] - dk;
-  const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long st = DIR == 0 ? 1 : DIR == 1 ? (long)b.ni : (long)b.ni * b.nj;
-  const int own = DIR == 0 ? i : DIR == 1 ? j : k;
-  // the lowest face's left state is never consumed (the reference reads unwritten ql
-  // there); use the right state when its stencil would leave the block
-  const bool left_ok = own - 1 - H >= 0;
-
-  // cells c-(H+1)*st .. c+H*st: the left edge state uses the first 2H+1 of them, the
-  // right edge state the last 2H+1; nothing outside the block is read
-  double ql[NP], qr[NP];
-#pragma unroll
-  for (int v = 0; v < NP; ++v) {
-    const double *pv = prim + v * b.ncell + c;
-    double q[2 * H + 2];
-#pragma unroll
-    for (int o = 1; o < 2 * H + 2; ++o) q[o] = pv[(o - H - 1) * st];
-    q[0] = left_ok ? pv[-(H + 1) * st] : q[1];
-    double lo, hi;
-    reconstruct_edges<RECON>(q + 1, lo, hi);
-    qr[v] = hi;  // lower edge of cell (k,j,i)
-    reconstruct_edges<RECON>(q, lo, hi);
-    ql[v] = left_ok ? lo : qr[v];  // upper edge of the cell below
-  }
-  double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  X[DIR] = xf(b, DIR, own);
-  const M g(p.a, X[0], X[1], X[2]);
-  double f[NC];
-  const double s = riemann_flux<MHD, RIEMANN>(p, DIR, g, X, ql, qr, f);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) flux[((long)DIR * NC + m) * b.ncell + c] = f[m];
-  fsig[(long)DIR * b.ncell + c] = s;
-}
-
 // ---- FluxCT (fluid.cpp:775-841) --------------------------------------------------------
 template <int NC>
 __global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, const double *__restrict__ flux,
@@ -609,65 +516,12 @@ int check_block(phx_ctx *ctx, const phx_block *b, int recon) {
     if (d >= b->ndim && b->nx[d] != 1)
       return ctx->fail(PHX_ERR_INVALID, "nx must be 1 in inactive dimensions");
   }
-  const int need = recon >= PHX_RECON_WENO5Z ? 4 : recon == PHX_RECON_LINEAR ? 2 : 1;
+  const int need = recon >= PHX_RECON_WENO5Z ? 4 : 2;
   if (b->ng < need) return ctx->fail(PHX_ERR_INVALID, "not enough ghost zones for the reconstruction");
   long cells = 1;
   for (int d = 0; d < 3; ++d) cells *= b->nx[d] + (d < b->ndim ? 2 * b->ng : 0);
   if (cells > (1L << 31)) return ctx->fail(PHX_ERR_INVALID, "block too large");
   return PHX_OK;
-}
-
-// dispatch helpers ----------------------------------------------------------------------
-template <int DIR, int RECON, int RIEMANN, bool MHD>
-static void launch_flux_geom(int geom, const DevParams &dp, const BlockGeom &bg,
-                             const double *prim, double *flux, double *fsig, long nface,
-                             hipStream_t s) {
-  if (geom == PHX_GEOM_MINKOWSKI)
-    hipLaunchKernelGGL((k_fluxes<DIR, RECON, RIEMANN, MHD, PHX_GEOM_MINKOWSKI>),
-                       dim3(nblocks(nface, 256)), dim3(256), 0, s, dp, bg, prim, flux, fsig);
-  else
-    hipLaunchKernelGGL((k_fluxes<DIR, RECON, RIEMANN, MHD, PHX_GEOM_SPHERICAL_KS>),
-                       dim3(nblocks(nface, 256)), dim3(256), 0, s, dp, bg, prim, flux, fsig);
-}
-
-template <int DIR, int RECON, int RIEMANN>
-static void launch_flux_mhd(bool mhd, int geom, const DevParams &dp, const BlockGeom &bg,
-                            const double *prim, double *flux, double *fsig, long nface,
-                            hipStream_t s) {
-  if (mhd)
-    launch_flux_geom<DIR, RECON, RIEMANN, true>(geom, dp, bg, prim, flux, fsig, nface, s);
-  else
-    launch_flux_geom<DIR, RECON, RIEMANN, false>(geom, dp, bg, prim, flux, fsig, nface, s);
-}
-
-template <int DIR, int RECON>
-static void launch_flux_riemann(int riemann, bool mhd, int geom, const DevParams &dp,
-                                const BlockGeom &bg, const double *prim, double *flux,
-                                double *fsig, long nface, hipStream_t s) {
-  if (riemann == PHX_RIEMANN_HLL)
-    launch_flux_mhd<DIR, RECON, PHX_RIEMANN_HLL>(mhd, geom, dp, bg, prim, flux, fsig, nface, s);
-  else
-    launch_flux_mhd<DIR, RECON, PHX_RIEMANN_LLF>(mhd, geom, dp, bg, prim, flux, fsig, nface, s);
-}
-
-template <int DIR>
-static void launch_flux_dir(int recon, int riemann, bool mhd, int geom, const DevParams &dp,
-                            const BlockGeom &bg, const double *prim, double *flux, double *fsig,
-                            long nface, hipStream_t s) {
-  switch (recon) {
-  case PHX_RECON_CONSTANT:
-    launch_flux_riemann<DIR, PHX_RECON_CONSTANT>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
-    break;
-  case PHX_RECON_LINEAR:
-    launch_flux_riemann<DIR, PHX_RECON_LINEAR>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
-    break;
-  case PHX_RECON_WENO5Z:
-    launch_flux_riemann<DIR, PHX_RECON_WENO5Z>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
-    break;
-  default:
-    launch_flux_riemann<DIR, PHX_RECON_MP5>(riemann, mhd, geom, dp, bg, prim, flux, fsig, nface, s);
-    break;
-  }
 }
 
 }  // namespace phx
@@ -689,14 +543,10 @@ int phx_calculate_fluxes(phx_ctx *ctx, const phx_fluid_params *p, const phx_bloc
   hipStream_t s = ctx->stream_of(stream);
   const long nface = (long)(b->nx[0] + 2) * (b->nx[1] + (b->ndim > 1 ? 2 : 0)) *
                      (b->nx[2] + (b->ndim > 2 ? 2 : 0));
-  for (int d = 0; d < b->ndim; ++d) {
-    if (d == 0)
-      launch_flux_dir<0>(p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux, f->face_sig, nface, s);
-    else if (d == 1)
-      launch_flux_dir<1>(p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux, f->face_sig, nface, s);
-    else
-      launch_flux_dir<2>(p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux, f->face_sig, nface, s);
-  }
+  (void)nface;
+  for (int d = 0; d < b->ndim; ++d)
+    launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
+                  f->face_sig, s);
   return ctx->check_launch();
 }
 
