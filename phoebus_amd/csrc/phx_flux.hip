@@ -4,10 +4,10 @@
 // kernel reads them back.  Here each cell is reconstructed once per direction in
 // registers and the face states never reach memory:
 //
-//  * x (unit stride): one lane per cell along a row; the cell's upper-edge state is handed
-//    to the next lane with a wavefront shuffle, so lane l forms face i from the upper edge
-//    of cell i-1 (lane l-1) and the lower edge of cell i (its own).  Wavefronts overlap by
-//    one cell.
+//  * x (unit stride): one lane per cell along a row; primitives are staged through LDS
+//    once per variable and the cell's upper-edge state is published in LDS, so lane l forms
+//    face i from the upper edge of cell i-1 (lane l-1) and the lower edge of cell i (its
+//    own).  Blocks overlap by 2H+1 cells.
 //  * y, z (strided): lanes run along i (coalesced) and each lane marches a segment of its
 //    column, sliding a register window of 2H+1 cells, carrying the previous cell's upper
 //    edge from one step to the next.
@@ -27,6 +27,11 @@ struct ReconWidth {
   static constexpr int H = RECON == PHX_RECON_CONSTANT ? 0 : RECON == PHX_RECON_LINEAR ? 1 : 2;
 };
 
+// Materialize a value in a register at this point.  Without it the compiler sinks the
+// reconstruction arithmetic into the Riemann solver and keeps every stencil operand alive,
+// which more than doubles the register footprint of the kernel.
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+
 template <bool MHD, int RIEMANN, class M>
 __device__ __forceinline__ void face_out(const DevParams &p, int dir, const BlockGeom &b,
                                          const double X[3], const double *ql, const double *qr,
@@ -41,48 +46,58 @@ __device__ __forceinline__ void face_out(const DevParams &p, int dir, const Bloc
   fsig[(long)dir * b.ncell + c] = s;
 }
 
-// ---- x: one lane per cell, upper edges passed one lane up --------------------------------
+// ---- x: one lane per cell of full rows, stencils and upper edges staged in LDS ----------
+// A block of XB lanes covers XB consecutive cells of the rows it spans; lanes H+1 ..
+// XB-H-1 emit the face at their cell's lower edge, so consecutive blocks overlap by 2H+1
+// cells.  Each primitive is read from HBM once per lane into LDS, reconstructed from LDS,
+// and the cell's upper edge is published in LDS for the next lane.
+constexpr int XB = 256;
+
 template <int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(256) void k_flux_x(DevParams p, BlockGeom b,
-                                                const double *__restrict__ prim,
-                                                double *__restrict__ flux,
-                                                double *__restrict__ fsig) {
+__global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
+                                                   const double *__restrict__ prim,
+                                                   double *__restrict__ flux,
+                                                   double *__restrict__ fsig) {
   using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
+  __shared__ double stencil[XB];
+  __shared__ double upper[NP][XB];
   const int dj = b.ndim > 1, dk = b.ndim > 2;
-  const int nrc = b.ie[0] - b.is[0] + 4;  // cells is-2 .. ie+1 of a row
   const int fj = b.ie[1] - b.is[1] + 1 + 2 * dj;
   const int fk = b.ie[2] - b.is[2] + 1 + 2 * dk;
-  const long total = (long)nrc * fj * fk;
-  const int lane = threadIdx.x & 63;
-  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  long L = wave * 63 + lane;
+  const long total = (long)b.ni * fj * fk;  // full rows of the face range
+  const int t = threadIdx.x;
+  long L = (long)blockIdx.x * (XB - 2 * H - 1) + t;
   const bool valid = L < total;
-  if (!valid) L = total - 1;  // keep every lane for the shuffle; loads stay in bounds
-  const int ip = (int)(L % nrc);
-  const long row = L / nrc;
-  const int i = ip + b.is[0] - 2;
+  if (!valid) L = total - 1;
+  const int i = (int)(L % b.ni);
+  const long row = L / b.ni;
   const int j = (int)(row % fj) + b.is[1] - dj;
   const int k = (int)(row / fj) + b.is[2] - dk;
   const long c = ((long)k * b.nj + j) * b.ni + i;
-  // the first cell of a row only feeds the lowest face, which the stage never consumes;
-  // when its stencil would leave the block (ng < H + 2) it is reconstructed flat
-  const bool in_block = i - H >= 0;
+  // cells whose stencil lies in this block and in this row
+  const bool recon_ok = t >= H && t < XB - H && i >= H && i < b.ni - H;
 
-  double up[NP], lo[NP];
+  double lo[NP];
 #pragma unroll
   for (int v = 0; v < NP; ++v) {
-    const double *pv = prim + v * b.ncell + c;
+    stencil[t] = prim[v * b.ncell + c];
+    __syncthreads();
     double q[2 * H + 1];
 #pragma unroll
-    for (int o = 0; o < 2 * H + 1; ++o) q[o] = in_block ? pv[o - H] : pv[0];
-    reconstruct_edges<RECON>(q, up[v], lo[v]);
+    for (int o = 0; o < 2 * H + 1; ++o) q[o] = stencil[recon_ok ? t + o - H : t];
+    double up;
+    reconstruct_edges<RECON>(q, up, lo[v]);
+    upper[v][t] = up;
+    pin(lo[v]);
+    __syncthreads();
   }
+  const bool emit = valid && t >= H + 1 && t < XB - H && i >= b.is[0] - 1 && i <= b.ie[0] + 1;
+  if (!emit) return;
   double ql[NP];
 #pragma unroll
-  for (int v = 0; v < NP; ++v) ql[v] = __shfl_up(up[v], 1, 64);
-  if (!valid || lane == 0 || ip == 0) return;
+  for (int v = 0; v < NP; ++v) ql[v] = upper[v][t - 1];
   const double X[3] = {xf(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
   face_out<MHD, RIEMANN, M>(p, 0, b, X, ql, lo, c, flux, fsig);
 }
@@ -145,6 +160,8 @@ __global__ __launch_bounds__(256) void k_flux_march(DevParams p, BlockGeom b,
       } else {
         reconstruct_edges<RECON>(q[v], up[v], lo[v]);
       }
+      pin(up[v]);
+      pin(lo[v]);
     }
     if (cell >= f0) {
       double X[3];
@@ -175,11 +192,12 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
                        double *flux, double *fsig, hipStream_t s) {
   const int dj = bg.ndim > 1, dk = bg.ndim > 2;
   if (d == 0) {
-    const long total = (long)(bg.ie[0] - bg.is[0] + 4) * (bg.ie[1] - bg.is[1] + 1 + 2 * dj) *
+    constexpr int H = ReconWidth<RECON>::H;
+    const long total = (long)bg.ni * (bg.ie[1] - bg.is[1] + 1 + 2 * dj) *
                        (bg.ie[2] - bg.is[2] + 1 + 2 * dk);
-    const long waves = (total + 62) / 63;
-    const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
-    hipLaunchKernelGGL((k_flux_x<RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(256), 0, s, dp,
+    const long stride = XB - 2 * H - 1;
+    const unsigned blocks = (unsigned)((total - 2 * H - 1 + stride - 1) / stride);
+    hipLaunchKernelGGL((k_flux_x<RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(XB), 0, s, dp,
                        bg, prim, flux, fsig);
     return;
   }
