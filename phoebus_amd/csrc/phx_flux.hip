@@ -8,9 +8,8 @@
 //    once per variable and the cell's upper-edge state is published in LDS, so lane l forms
 //    face i from the upper edge of cell i-1 (lane l-1) and the lower edge of cell i (its
 //    own).  Blocks overlap by 2H+1 cells.
-//  * y, z (strided): lanes run along i (coalesced) and each lane marches a segment of its
-//    column, sliding a register window of 2H+1 cells, carrying the previous cell's upper
-//    edge from one step to the next.
+//  * y, z (strided): tiles of 16 (i) x 32 (own direction) cells; rows of 16 consecutive i
+//    are read coalesced into LDS and each lane reconstructs along its column from LDS.
 //
 // Results are bit-identical to reconstructing per face: the same arithmetic runs on the
 // same operands.
@@ -102,91 +101,70 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
   face_out<MHD, RIEMANN, M>(p, 0, b, X, ql, lo, c, flux, fsig);
 }
 
-// ---- y, z: lanes along i, each marching a segment of its column --------------------------
-template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM, int LS>
-__global__ __launch_bounds__(256) void k_flux_march(DevParams p, BlockGeom b,
-                                                    const double *__restrict__ prim,
-                                                    double *__restrict__ flux,
-                                                    double *__restrict__ fsig) {
+// ---- y, z: LDS tiles of TI (i) x TS (own direction) cells ----------------------------------
+// Rows of TI consecutive i are read coalesced into LDS; each lane reconstructs its cell
+// along the own direction from LDS and publishes the upper edge for the lane one row up.
+// Lanes with own-row H+1 .. TS-H-1 emit faces; tiles overlap by 2H+1 cells.
+constexpr int TI = 16, TS = 32;
+
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
+__global__ __launch_bounds__(TI * TS) void k_flux_tile(DevParams p, BlockGeom b,
+                                                       const double *__restrict__ prim,
+                                                       double *__restrict__ flux,
+                                                       double *__restrict__ fsig) {
   using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
-  constexpr int W = 2 * H + 1;
-  // transverse dimension T: z for DIR = 1, y for DIR = 2
-  constexpr int T = DIR == 1 ? 2 : 1;
+  constexpr int S = TS - 2 * H - 1;  // faces per tile
+  constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
+  __shared__ double stencil[TS][TI];
+  __shared__ double upper[NP][TS][TI];
+  const int ti = threadIdx.x % TI, ts = threadIdx.x / TI;
   const int dt_ = b.ndim > T;
-  const int fi = b.ie[0] - b.is[0] + 3;                    // i in [is-1, ie+1]
-  const int ft = b.ie[T] - b.is[T] + 1 + 2 * dt_;          // t in [ts-1, te+1]
-  const int nfo = b.ie[DIR] - b.is[DIR] + 3;               // own faces [s-1, e+1]
-  const int nseg = (nfo + LS - 1) / LS;
-  const long total = (long)fi * ft * nseg;
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const int i = (int)(g % fi) + b.is[0] - 1;
-  const long r = g / fi;
-  const int t = (int)(r % ft) + b.is[T] - dt_;
-  const int seg = (int)(r / ft);
-  const int f0 = b.is[DIR] - 1 + seg * LS;
-  const int f1 = min(f0 + LS, b.ie[DIR] + 2);  // faces [f0, f1)
-  const long st = DIR == 1 ? (long)b.ni : (long)b.ni * b.nj;
-  const int j0 = DIR == 1 ? 0 : t, k0 = DIR == 1 ? t : 0;
-  // cell index of (own = 0) in this column
-  const long base = DIR == 1 ? ((long)k0 * b.nj) * b.ni + i : (long)j0 * b.ni + i;
+  const int nic = (b.ie[0] - b.is[0] + 3 + TI - 1) / TI;   // i chunks over [is-1, ie+1]
+  const int nft = (b.ie[DIR] - b.is[DIR] + 3 + S - 1) / S;  // tiles over faces [s-1, e+1]
+  const int bx = blockIdx.x;
+  const int ichunk = bx % nic;
+  const int tile = (bx / nic) % nft;
+  const int tr = bx / (nic * nft);
+  const int nown = DIR == 1 ? b.nj : b.nk;
+  const int i_raw = b.is[0] - 1 + ichunk * TI + ti;
+  const int o_raw = b.is[DIR] - 2 - H + tile * S + ts;
+  const int t = tr + b.is[T] - dt_;
+  const int i = min(i_raw, b.ie[0] + 1);
+  const int o = min(max(o_raw, 0), nown - 1);
+  const int j = DIR == 1 ? o : t, k = DIR == 1 ? t : o;
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const bool recon_ok = ts >= H && ts < TS - H && o_raw - H >= 0 && o_raw + H <= nown - 1;
 
-  // window of cells cell-H .. cell+H for the first cell f0-1
-  double q[NP][W];
-  const int cfirst = f0 - 1;
+  double lo[NP];
 #pragma unroll
   for (int v = 0; v < NP; ++v) {
-    const double *pv = prim + v * b.ncell + base;
+    stencil[ts][ti] = prim[v * b.ncell + c];
+    __syncthreads();
+    double q[2 * H + 1];
 #pragma unroll
-    for (int o = 0; o < W; ++o) {
-      const int cc = cfirst - H + o;
-      q[v][o] = pv[(long)(cc < 0 ? cfirst : cc) * st];
-    }
+    for (int oo = 0; oo < 2 * H + 1; ++oo) q[oo] = stencil[recon_ok ? ts + oo - H : ts][ti];
+    double up;
+    reconstruct_edges<RECON>(q, up, lo[v]);
+    upper[v][ts][ti] = up;
+    pin(lo[v]);
+    __syncthreads();
   }
-  double upp[NP];
-  const double xt = xc(b, T, t), xi = xc(b, 0, i);
-  for (int cell = cfirst; cell < f1; ++cell) {
-    double up[NP], lo[NP];
-    const bool flat = cell - H < 0;  // only the unconsumed lowest face when ng < H + 2
+  const bool emit = ts >= H + 1 && ts < TS - H && i_raw <= b.ie[0] + 1 &&
+                    o_raw >= b.is[DIR] - 1 && o_raw <= b.ie[DIR] + 1;
+  if (!emit) return;
+  double ql[NP];
 #pragma unroll
-    for (int v = 0; v < NP; ++v) {
-      if (flat) {
-        double qq[W];
-#pragma unroll
-        for (int o = 0; o < W; ++o) qq[o] = q[v][H];
-        reconstruct_edges<RECON>(qq, up[v], lo[v]);
-      } else {
-        reconstruct_edges<RECON>(q[v], up[v], lo[v]);
-      }
-      pin(up[v]);
-      pin(lo[v]);
-    }
-    if (cell >= f0) {
-      double X[3];
-      X[0] = xi;
-      X[T] = xt;
-      X[DIR] = xf(b, DIR, cell);
-      face_out<MHD, RIEMANN, M>(p, DIR, b, X, upp, lo, base + (long)cell * st, flux, fsig);
-    }
-#pragma unroll
-    for (int v = 0; v < NP; ++v) upp[v] = up[v];
-    if (cell + 1 < f1) {
-      const int nxt = cell + 1 + H;  // <= ie + 1 + H, inside the block for ng >= H + 1
-#pragma unroll
-      for (int v = 0; v < NP; ++v) {
-#pragma unroll
-        for (int o = 0; o < W - 1; ++o) q[v][o] = q[v][o + 1];
-        q[v][W - 1] = prim[v * b.ncell + base + (long)nxt * st];
-      }
-    }
-  }
+  for (int v = 0; v < NP; ++v) ql[v] = upper[v][ts - 1][ti];
+  double X[3];
+  X[0] = xc(b, 0, i);
+  X[T] = xc(b, T, t);
+  X[DIR] = xf(b, DIR, o);
+  face_out<MHD, RIEMANN, M>(p, DIR, b, X, ql, lo, c, flux, fsig);
 }
 
 // ---- dispatch --------------------------------------------------------------------------
-constexpr int kSegment = 16;
-
 template <int RECON, int RIEMANN, bool MHD, int GEOM>
 static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const double *prim,
                        double *flux, double *fsig, hipStream_t s) {
@@ -201,19 +179,20 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
                        bg, prim, flux, fsig);
     return;
   }
+  constexpr int H = ReconWidth<RECON>::H;
+  constexpr int S = TS - 2 * H - 1;
   const int T = d == 1 ? 2 : 1;
   const int dt_ = bg.ndim > T;
-  const long fi = bg.ie[0] - bg.is[0] + 3;
-  const long ft = bg.ie[T] - bg.is[T] + 1 + 2 * dt_;
-  const long nseg = (bg.ie[d] - bg.is[d] + 3 + kSegment - 1) / kSegment;
-  const long total = fi * ft * nseg;
-  const unsigned blocks = (unsigned)((total + 255) / 256);
+  const long nic = (bg.ie[0] - bg.is[0] + 3 + TI - 1) / TI;
+  const long nft = (bg.ie[d] - bg.is[d] + 3 + S - 1) / S;
+  const long ntr = bg.ie[T] - bg.is[T] + 1 + 2 * dt_;
+  const unsigned blocks = (unsigned)(nic * nft * ntr);
   if (d == 1)
-    hipLaunchKernelGGL((k_flux_march<1, RECON, RIEMANN, MHD, GEOM, kSegment>), dim3(blocks),
-                       dim3(256), 0, s, dp, bg, prim, flux, fsig);
+    hipLaunchKernelGGL((k_flux_tile<1, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(TI * TS),
+                       0, s, dp, bg, prim, flux, fsig);
   else
-    hipLaunchKernelGGL((k_flux_march<2, RECON, RIEMANN, MHD, GEOM, kSegment>), dim3(blocks),
-                       dim3(256), 0, s, dp, bg, prim, flux, fsig);
+    hipLaunchKernelGGL((k_flux_tile<2, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(TI * TS),
+                       0, s, dp, bg, prim, flux, fsig);
 }
 
 template <int RECON, int RIEMANN, bool MHD>
