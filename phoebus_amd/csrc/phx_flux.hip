@@ -8,8 +8,8 @@
 //    once per variable and the cell's upper-edge state is published in LDS, so lane l forms
 //    face i from the upper edge of cell i-1 (lane l-1) and the lower edge of cell i (its
 //    own).  Blocks overlap by 2H+1 cells.
-//  * y, z (strided): tiles of 16 (i) x 32 (own direction) cells; rows of 16 consecutive i
-//    are read coalesced into LDS and each lane reconstructs along its column from LDS.
+//  * y, z (strided): tiles spanning the whole own-direction extent for a few consecutive i;
+//    rows are read into LDS and each lane reconstructs along its column from LDS.
 //
 // Results are bit-identical to reconstructing per face: the same arithmetic runs on the
 // same operands.
@@ -101,62 +101,73 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
   face_out<MHD, RIEMANN, M>(p, 0, b, X, ql, lo, c, flux, fsig);
 }
 
-// ---- y, z: LDS tiles of TI (i) x TS (own direction) cells ----------------------------------
-// Rows of TI consecutive i are read coalesced into LDS; each lane reconstructs its cell
-// along the own direction from LDS and publishes the upper edge for the lane one row up.
-// Lanes with own-row H+1 .. TS-H-1 emit faces; tiles overlap by 2H+1 cells.
-constexpr int TI = 16, TS = 32;
+// ---- y, z: LDS tiles of ti_n (i) x ts_n (own direction) cells ----------------------------
+// Rows of ti_n consecutive i are read into LDS; each lane reconstructs its cell along the
+// own direction from LDS and publishes the upper edge for the lane one row up.  Lanes in
+// own-rows H+1 .. ts_n-H-1 emit faces; tiles along the own direction overlap by 2H+1
+// cells.  The host sizes ts_n to the whole own extent of the face range when it fits
+// (135 cells for a 128-zone WENO5 block), which keeps 92% of the lanes emitting; the
+// kernel is FP64-issue-bound, so the narrow rows cost little in load efficiency.
+constexpr int kTileThreads = 1024;
+constexpr int kTileMaxRows = 256;
 
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(TI * TS) void k_flux_tile(DevParams p, BlockGeom b,
-                                                       const double *__restrict__ prim,
-                                                       double *__restrict__ flux,
-                                                       double *__restrict__ fsig) {
+__global__ __launch_bounds__(kTileThreads) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
+                                                            int ts_n,
+                                                            const double *__restrict__ prim,
+                                                            double *__restrict__ flux,
+                                                            double *__restrict__ fsig) {
   using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
-  constexpr int S = TS - 2 * H - 1;  // faces per tile
   constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
-  __shared__ double stencil[TS][TI];
-  __shared__ double upper[NP][TS][TI];
-  const int ti = threadIdx.x % TI, ts = threadIdx.x / TI;
+  extern __shared__ double lds[];
+  double *stencil = lds;                     // [ts_n][ti_n]
+  double *upper = lds + ts_n * ti_n;         // [NP][ts_n][ti_n]
+  const int S = ts_n - 2 * H - 1;            // faces per tile
+  const int tid = threadIdx.x;
+  const int ti = tid % ti_n, ts = tid / ti_n;
+  const bool live = ts < ts_n;                // threads beyond the tile only join barriers
   const int dt_ = b.ndim > T;
-  const int nic = (b.ie[0] - b.is[0] + 3 + TI - 1) / TI;   // i chunks over [is-1, ie+1]
-  const int nft = (b.ie[DIR] - b.is[DIR] + 3 + S - 1) / S;  // tiles over faces [s-1, e+1]
+  const int nic = (b.ie[0] - b.is[0] + 3 + ti_n - 1) / ti_n;  // i chunks over [is-1, ie+1]
+  const int nft = (b.ie[DIR] - b.is[DIR] + 3 + S - 1) / S;    // tiles over faces [s-1, e+1]
   const int bx = blockIdx.x;
   const int ichunk = bx % nic;
   const int tile = (bx / nic) % nft;
   const int tr = bx / (nic * nft);
   const int nown = DIR == 1 ? b.nj : b.nk;
-  const int i_raw = b.is[0] - 1 + ichunk * TI + ti;
+  const int i_raw = b.is[0] - 1 + ichunk * ti_n + ti;
   const int o_raw = b.is[DIR] - 2 - H + tile * S + ts;
   const int t = tr + b.is[T] - dt_;
   const int i = min(i_raw, b.ie[0] + 1);
   const int o = min(max(o_raw, 0), nown - 1);
   const int j = DIR == 1 ? o : t, k = DIR == 1 ? t : o;
   const long c = ((long)k * b.nj + j) * b.ni + i;
-  const bool recon_ok = ts >= H && ts < TS - H && o_raw - H >= 0 && o_raw + H <= nown - 1;
+  const bool recon_ok =
+      live && ts >= H && ts < ts_n - H && o_raw - H >= 0 && o_raw + H <= nown - 1;
+  const int me = ts * ti_n + ti;
 
   double lo[NP];
 #pragma unroll
   for (int v = 0; v < NP; ++v) {
-    stencil[ts][ti] = prim[v * b.ncell + c];
+    if (live) stencil[me] = prim[v * b.ncell + c];
     __syncthreads();
     double q[2 * H + 1];
 #pragma unroll
-    for (int oo = 0; oo < 2 * H + 1; ++oo) q[oo] = stencil[recon_ok ? ts + oo - H : ts][ti];
+    for (int oo = 0; oo < 2 * H + 1; ++oo)
+      q[oo] = live ? stencil[recon_ok ? me + (oo - H) * ti_n : me] : 0.0;
     double up;
     reconstruct_edges<RECON>(q, up, lo[v]);
-    upper[v][ts][ti] = up;
+    if (live) upper[(v * ts_n) * ti_n + me] = up;
     pin(lo[v]);
     __syncthreads();
   }
-  const bool emit = ts >= H + 1 && ts < TS - H && i_raw <= b.ie[0] + 1 &&
+  const bool emit = live && ts >= H + 1 && ts < ts_n - H && i_raw <= b.ie[0] + 1 &&
                     o_raw >= b.is[DIR] - 1 && o_raw <= b.ie[DIR] + 1;
   if (!emit) return;
   double ql[NP];
 #pragma unroll
-  for (int v = 0; v < NP; ++v) ql[v] = upper[v][ts - 1][ti];
+  for (int v = 0; v < NP; ++v) ql[v] = upper[(v * ts_n) * ti_n + me - ti_n];
   double X[3];
   X[0] = xc(b, 0, i);
   X[T] = xc(b, T, t);
@@ -180,19 +191,26 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
     return;
   }
   constexpr int H = ReconWidth<RECON>::H;
-  constexpr int S = TS - 2 * H - 1;
+  constexpr int NP = Layout<MHD>::nprim;
   const int T = d == 1 ? 2 : 1;
   const int dt_ = bg.ndim > T;
-  const long nic = (bg.ie[0] - bg.is[0] + 3 + TI - 1) / TI;
+  // own-direction cells the face range needs: s-2-H .. e+1+H
+  const int need = bg.ie[d] - bg.is[d] + 4 + 2 * H;
+  const int ts_n = need <= kTileMaxRows ? need : kTileMaxRows;
+  const int ti_n = kTileThreads / ts_n < 16 ? kTileThreads / ts_n : 16;
+  const int S = ts_n - 2 * H - 1;
+  const long nic = (bg.ie[0] - bg.is[0] + 3 + ti_n - 1) / ti_n;
   const long nft = (bg.ie[d] - bg.is[d] + 3 + S - 1) / S;
   const long ntr = bg.ie[T] - bg.is[T] + 1 + 2 * dt_;
   const unsigned blocks = (unsigned)(nic * nft * ntr);
+  const int threads = ((ti_n * ts_n + 63) / 64) * 64;
+  const size_t lds = sizeof(double) * (size_t)(NP + 1) * ts_n * ti_n;
   if (d == 1)
-    hipLaunchKernelGGL((k_flux_tile<1, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(TI * TS),
-                       0, s, dp, bg, prim, flux, fsig);
+    hipLaunchKernelGGL((k_flux_tile<1, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(threads),
+                       lds, s, dp, bg, ti_n, ts_n, prim, flux, fsig);
   else
-    hipLaunchKernelGGL((k_flux_tile<2, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(TI * TS),
-                       0, s, dp, bg, prim, flux, fsig);
+    hipLaunchKernelGGL((k_flux_tile<2, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(threads),
+                       lds, s, dp, bg, ti_n, ts_n, prim, flux, fsig);
 }
 
 template <int RECON, int RIEMANN, bool MHD>
