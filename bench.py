@@ -63,16 +63,19 @@ def make_input(rng, n_int, ng):
 
 
 def kernel_bytes(nx):
-    """Algorithmic bytes per phase (compulsory reads + writes of its inputs/outputs)."""
+    """Algorithmic bytes per launch (each input read once, each output written once)."""
     cells = nx ** 3
-    faces = (nx + 2) * (nx + 2) * (nx + 2)  # per direction (the reference's face ranges)
+    faces = (nx + 2) ** 3  # per direction (the reference's face ranges)
+    flux_dir = faces * (10 + 8 + 1) * 8  # 10 primitives in, 8 fluxes + 1 speed out
     return {
-        # per direction: read 10 prims per face once, write 8 fluxes + 1 signal speed
-        "fluxes": 3 * faces * (10 + 8 + 1) * 8,
-        "flux_ct": (nx + 1) ** 3 * (12 + 3 + 3 + 6) * 8,
-        "update": cells * (3 * 8 + 8 + 8 + 8 + 8) * 8,
-        "con2prim": cells * (8 + 1 + 10 + 1 + 1 + 1 + 3) * 8,
-        "dt": cells * (3 + 3) * 8,
+        "flux_x": flux_dir, "flux_y": flux_dir, "flux_z": flux_dir,
+        # 6 transverse B fluxes in, 3 EMFs out and back in, 9 B fluxes out
+        "flux_ct": (nx + 1) ** 3 * (6 + 3 + 3 + 9) * 8,
+        # 24 face fluxes, U_s, U^n, mu, 3 face speeds in; U_s, U_{s+1}, 10 prims, T, mu,
+        # fail, 3 cell speeds, int32 iteration count out
+        "update_con2prim": cells * ((24 + 8 + 8 + 1 + 3) + (8 + 8 + 10 + 1 + 1 + 1 + 3)) * 8
+                           + cells * 4,
+        "dt_finish": 8 * ((cells + 255) // 256),
     }
 
 
@@ -160,58 +163,53 @@ def main():
     prim_out = DeviceArray(st.prim.shape)
     iters = DeviceArray(st.shape, np.int32)
     dt_d = DeviceArray((1,))
-    f_flux = st.fields()
-    f_c2p = Phx.fields(prim=prim_out, cons=u_out, temp=st.temp, c2p_mu=st.c2p_mu,
-                       fail=st.fail, cell_sig=st.cell_sig, face_sig=st.face_sig)
+    f_stage = st.fields()
     sp = C.c_void_p()
     phx.L.phx_rt_stream_create(C.byref(sp))
     s = sp.value
-    names = ["fluxes", "flux_ct", "update", "con2prim", "dt"]
-    evs = [Event() for _ in range(len(names) + 1)]
 
-    def stage(timed):
-        st.c2p_mu.fill_from(mu0)  # restore the warm-start guess (stream-ordered copy)
-        if timed: evs[0].record(s)
-        phx.calculate_fluxes(p, b, f_flux, stream=s)
-        if timed: evs[1].record(s)
-        phx.flux_ct(p, b, f_flux, stream=s)
-        if timed: evs[2].record(s)
-        phx.flux_divergence_update(p, b, st.flux, None, st.cons, st.cons, 1.0, 1e-4, u_out,
-                                   stream=s)
-        if timed: evs[3].record(s)
-        phx.con2prim(p, b, f_c2p, lo, hi, iters=iters, stream=s)
-        if timed: evs[4].record(s)
-        phx.estimate_dt(p, b, f_c2p, 1, 0.4, dt_d, stream=s)
-        if timed: evs[5].record(s)
+    def stage():
+        # the warm-start guess of each repeated stage is the input state's own root
+        phx.L.phx_rt_memcpy_dtod(st.c2p_mu.ptr, mu0.ptr, mu0.nbytes)
+        phx.fluid_stage(p, b, f_stage, 1.0, 1e-4, st.cons, u_out, prim_out, iters=iters,
+                        cfl=0.4, dt_out=dt_d, stream=s)
 
     for _ in range(args.warmup):
-        stage(False)
+        stage()
     phx.L.phx_rt_stream_synchronize(s)
     synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stage(False)
+        stage()
     phx.L.phx_rt_stream_synchronize(s)
     synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
 
-    # per-phase device time: HIP events on the stream the kernels run on
-    acc = {k: 0.0 for k in names}
+    # per-launch device time: HIP events the library records around each launch of the
+    # stage, on the stream the launch uses
+    phx.L.phx_rt_profile_start(phx.h)
+    acc, cnt = {}, {}
     for _ in range(args.steps):
-        stage(True)
+        phx.L.phx_rt_profile_reset(phx.h)
+        stage()
         phx.L.phx_rt_stream_synchronize(s)
-        for i, k in enumerate(names):
-            acc[k] += evs[i].elapsed_ms(evs[i + 1])
-    phase_ms = {k: v / args.steps for k, v in acc.items()}
+        n = C.c_int()
+        phx.L.phx_rt_profile_count(phx.h, C.byref(n))
+        for r in range(n.value):
+            name = C.create_string_buffer(64)
+            ms = C.c_float()
+            phx.L.phx_rt_profile_get(phx.h, r, name, 64, C.byref(ms))
+            k = name.value.decode()
+            acc[k] = acc.get(k, 0.0) + ms.value
+            cnt[k] = cnt.get(k, 0) + 1
+    phx.L.phx_rt_profile_stop(phx.h)
     phx.L.phx_rt_stream_destroy(s)
+    launch_ms = {k: acc[k] / cnt[k] for k in acc}
 
     kb = kernel_bytes(nx)
-    # dominant phase; the flux phase is three launches (x, y, z)
-    dom = max(phase_ms, key=lambda k: phase_ms[k])
-    launches = 3 if dom == "fluxes" else 1
-    per_launch_ms = phase_ms[dom] / launches
-    achieved = kb[dom] / launches / (per_launch_ms * 1e-3) / 1e9
+    dom = max(launch_ms, key=lambda k: launch_ms[k])
+    achieved = kb[dom] / (launch_ms[dom] * 1e-3) / 1e9
     warm_iters = iters.get()[I]
     value = nx ** 3 / (ms_per_step * 1e-3)
     out = {
@@ -233,12 +231,12 @@ def main():
                    "zones": nx ** 3, "recon": "weno5z", "riemann": "hll"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "launches_per_step": launches,
-                     "avg_launch_ms": per_launch_ms},
+                     "traffic": None, "bytes_per_launch": kb[dom],
+                     "avg_launch_ms": launch_ms[dom]},
         "stage_roofline": {"bytes_per_zone_update": B_ALG_STAGE,
                            "achieved_GBs": value * B_ALG_STAGE / 1e9,
                            "frac": value * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS},
-        "phase_ms": phase_ms,
+        "launch_ms": launch_ms,
         "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
                      "mean_iters_cold": float(np.mean(cold_iters)),
                      "hist_warm": np.bincount(np.maximum(warm_iters, 0).ravel()).tolist(),

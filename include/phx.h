@@ -167,6 +167,35 @@ int phx_estimate_dt(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
                     const phx_block_fields *f, int use_face_speeds, double cfl,
                     double *dt_out, phx_stream stream);
 
+/* ---- one RK stage --------------------------------------------------------------------
+ * The fluid tasks of one stage of PhoebusDriver::RungeKuttaStage (phoebus_driver.cpp:
+ * 330-722) for one block, fused into five launches: CalculateFluxes (:344), FluxCT (:345),
+ * CalculateFluidSourceTerms (:322, curved metrics only), then FluxDivergence + SumData +
+ * AverageIndependentData + UpdateIndependentData (:560-576) fused with the robust
+ * ConservedToPrimitive (FillDerived, :588) and the CFL reduction (:720-722) in one kernel.
+ * con2prim runs on the interior cells: the ghost zones are refilled by the boundary
+ * exchange and re-inverted there (:646-668), which this call does not cover.
+ *
+ * Reads f->prim (the stage's primitives) and f->c2p_mu (guess); f->flux, f->face_sig,
+ * f->emf and f->src (curved metrics) are written as scratch and keep this stage's values.
+ * f->cons is U_s (averaged in place); writes a->u_out (U_{s+1}, after any con2prim
+ * re-synchronisation), a->prim_out (may alias f->prim), f->temp, f->c2p_mu, f->cell_sig,
+ * f->fail and, when non-NULL, a->iters and a->dt_out (device; cfl * min over the block,
+ * with face speeds as fluid.cpp:913-928).  Bit-identical to the separate task entry
+ * points called in the same order. */
+typedef struct {
+  double beta, dt;       /* stage weights (LowStorageIntegrator beta, step dt) */
+  const double *u_base;  /* [ncons][cells] U^n                                  */
+  double *u_out;         /* [ncons][cells] U_{s+1}; may alias u_base            */
+  double *prim_out;      /* [nprim][cells]; may alias f->prim                   */
+  int32_t *iters;        /* [cells] con2prim iterations, nullable               */
+  double cfl;
+  double *dt_out;        /* [1] device, nullable                                */
+} phx_stage_args;
+
+int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                    const phx_block_fields *f, const phx_stage_args *a, phx_stream stream);
+
 /* ---- batched con2prim (independent states) ---------------------------------------
  * The per-zone solve of con2prim_robust.hpp:299-453 over n states at spherical
  * Kerr-Schild points coords = [r[n], theta[n]] (phi = 0), or Minkowski when

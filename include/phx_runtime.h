@@ -34,6 +34,14 @@ int phx_rt_event_create(phx_event *e);
 int phx_rt_event_destroy(phx_event e);
 int phx_rt_event_record(phx_event e, phx_stream s);
 int phx_rt_event_elapsed_ms(phx_event start, phx_event stop, float *ms);
+/* Live per-launch timing of phx_fluid_stage: while started, the context brackets each
+ * kernel launch of a stage with HIP events on the stream the launch uses (up to 64 per
+ * reset).  Records are read back after the stream has finished. */
+int phx_rt_profile_start(phx_ctx *ctx);
+int phx_rt_profile_stop(phx_ctx *ctx);
+int phx_rt_profile_reset(phx_ctx *ctx);
+int phx_rt_profile_count(phx_ctx *ctx, int *n);
+int phx_rt_profile_get(phx_ctx *ctx, int idx, char *name, int name_len, float *ms);
 /* device name and compute units, for reports */
 int phx_rt_device_info(char *name, int name_len, int *compute_units, double *mem_gb);
 

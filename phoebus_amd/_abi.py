@@ -48,6 +48,12 @@ class BlockFields(C.Structure):
                                            "face_sig", "flux", "emf", "src")]
 
 
+class StageArgs(C.Structure):
+    _fields_ = [("beta", C.c_double), ("dt", C.c_double), ("u_base", C.c_void_p),
+                ("u_out", C.c_void_p), ("prim_out", C.c_void_p), ("iters", C.c_void_p),
+                ("cfl", C.c_double), ("dt_out", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -77,6 +83,7 @@ def lib():
                                              dbl, vp, vp]),
         "phx_con2prim": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(i32), P(i32), vp, vp]),
         "phx_estimate_dt": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), i32, dbl, vp, vp]),
+        "phx_fluid_stage": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(StageArgs), vp]),
         "phx_con2prim_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
         "phx_prim_to_con_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp]),
         "phx_rt_device_count": (i32, [P(i32)]),
@@ -95,6 +102,11 @@ def lib():
         "phx_rt_event_destroy": (i32, [vp]),
         "phx_rt_event_record": (i32, [vp, vp]),
         "phx_rt_event_elapsed_ms": (i32, [vp, vp, P(C.c_float)]),
+        "phx_rt_profile_start": (i32, [vp]),
+        "phx_rt_profile_stop": (i32, [vp]),
+        "phx_rt_profile_reset": (i32, [vp]),
+        "phx_rt_profile_count": (i32, [vp, P(i32)]),
+        "phx_rt_profile_get": (i32, [vp, i32, C.c_char_p, i32, P(C.c_float)]),
         "phx_rt_device_info": (i32, [C.c_char_p, i32, P(i32), P(dbl)]),
     }
     for name, (res, args) in sig.items():

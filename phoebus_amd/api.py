@@ -119,6 +119,12 @@ class Phx:
                                                 int(use_face_speeds), cfl, _p(dt_out), stream),
                          "phx_estimate_dt")
 
+    def fluid_stage(self, p, b, f, beta, dt, u_base, u_out, prim_out, iters=None, cfl=0.4,
+                    dt_out=None, stream=None):
+        a = A.StageArgs(beta, dt, _p(u_base), _p(u_out), _p(prim_out), _p(iters), cfl, _p(dt_out))
+        return self._chk(self.L.phx_fluid_stage(self.h, C.byref(p), C.byref(b), C.byref(f),
+                                                C.byref(a), stream), "phx_fluid_stage")
+
     # ---- batched con2prim ----------------------------------------------------------------
     def con2prim_batch(self, p, n, coords, cons, prim, mu, fail, iters=None, stream=None):
         return self._chk(self.L.phx_con2prim_batch(self.h, C.byref(p), n, _p(coords), _p(cons),

@@ -778,4 +778,23 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   return 1.0;
 }
 
+// ---- wavefront / block minimum (64-lane wavefronts) ----------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double block_min(double v) {
+  __shared__ double part[16];
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  v = threadIdx.x < nw ? part[threadIdx.x] : DBL_MAX;
+  if (w == 0) v = wave_min(v);
+  return v;
+}
+
 }  // namespace phx

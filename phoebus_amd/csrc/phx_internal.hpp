@@ -9,6 +9,8 @@
 
 #include "phx_device.hpp"
 
+struct phx_ctx;
+
 namespace phx {
 
 inline BlockGeom make_block_geom(const phx_block *b) {
@@ -38,11 +40,30 @@ void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevPa
                    const BlockGeom &bg, const double *prim, double *flux, double *fsig,
                    hipStream_t s);
 
+DevParams make_dev_params(const phx_fluid_params *p);
+int check_params(phx_ctx *ctx, const phx_fluid_params *p);
+int check_block(phx_ctx *ctx, const phx_block *b, int recon);
+
+void launch_flux_ct(const BlockGeom &bg, double *flux, double *emf, hipStream_t s);
+void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
+                    const double *prim, double *src, hipStream_t s);
+void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s);
+
 // partial minima of the CFL reduction; bounds its grid (grid-stride loop beyond)
 constexpr int kDtPartials = 4096;
 }  // namespace phx
 
 struct phx_ctx {
+  // optional per-launch timing: HIP events recorded around each launch of a stage
+  struct TimedLaunch {
+    const char *name;
+    hipEvent_t start, stop;
+  };
+  static constexpr int kMaxTimed = 64;
+  bool profiling = false;
+  int n_timed = 0;
+  TimedLaunch timed[kMaxTimed] = {};
+
   int device = 0;
   hipStream_t stream = nullptr;  // default stream of the context
   double *dt_partial = nullptr;  // [kDtPartials]
@@ -54,6 +75,17 @@ struct phx_ctx {
   int fail(int code, const std::string &msg) {
     last_error = msg;
     return code;
+  }
+  // events bracketing one launch on stream s (no-op unless profiling)
+  void mark_begin(const char *name, hipStream_t s) {
+    if (!profiling || n_timed >= kMaxTimed) return;
+    timed[n_timed].name = name;
+    (void)hipEventRecord(timed[n_timed].start, s);
+  }
+  void mark_end(hipStream_t s) {
+    if (!profiling || n_timed >= kMaxTimed) return;
+    (void)hipEventRecord(timed[n_timed].stop, s);
+    ++n_timed;
   }
   int check_launch() {
     const hipError_t e = hipGetLastError();

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_ct_flux_2d(BlockGeom b, double *__restr
 template <int NC>
 __global__ __launch_bounds__(256) void k_update(BlockGeom b, const double *__restrict__ flux,
                                                 const double *__restrict__ src,
-                                                double *__restrict__ u_stage,
+                                                double *u_stage,
                                                 const double *u_base, double beta, double bdt,
                                                 double *u_out) {
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
@@ -220,24 +220,6 @@ __global__ __launch_bounds__(256) void k_prim_to_con(DevParams p, BlockGeom b, R
 }
 
 // ---- CFL timestep (fluid.cpp:899-944): per-block minimum, then one finishing block ----
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-__device__ __forceinline__ double block_min(double v) {
-  __shared__ double part[16];
-  v = wave_min(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) part[w] = v;
-  __syncthreads();
-  const int nw = (blockDim.x + 63) >> 6;
-  v = threadIdx.x < nw ? part[threadIdx.x] : DBL_MAX;
-  if (w == 0) v = wave_min(v);
-  return v;
-}
-
 __global__ __launch_bounds__(256) void k_dt_partial(BlockGeom b, const double *__restrict__ csig,
                                                     const double *__restrict__ fsig,
                                                     double *__restrict__ partial) {
@@ -524,6 +506,40 @@ int check_block(phx_ctx *ctx, const phx_block *b, int recon) {
   return PHX_OK;
 }
 
+void launch_flux_ct(const BlockGeom &bg, double *flux, double *emf, hipStream_t s) {
+  const int nx0 = bg.ie[0] - bg.is[0] + 1, nx1 = bg.ie[1] - bg.is[1] + 1;
+  const int nx2 = bg.ie[2] - bg.is[2] + 1;
+  if (bg.ndim == 3) {
+    const long n = (long)(nx0 + 1) * (nx1 + 1) * (nx2 + 1);
+    hipLaunchKernelGGL(k_ct_emf_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, emf);
+    hipLaunchKernelGGL(k_ct_flux_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, emf);
+  } else {
+    const long n = (long)(nx0 + 1) * (nx1 + 1) * nx2;
+    hipLaunchKernelGGL(k_ct_emf_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, emf);
+    hipLaunchKernelGGL(k_ct_flux_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, emf);
+  }
+}
+
+void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
+                    const double *prim, double *src, hipStream_t s) {
+  if (geom == PHX_GEOM_MINKOWSKI) {
+    if (mhd)
+      hipLaunchKernelGGL(k_zero_sources<8>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, src);
+    else
+      hipLaunchKernelGGL(k_zero_sources<5>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, src);
+    return;
+  }
+  const long n = (long)(bg.ie[0] - bg.is[0] + 1) * (bg.ie[1] - bg.is[1] + 1) * (bg.ie[2] - bg.is[2] + 1);
+  if (mhd)
+    hipLaunchKernelGGL(k_sources_ks<true>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, prim, src);
+  else
+    hipLaunchKernelGGL(k_sources_ks<false>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, prim, src);
+}
+
+void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1024), 0, s, partial, np, cfl, dt);
+}
+
 }  // namespace phx
 
 using namespace phx;
@@ -558,17 +574,7 @@ int phx_flux_ct(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   if ((rc = check_block(ctx, b, PHX_RECON_LINEAR))) return rc;  // CT reads 2 ghost layers
   if (!p->mhd || b->ndim == 1) return PHX_OK;
   if (!f || !f->flux || !f->emf) return ctx->fail(PHX_ERR_INVALID, "null field");
-  const BlockGeom bg = make_block_geom(b);
-  hipStream_t s = ctx->stream_of(stream);
-  if (b->ndim == 3) {
-    const long n = (long)(b->nx[0] + 1) * (b->nx[1] + 1) * (b->nx[2] + 1);
-    hipLaunchKernelGGL(k_ct_emf_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
-    hipLaunchKernelGGL(k_ct_flux_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
-  } else {
-    const long n = (long)(b->nx[0] + 1) * (b->nx[1] + 1) * b->nx[2];
-    hipLaunchKernelGGL(k_ct_emf_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
-    hipLaunchKernelGGL(k_ct_flux_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, f->flux, f->emf);
-  }
+  launch_flux_ct(make_block_geom(b), f->flux, f->emf, ctx->stream_of(stream));
   return ctx->check_launch();
 }
 
@@ -579,21 +585,8 @@ int phx_fluid_source_terms(phx_ctx *ctx, const phx_fluid_params *p, const phx_bl
   if (rc) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
   if (!f || !f->src || !f->prim) return ctx->fail(PHX_ERR_INVALID, "null field");
-  const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
-  hipStream_t s = ctx->stream_of(stream);
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd)
-      hipLaunchKernelGGL(k_zero_sources<8>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, f->src);
-    else
-      hipLaunchKernelGGL(k_zero_sources<5>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, f->src);
-  } else {
-    const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
-    if (p->mhd)
-      hipLaunchKernelGGL(k_sources_ks<true>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, f->prim, f->src);
-    else
-      hipLaunchKernelGGL(k_sources_ks<false>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, f->prim, f->src);
-  }
+  launch_sources(p->geom.kind, p->mhd, make_dev_params(p), make_block_geom(b), f->prim, f->src,
+                 ctx->stream_of(stream));
   return ctx->check_launch();
 }
 
@@ -698,7 +691,7 @@ int phx_estimate_dt(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   const int np = (int)std::min<long>(nblocks(n, 256), kDtPartials);
   hipLaunchKernelGGL(k_dt_partial, dim3(np), dim3(256), 0, s, bg, f->cell_sig,
                      use_face_speeds ? f->face_sig : nullptr, ctx->dt_partial);
-  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1024), 0, s, ctx->dt_partial, np, cfl, dt_out);
+  launch_dt_finish(ctx->dt_partial, np, cfl, dt_out, s);
   return ctx->check_launch();
 }
 

@@ -36,6 +36,7 @@ int phx_destroy(phx_ctx *c) {
   if (!c) return PHX_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->profiling) (void)phx_rt_profile_stop(c);
   if (c->dt_partial) (void)hipFree(c->dt_partial);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -80,6 +81,55 @@ int phx_rt_event_record(phx_event e, phx_stream s) {
 int phx_rt_event_elapsed_ms(phx_event a, phx_event b, float *ms) {
   return rt(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
 }
+int phx_rt_profile_start(phx_ctx *c) {
+  if (!c) return PHX_ERR_INVALID;
+  if (!c->profiling) {
+    for (int i = 0; i < phx_ctx::kMaxTimed; ++i) {
+      if (hipEventCreate(&c->timed[i].start) != hipSuccess ||
+          hipEventCreate(&c->timed[i].stop) != hipSuccess)
+        return PHX_ERR_DEVICE;
+    }
+    c->profiling = true;
+  }
+  c->n_timed = 0;
+  return PHX_OK;
+}
+
+int phx_rt_profile_stop(phx_ctx *c) {
+  if (!c || !c->profiling) return PHX_ERR_INVALID;
+  for (int i = 0; i < phx_ctx::kMaxTimed; ++i) {
+    (void)hipEventDestroy(c->timed[i].start);
+    (void)hipEventDestroy(c->timed[i].stop);
+  }
+  c->profiling = false;
+  c->n_timed = 0;
+  return PHX_OK;
+}
+
+int phx_rt_profile_count(phx_ctx *c, int *n) {
+  if (!c || !n) return PHX_ERR_INVALID;
+  *n = c->n_timed;
+  return PHX_OK;
+}
+
+int phx_rt_profile_get(phx_ctx *c, int idx, char *name, int name_len, float *ms) {
+  if (!c || idx < 0 || idx >= c->n_timed) return PHX_ERR_INVALID;
+  if (hipEventSynchronize(c->timed[idx].stop) != hipSuccess) return PHX_ERR_DEVICE;
+  if (hipEventElapsedTime(ms, c->timed[idx].start, c->timed[idx].stop) != hipSuccess)
+    return PHX_ERR_DEVICE;
+  if (name && name_len > 0) {
+    std::strncpy(name, c->timed[idx].name, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return PHX_OK;
+}
+
+int phx_rt_profile_reset(phx_ctx *c) {
+  if (!c) return PHX_ERR_INVALID;
+  c->n_timed = 0;
+  return PHX_OK;
+}
+
 int phx_rt_device_info(char *name, int len, int *cus, double *mem_gb) {
   int d = 0;
   if (hipGetDevice(&d) != hipSuccess) return PHX_ERR_DEVICE;

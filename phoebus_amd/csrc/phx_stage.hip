@@ -1,0 +1,164 @@
+// phx_stage.hip -- one fused RK stage of the fluid path for a block (include/phx.h,
+// phx_fluid_stage).  After the flux, CT and source launches, one kernel per interior cell
+// forms dU/dt from the face fluxes, averages and updates the conserved state, inverts it
+// (robust con2prim) and folds the cell into the CFL minimum.  U_{s+1} is written once,
+// after con2prim, so it never makes a round trip through HBM between the update and the
+// inversion.
+#include <hip/hip_runtime.h>
+
+#include "phx.h"
+#include "phx_device.hpp"
+#include "phx_internal.hpp"
+
+namespace phx {
+
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_update_con2prim(
+    DevParams p, BlockGeom b, const double *__restrict__ flux, const double *__restrict__ src,
+    double *u_stage, const double *u_base, double beta, double bdt,
+    double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
+    double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
+    double *__restrict__ fail, int32_t *__restrict__ iters, double *__restrict__ partial) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  constexpr int NC = L::ncons;
+  const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = t < (long)ni * nj * nk;
+  double ldt_min = DBL_MAX;
+  if (active) {
+    const int i = (int)(t % ni) + b.is[0];
+    const int j = (int)((t / ni) % nj) + b.is[1];
+    const int k = (int)(t / ((long)ni * nj)) + b.is[2];
+    const long c = ((long)k * b.nj + j) * b.ni + i;
+    const long n = b.ncell;
+    const long sj = b.ni, sk = (long)b.ni * b.nj;
+    // parthenon FluxDivergence (uniform coordinates), SumData, AverageIndependentData,
+    // UpdateIndependentData -- the operation order of the separate kernels
+    const double a1 = b.dx[1] * b.dx[2], a2 = b.dx[0] * b.dx[2], a3 = b.dx[0] * b.dx[1];
+    const double vol = b.dx[0] * b.dx[1] * b.dx[2];
+    double cs[NC];
+#pragma unroll
+    for (int l = 0; l < NC; ++l) {
+      const double *F1 = flux + (long)l * n;
+      double du = a1 * F1[c + 1] - a1 * F1[c];
+      if (b.ndim >= 2) {
+        const double *F2 = flux + ((long)NC + l) * n;
+        du += a2 * F2[c + sj] - a2 * F2[c];
+      }
+      if (b.ndim == 3) {
+        const double *F3 = flux + ((long)2 * NC + l) * n;
+        du += a3 * F3[c + sk] - a3 * F3[c];
+      }
+      double dudt = -du / vol;
+      if (src != nullptr && l >= 1 && l <= 4) dudt = dudt + src[(long)l * n + c];
+      const double us = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
+      u_stage[(long)l * n + c] = us;
+      cs[l] = us + bdt * dudt;
+    }
+    // ConservedToPrimitive on U_{s+1}
+    double pr[L::nprim], sig[3] = {0.0, 0.0, 0.0};
+    int it = 0;
+    double st = 0.0;
+    const bool nan_in = isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0;
+    if (!nan_in) {
+      const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+      const M g(p.a, X[0], X[1], X[2]);
+#pragma unroll
+      for (int v = 0; v < L::nprim; ++v) pr[v] = prim_out[v * n + c];
+      double tt = temp ? temp[c] : 0.0, m = mu[c];
+      st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
+#pragma unroll
+      for (int v = 0; v < L::nprim; ++v) prim_out[v * n + c] = pr[v];
+      for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
+      if (temp) temp[c] = tt;
+      mu[c] = m;
+    } else {
+      for (int d = 0; d < b.ndim; ++d) sig[d] = csig[d * n + c];
+    }
+#pragma unroll
+    for (int l = 0; l < NC; ++l) u_out[(long)l * n + c] = cs[l];
+    fail[c] = st;
+    if (iters) iters[c] = it;
+    // EstimateTimestepBlock with face speeds (fluid.cpp:913-928)
+    if (partial) {
+      double ldt = 0.0;
+      for (int d = 0; d < b.ndim; ++d) {
+        const long off = d == 0 ? 1 : d == 1 ? sj : sk;
+        const double s = smax(sig[d], smax(fsig[d * n + c], fsig[d * n + c + off]));
+        ldt += s / b.dx[d];
+      }
+      ldt_min = 1.0 / (ldt + 1.e-50);
+    }
+  }
+  if (partial) {
+    const double v = block_min(ldt_min);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+  }
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                               const phx_block_fields *f, const phx_stage_args *a,
+                               phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, p->recon))) return rc;
+  if (!a || !f || !f->prim || !f->cons || !f->flux || !f->face_sig || !f->c2p_mu ||
+      !f->cell_sig || !f->fail || !a->u_base || !a->u_out || !a->prim_out)
+    return ctx->fail(PHX_ERR_INVALID, "null field");
+  if (p->mhd && b->ndim > 1 && (!f->emf || b->ng < 2)) return ctx->fail(PHX_ERR_INVALID, "CT needs emf");
+  const bool curved = p->geom.kind != PHX_GEOM_MINKOWSKI;
+  if (curved && !f->src) return ctx->fail(PHX_ERR_INVALID, "curved metric needs src scratch");
+  const DevParams dp = make_dev_params(p);
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  static const char *kFluxName[3] = {"flux_x", "flux_y", "flux_z"};
+  for (int d = 0; d < b->ndim; ++d) {
+    ctx->mark_begin(kFluxName[d], s);
+    launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
+                  f->face_sig, s);
+    ctx->mark_end(s);
+  }
+  if (p->mhd && b->ndim > 1) {
+    ctx->mark_begin("flux_ct", s);
+    launch_flux_ct(bg, f->flux, f->emf, s);
+    ctx->mark_end(s);
+  }
+  if (curved) {
+    ctx->mark_begin("sources", s);
+    launch_sources(p->geom.kind, p->mhd, dp, bg, f->prim, f->src, s);
+    ctx->mark_end(s);
+  }
+  const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
+  const long nb = (n + 255) / 256;
+  double *partial = nullptr;
+  if (a->dt_out) {
+    if (nb > kDtPartials) return ctx->fail(PHX_ERR_INVALID, "block too large for the CFL reduction");
+    partial = ctx->dt_partial;
+  }
+  const double *src = curved ? f->src : nullptr;
+  const double bdt = a->beta * a->dt;
+  ctx->mark_begin("update_con2prim", s);
+#define ARGS dp, bg, f->flux, src, f->cons, a->u_base, a->beta, bdt, a->u_out, a->prim_out, \
+             f->temp, f->c2p_mu, f->cell_sig, f->face_sig, f->fail, a->iters, partial
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_update_con2prim<true, PHX_GEOM_MINKOWSKI>), dim3(nb), dim3(256), 0, s, ARGS);
+    else hipLaunchKernelGGL((k_update_con2prim<false, PHX_GEOM_MINKOWSKI>), dim3(nb), dim3(256), 0, s, ARGS);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_update_con2prim<true, PHX_GEOM_SPHERICAL_KS>), dim3(nb), dim3(256), 0, s, ARGS);
+    else hipLaunchKernelGGL((k_update_con2prim<false, PHX_GEOM_SPHERICAL_KS>), dim3(nb), dim3(256), 0, s, ARGS);
+  }
+#undef ARGS
+  ctx->mark_end(s);
+  if (a->dt_out) {
+    ctx->mark_begin("dt_finish", s);
+    launch_dt_finish(ctx->dt_partial, (int)nb, a->cfl, a->dt_out, s);
+    ctx->mark_end(s);
+  }
+  return ctx->check_launch();
+}

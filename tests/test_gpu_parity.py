@@ -338,3 +338,69 @@ def test_abi_rejects_bad_configs(phx):
     p.geom.kind = 7
     with pytest.raises(A.PhxError):
         phx.calculate_fluxes(p, block(1, (16,), ng=4), st.fields())
+
+
+STAGE_CASES = [
+    (3, (16, 12, 10), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_MINKOWSKI, 1.0),
+    (3, (12, 12, 12), True, A.RECON_MP5, A.RIEMANN_LLF, A.GEOM_MINKOWSKI, 0.5),
+    (2, (24, 20), False, A.RECON_LINEAR, A.RIEMANN_HLL, A.GEOM_MINKOWSKI, 0.5),
+    (1, (96,), False, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_MINKOWSKI, 1.0),
+    (3, (10, 8, 6), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_SPHERICAL_KS, 0.5),
+]
+
+
+@pytest.mark.parametrize("case", STAGE_CASES, ids=[case_id(c) + f"-b{c[6]}" for c in STAGE_CASES])
+def test_fused_stage_matches_oracle_sequence(phx, case):
+    """phx_fluid_stage against the oracle running the tasks in the reference's order."""
+    ndim, nx, mhd, recon, riemann, geom, beta = case
+    say(f"[gpu] fused stage {case_id(case)} beta={beta}")
+    rng = np.random.default_rng(7 + ndim + 10 * recon)
+    gamma = 13.0 / 9.0
+    if geom == A.GEOM_SPHERICAL_KS:
+        b = block(ndim, nx, 4, xmin=(4.0, 0.6, 0.0), dx=(0.25, 0.05, 0.05))
+    else:
+        b = block(ndim, nx, 4, dx=(0.1, 0.2, 0.3))
+    p = fluid_params(gamma=gamma, mhd=mhd, recon=recon, riemann=riemann, geom=geom, a=0.9375)
+    op, ob = oparams(p), oblock(b)
+    exact = geom == A.GEOM_MINKOWSKI
+    prim = random_block(rng, b, mhd, gamma, geom)
+    u_stage, _ = O.prim_to_con_block(op, ob, prim)
+    u_base = u_stage * (1.0 + 0.01 * rng.uniform(-1, 1, u_stage.shape))
+    # oracle: fluxes -> CT -> sources -> divergence/average/update -> c2p(interior) -> dt
+    flux, fsig = O.calculate_fluxes(op, ob, prim)
+    O.flux_ct(op, ob, flux)
+    src = O.source_terms_block(op, ob, prim) if not exact else None
+    stage_ref = u_stage.copy()
+    out_ref = O.flux_divergence_update(op, ob, flux, src, stage_ref, u_base, beta, 2e-3)
+    I = O.interior(ob)
+    lo = [s.start for s in I[::-1]]
+    hi = [s.stop - 1 for s in I[::-1]]
+    prim_ref, mu_ref = prim.copy(), np.zeros(O.block_shape(ob))
+    r = O.con2prim_block(op, ob, out_ref, prim_ref, mu_ref, lo=lo, hi=hi)
+    dt_ref = O.estimate_dt(op, ob, r["sig"], fsig, 0.4)
+    # library
+    st = BlockState(p, b)
+    st.prim.upload(prim)
+    st.cons.upload(u_stage)
+    d_base = DeviceArray.from_host(u_base)
+    d_out = DeviceArray.from_host(u_stage)
+    d_it = DeviceArray(O.block_shape(ob), np.int32)
+    d_dt = DeviceArray((1,))
+    phx.fluid_stage(p, b, st.fields(with_src=True), beta, 2e-3, d_base, d_out, st.prim,
+                    iters=d_it, cfl=0.4, dt_out=d_dt)
+    synchronize()
+    gout, gprim, gfail = d_out.get(), st.prim.get(), st.fail.get()
+    assert np.array_equal(gfail[I], r["fail"][I])
+    if exact:
+        for m in range(gout.shape[0]):
+            assert_match(gout[m][I], out_ref[m][I], True, f"U_out m={m}")
+        assert_match(st.cons.get()[(slice(None),) + I], stage_ref[(slice(None),) + I], True, "U_s")
+        assert_match(gprim[(slice(None),) + I], prim_ref[(slice(None),) + I], True, "prims")
+        assert np.array_equal(d_it.get()[I], r["iters"][I])
+        assert d_dt.get()[0] == dt_ref
+    else:
+        for m in range(gout.shape[0]):
+            assert_match(gout[m][I], out_ref[m][I], False, f"U_out m={m}", rtol=1e-10,
+                         atol_scale=np.max(np.abs(out_ref[m][I])))
+        assert np.max(np.abs(st.c2p_mu.get()[I] - mu_ref[I])) <= 4 * p.c2p_tol
+        assert abs(d_dt.get()[0] - dt_ref) <= 1e-8 * dt_ref
