@@ -49,8 +49,9 @@ void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
                     const double *prim, double *src, hipStream_t s);
 void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s);
 
-// partial minima of the CFL reduction; bounds its grid (grid-stride loop beyond)
-constexpr int kDtPartials = 4096;
+// partial minima of the CFL reduction (one per 256-cell block of the fused stage kernel,
+// so one call covers up to 16.7 M cells; the separate reduction strides beyond that)
+constexpr int kDtPartials = 1 << 16;
 }  // namespace phx
 
 struct phx_ctx {
