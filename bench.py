@@ -170,7 +170,7 @@ def main():
 
     def stage():
         # the warm-start guess of each repeated stage is the input state's own root
-        phx.L.phx_rt_memcpy_dtod(st.c2p_mu.ptr, mu0.ptr, mu0.nbytes)
+        phx.L.phx_rt_memcpy_dtod_async(st.c2p_mu.ptr, mu0.ptr, mu0.nbytes, s)
         phx.fluid_stage(p, b, f_stage, 1.0, 1e-4, st.cons, u_out, prim_out, iters=iters,
                         cfl=0.4, dt_out=dt_d, stream=s)
 

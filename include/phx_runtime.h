@@ -25,6 +25,7 @@ int phx_rt_free(void *ptr);
 int phx_rt_memcpy_htod(void *dst, const void *src, size_t bytes);
 int phx_rt_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int phx_rt_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int phx_rt_memcpy_dtod_async(void *dst, const void *src, size_t bytes, phx_stream s);
 int phx_rt_memset(void *ptr, int value, size_t bytes);
 int phx_rt_device_synchronize(void);
 int phx_rt_stream_create(phx_stream *s);

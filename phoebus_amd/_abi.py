@@ -93,6 +93,7 @@ def lib():
         "phx_rt_memcpy_htod": (i32, [vp, vp, C.c_size_t]),
         "phx_rt_memcpy_dtoh": (i32, [vp, vp, C.c_size_t]),
         "phx_rt_memcpy_dtod": (i32, [vp, vp, C.c_size_t]),
+        "phx_rt_memcpy_dtod_async": (i32, [vp, vp, C.c_size_t, vp]),
         "phx_rt_memset": (i32, [vp, i32, C.c_size_t]),
         "phx_rt_device_synchronize": (i32, []),
         "phx_rt_stream_create": (i32, [P(vp)]),

@@ -60,8 +60,7 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
   using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
-  __shared__ double stencil[XB];
-  __shared__ double upper[NP][XB];
+  __shared__ double stage[NP][XB];  // primitives, then upper-edge states
   const int dj = b.ndim > 1, dk = b.ndim > 2;
   const int fj = b.ie[1] - b.is[1] + 1 + 2 * dj;
   const int fk = b.ie[2] - b.is[2] + 1 + 2 * dk;
@@ -78,25 +77,29 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
   // cells whose stencil lies in this block and in this row
   const bool recon_ok = t >= H && t < XB - H && i >= H && i < b.ni - H;
 
-  double lo[NP];
+  // three barriers per block: primitives in, edges computed, upper edges out
+#pragma unroll
+  for (int v = 0; v < NP; ++v) stage[v][t] = prim[v * b.ncell + c];
+  __syncthreads();
+  double lo[NP], up[NP];
 #pragma unroll
   for (int v = 0; v < NP; ++v) {
-    stencil[t] = prim[v * b.ncell + c];
-    __syncthreads();
     double q[2 * H + 1];
 #pragma unroll
-    for (int o = 0; o < 2 * H + 1; ++o) q[o] = stencil[recon_ok ? t + o - H : t];
-    double up;
-    reconstruct_edges<RECON>(q, up, lo[v]);
-    upper[v][t] = up;
+    for (int o = 0; o < 2 * H + 1; ++o) q[o] = stage[v][recon_ok ? t + o - H : t];
+    reconstruct_edges<RECON>(q, up[v], lo[v]);
+    pin(up[v]);
     pin(lo[v]);
-    __syncthreads();
   }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NP; ++v) stage[v][t] = up[v];
+  __syncthreads();
   const bool emit = valid && t >= H + 1 && t < XB - H && i >= b.is[0] - 1 && i <= b.ie[0] + 1;
   if (!emit) return;
   double ql[NP];
 #pragma unroll
-  for (int v = 0; v < NP; ++v) ql[v] = upper[v][t - 1];
+  for (int v = 0; v < NP; ++v) ql[v] = stage[v][t - 1];
   const double X[3] = {xf(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
   face_out<MHD, RIEMANN, M>(p, 0, b, X, ql, lo, c, flux, fsig);
 }
@@ -121,9 +124,7 @@ __global__ __launch_bounds__(kTileThreads) void k_flux_tile(DevParams p, BlockGe
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
   constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
-  extern __shared__ double lds[];
-  double *stencil = lds;                     // [ts_n][ti_n]
-  double *upper = lds + ts_n * ti_n;         // [NP][ts_n][ti_n]
+  extern __shared__ double lds[];            // [NP][ts_n][ti_n]: primitives, then upper edges
   const int S = ts_n - 2 * H - 1;            // faces per tile
   const int tid = threadIdx.x;
   const int ti = tid % ti_n, ts = tid / ti_n;
@@ -147,27 +148,36 @@ __global__ __launch_bounds__(kTileThreads) void k_flux_tile(DevParams p, BlockGe
       live && ts >= H && ts < ts_n - H && o_raw - H >= 0 && o_raw + H <= nown - 1;
   const int me = ts * ti_n + ti;
 
-  double lo[NP];
+  const int plane = ts_n * ti_n;
+  // three barriers per block: primitives in, edges computed, upper edges out
+  if (live) {
+#pragma unroll
+    for (int v = 0; v < NP; ++v) lds[v * plane + me] = prim[v * b.ncell + c];
+  }
+  __syncthreads();
+  double lo[NP], up[NP];
 #pragma unroll
   for (int v = 0; v < NP; ++v) {
-    if (live) stencil[me] = prim[v * b.ncell + c];
-    __syncthreads();
     double q[2 * H + 1];
 #pragma unroll
     for (int oo = 0; oo < 2 * H + 1; ++oo)
-      q[oo] = live ? stencil[recon_ok ? me + (oo - H) * ti_n : me] : 0.0;
-    double up;
-    reconstruct_edges<RECON>(q, up, lo[v]);
-    if (live) upper[(v * ts_n) * ti_n + me] = up;
+      q[oo] = live ? lds[v * plane + (recon_ok ? me + (oo - H) * ti_n : me)] : 0.0;
+    reconstruct_edges<RECON>(q, up[v], lo[v]);
+    pin(up[v]);
     pin(lo[v]);
-    __syncthreads();
   }
+  __syncthreads();
+  if (live) {
+#pragma unroll
+    for (int v = 0; v < NP; ++v) lds[v * plane + me] = up[v];
+  }
+  __syncthreads();
   const bool emit = live && ts >= H + 1 && ts < ts_n - H && i_raw <= b.ie[0] + 1 &&
                     o_raw >= b.is[DIR] - 1 && o_raw <= b.ie[DIR] + 1;
   if (!emit) return;
   double ql[NP];
 #pragma unroll
-  for (int v = 0; v < NP; ++v) ql[v] = upper[(v * ts_n) * ti_n + me - ti_n];
+  for (int v = 0; v < NP; ++v) ql[v] = lds[v * plane + me - ti_n];
   double X[3];
   X[0] = xc(b, 0, i);
   X[T] = xc(b, T, t);
@@ -204,7 +214,7 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
   const long ntr = bg.ie[T] - bg.is[T] + 1 + 2 * dt_;
   const unsigned blocks = (unsigned)(nic * nft * ntr);
   const int threads = ((ti_n * ts_n + 63) / 64) * 64;
-  const size_t lds = sizeof(double) * (size_t)(NP + 1) * ts_n * ti_n;
+  const size_t lds = sizeof(double) * (size_t)NP * ts_n * ti_n;
   if (d == 1)
     hipLaunchKernelGGL((k_flux_tile<1, RECON, RIEMANN, MHD, GEOM>), dim3(blocks), dim3(threads),
                        lds, s, dp, bg, ti_n, ts_n, prim, flux, fsig);

@@ -58,6 +58,9 @@ int phx_rt_memcpy_dtoh(void *d, const void *s, size_t n) {
 int phx_rt_memcpy_dtod(void *d, const void *s, size_t n) {
   return rt(hipMemcpy(d, s, n, hipMemcpyDeviceToDevice));
 }
+int phx_rt_memcpy_dtod_async(void *d, const void *s, size_t n, phx_stream st) {
+  return rt(hipMemcpyAsync(d, s, n, hipMemcpyDeviceToDevice, (hipStream_t)st));
+}
 int phx_rt_memset(void *p, int v, size_t n) { return rt(hipMemset(p, v, n)); }
 int phx_rt_device_synchronize(void) { return rt(hipDeviceSynchronize()); }
 int phx_rt_stream_create(phx_stream *s) {
