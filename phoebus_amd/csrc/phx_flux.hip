@@ -111,11 +111,11 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
 // cells.  The host sizes ts_n to the whole own extent of the face range when it fits
 // (135 cells for a 128-zone WENO5 block), which keeps 92% of the lanes emitting; the
 // kernel is FP64-issue-bound, so the narrow rows cost little in load efficiency.
-constexpr int kTileThreads = 1024;
+constexpr int kTileThreads = 512;
 constexpr int kTileMaxRows = 256;
 
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(kTileThreads) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
+__global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
                                                             int ts_n,
                                                             const double *__restrict__ prim,
                                                             double *__restrict__ flux,
