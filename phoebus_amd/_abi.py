@@ -7,6 +7,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+MLP_NPARAM = 8770
 LIBPATH = os.path.join(HERE, "lib", "libphx.so")
 
 PHX_OK, PHX_ERR_INVALID, PHX_ERR_UNSUPPORTED, PHX_ERR_DEVICE, PHX_ERR_NOMEM = 0, -1, -2, -3, -4
@@ -86,6 +87,20 @@ def lib():
         "phx_fluid_stage": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(StageArgs), vp]),
         "phx_con2prim_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
         "phx_prim_to_con_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp]),
+        "phx_mlp_create": (i32, [vp, P(vp), C.c_uint64]),
+        "phx_mlp_destroy": (i32, [vp]),
+        "phx_mlp_set_params": (i32, [vp, vp, P(C.c_float)]),
+        "phx_mlp_get_params": (i32, [vp, vp, P(C.c_float)]),
+        "phx_mlp_set_normalization": (i32, [vp, vp, P(C.c_float), P(C.c_float), C.c_float, C.c_float]),
+        "phx_con2prim_features": (i32, [vp, P(FluidParams), vp, C.c_int64, vp, vp, vp, vp]),
+        "phx_mlp_forward": (i32, [vp, vp, C.c_int64, vp, vp, vp]),
+        "phx_mlp_gradient": (i32, [vp, vp, C.c_int64, vp, vp, C.c_float, vp, vp]),
+        "phx_mlp_read_gradient": (i32, [vp, vp, P(C.c_float)]),
+        "phx_mlp_zero_gradient": (i32, [vp, vp]),
+        "phx_mlp_adam_step": (i32, [vp, vp, C.c_float, vp]),
+        "phx_mlp_train_step": (i32, [vp, vp, C.c_int64, vp, vp, C.c_float, C.c_float, vp, vp]),
+        "phx_con2prim_batch_learned": (i32, [vp, P(FluidParams), vp, C.c_int64, vp, vp, vp, vp, vp,
+                                             vp, vp, vp, vp]),
         "phx_rt_device_count": (i32, [P(i32)]),
         "phx_rt_set_device": (i32, [i32]),
         "phx_rt_malloc": (i32, [P(vp), C.c_size_t]),

@@ -164,3 +164,80 @@ class BlockState:
         return Phx.fields(prim=self.prim, cons=self.cons, temp=self.temp, c2p_mu=self.c2p_mu,
                           fail=self.fail, cell_sig=self.cell_sig, face_sig=self.face_sig,
                           flux=self.flux, emf=self.emf, src=self.src if with_src else None)
+
+
+class Mlp:
+    """The learned con2prim bracket network (include/phx_mlp.h) on a context's device."""
+
+    def __init__(self, phx, seed=1):
+        self.phx, self.L = phx, phx.L
+        h = C.c_void_p()
+        phx._chk(self.L.phx_mlp_create(phx.h, C.byref(h), seed), "phx_mlp_create")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            self.L.phx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == A.MLP_NPARAM
+        self.phx._chk(self.L.phx_mlp_set_params(self.phx.h, self.h,
+                                                 w.ctypes.data_as(C.POINTER(C.c_float))),
+                      "phx_mlp_set_params")
+
+    def get_params(self):
+        w = np.zeros(A.MLP_NPARAM, np.float32)
+        self.phx._chk(self.L.phx_mlp_get_params(self.phx.h, self.h,
+                                                 w.ctypes.data_as(C.POINTER(C.c_float))),
+                      "phx_mlp_get_params")
+        return w
+
+    def set_normalization(self, mean, std, width_scale=4.0, width_min=1e-9):
+        m = (C.c_float * 4)(*[float(x) for x in mean])
+        s = (C.c_float * 4)(*[float(x) for x in std])
+        self.phx._chk(self.L.phx_mlp_set_normalization(self.phx.h, self.h, m, s, width_scale,
+                                                        width_min), "phx_mlp_set_normalization")
+
+    def features(self, p, n, coords, cons, out, stream=None):
+        self.phx._chk(self.L.phx_con2prim_features(self.phx.h, C.byref(p), self.h, n, _p(coords),
+                                                    _p(cons), _p(out), stream),
+                      "phx_con2prim_features")
+
+    def forward(self, n, feat, out, stream=None):
+        self.phx._chk(self.L.phx_mlp_forward(self.phx.h, self.h, n, _p(feat), _p(out), stream),
+                      "phx_mlp_forward")
+
+    def gradient(self, n, feat, target, lam, loss=None, stream=None):
+        self.phx._chk(self.L.phx_mlp_gradient(self.phx.h, self.h, n, _p(feat), _p(target), lam,
+                                               _p(loss), stream), "phx_mlp_gradient")
+
+    def read_gradient(self):
+        g = np.zeros(A.MLP_NPARAM, np.float32)
+        self.phx._chk(self.L.phx_mlp_read_gradient(self.phx.h, self.h,
+                                                    g.ctypes.data_as(C.POINTER(C.c_float))),
+                      "phx_mlp_read_gradient")
+        return g
+
+    def zero_gradient(self):
+        self.phx._chk(self.L.phx_mlp_zero_gradient(self.phx.h, self.h), "phx_mlp_zero_gradient")
+
+    def adam_step(self, lr, stream=None):
+        self.phx._chk(self.L.phx_mlp_adam_step(self.phx.h, self.h, lr, stream), "phx_mlp_adam_step")
+
+    def train_step(self, n, feat, target, lr, lam, loss=None, stream=None):
+        self.phx._chk(self.L.phx_mlp_train_step(self.phx.h, self.h, n, _p(feat), _p(target), lr,
+                                                 lam, _p(loss), stream), "phx_mlp_train_step")
+
+    def con2prim_batch(self, p, n, coords, cons, prim, mu, fail, iters=None, evals=None,
+                       fallback=None, stream=None):
+        self.phx._chk(self.L.phx_con2prim_batch_learned(
+            self.phx.h, C.byref(p), self.h, n, _p(coords), _p(cons), _p(prim), _p(mu), _p(fail),
+            _p(iters), _p(evals), _p(fallback), stream), "phx_con2prim_batch_learned")

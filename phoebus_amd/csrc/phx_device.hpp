@@ -653,16 +653,64 @@ __device__ __forceinline__ bool regula_falsi(F &f, double a, double b, double to
   return true;
 }
 
-// con2prim_robust.hpp:299-453 for one zone.  cons/prim in registers; returns the fail
-// flag value (1.0 success, 0.0 failure).  iters = -1 marks an unbracketed root.
+// root_find.hpp:123-166 on a bracket [a, b] whose residuals ya, yb are already known
+// (ya * yb <= 0); the loop and its iteration count are the reference's.
+template <class F>
+__device__ __forceinline__ double regula_falsi_bracketed(F &f, double a, double b, double ya,
+                                                         double yb, double tol, int max_iter,
+                                                         int &iters) {
+  int it = 0;
+  const double sign = (ya < 0 ? 1.0 : -1.0);
+  ya *= sign;
+  yb *= sign;
+  int b1 = 0, b2 = 0;
+  while (b - a > 2.0 * tol && it < max_iter) {
+    const double c = (a * yb - b * ya) / (yb - ya);
+    if (c == a) {
+      b = a;
+      it++;
+      continue;
+    } else if (c == b) {
+      a = b;
+      it++;
+      continue;
+    }
+    const double yc = sign * f(c);
+    if (yc > 0.0) {
+      b = c;
+      yb = yc;
+      b1++;
+      ya *= (b1 > 1 ? 0.5 : 1.0);
+      b2 = 0;
+    } else if (yc < 0.0) {
+      a = c;
+      ya = yc;
+      b2++;
+      yb *= (b2 > 1 ? 0.5 : 1.0);
+      b1 = 0;
+    } else {
+      a = c;
+      b = c;
+    }
+    it++;
+  }
+  iters = it;
+  return 0.5 * (a + b);
+}
+
+// con2prim_robust.hpp:299-453 for one zone, in three pieces so that a bracket predictor
+// can run between the set-up and the root find: c2p_setup (:303-367), the root find
+// (:374-376), c2p_unwrap (:382-452).
+struct C2PState {
+  Residual res;
+  double rcon[3], bu[3], bdotr;
+};
+
+// :303-367.  Returns false for the NaN early-out (the zone keeps every field).
 template <bool MHD, class M>
-__device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
-                                                const double X[3], double *cons,
-                                                double *prim, double &temp, double &mu_io,
-                                                double sig[3], int &iters) {
-  using L = Layout<MHD>;
-  iters = 0;
-  if (isnan(cons[0]) + isnan(cons[1]) + isnan(cons[4]) > 0) return 0.0;
+__device__ __forceinline__ bool c2p_setup(const DevParams &p, const M &g, const double X[3],
+                                          const double *cons, double *prim, C2PState &z) {
+  if (isnan(cons[0]) + isnan(cons[1]) + isnan(cons[4]) > 0) return false;
   const double igdet = 1.0 / g.detgamma();
   double rhoflr = 0.0, epsflr;
   get_floors(p.bnd, X[0], X[1], X[2], rhoflr, epsflr);
@@ -672,39 +720,40 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   const double tau = cons[4] * igdet;
   const double q = ratio(tau, D);
   double bsq = 0.0, bsq_rpsq = 0.0, rbsq = 0.0, rsq = 0.0, bdotr = 0.0;
-  double rcov[3], rcon[3], bu[3] = {0.0, 0.0, 0.0};
+  double rcov[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) rcov[i] = ratio(cons[1 + i], cons[0]);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    rcon[i] = 0.0;
+    z.rcon[i] = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-      if (M::nz_con(i, j)) rcon[i] += g.gcon(i, j) * rcov[j];
-    rsq += rcon[i] * rcov[i];
+      if (M::nz_con(i, j)) z.rcon[i] += g.gcon(i, j) * rcov[j];
+    rsq += z.rcon[i] * rcov[i];
   }
+  z.bu[0] = z.bu[1] = z.bu[2] = 0.0;
   if constexpr (MHD) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) prim[5 + i] = cons[5 + i] * igdet;
     const double sD = 1.0 / sqrt(smax(D, rhoflr));
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      bu[i] = prim[5 + i] * sD;
-      bdotr += bu[i] * rcov[i];
+      z.bu[i] = prim[5 + i] * sD;
+      bdotr += z.bu[i] * rcov[i];
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        if (M::nz_cov(i + 1, j + 1)) bsq += g.gcov(i + 1, j + 1) * bu[i] * bu[j];
+        if (M::nz_cov(i + 1, j + 1)) bsq += g.gcov(i + 1, j + 1) * z.bu[i] * z.bu[j];
     bsq = smax(0.0, bsq);
     rbsq = bdotr * bdotr;
     bsq_rpsq = bsq * rsq - rbsq;
   }
+  z.bdotr = bdotr;
   const double zsq = rsq / 1.0;
   const double v0sq = smin(zsq / (1.0 + zsq), 1.0 - 1.0 / (gam_max * gam_max));
-
-  Residual res;
+  Residual &res = z.res;
   res.p = &p;
   res.D = D; res.q = q; res.bsq = bsq; res.bsq_rpsq = bsq_rpsq; res.rsq = rsq;
   res.rbsq = rbsq; res.v0sq = v0sq; res.x1 = X[0]; res.x2 = X[1]; res.x3 = X[2];
@@ -717,14 +766,16 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   }
   res.e_floor = 0.0;
   res.used_density_floor = res.used_energy_floor = res.used_energy_max = res.used_gamma_max = false;
+  return true;
+}
 
-  double mu;
-  if (!regula_falsi(res, 0.0, 1.0, p.tol, mu_io, p.max_iter, mu, iters)) {
-    iters = -1;
-    return 0.0;
-  }
-  mu_io = mu;
-
+// :382-452 after the root mu; returns the fail-flag value
+template <bool MHD, class M>
+__device__ __forceinline__ double c2p_unwrap(const DevParams &p, const M &g, C2PState &z,
+                                             const double mu, double *cons, double *prim,
+                                             double &temp, double sig[3]) {
+  using L = Layout<MHD>;
+  Residual &res = z.res;
   const double x = res.x_mu(mu);
   const double rbarsq = res.rbarsq_mu(mu, x);
   const double vsq = res.vhatsq_mu(mu, rbarsq);
@@ -737,10 +788,10 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   prim[4] *= prim[0];
   prim[L::prs] = eos_p_rho_t(p, prim[0], temp);
   prim[L::gm1] = eos_bmod_rho_t(p, prim[0], temp) / prim[L::prs];
-  double vel[3];
+  double vel[3], bu[3] = {z.bu[0], z.bu[1], z.bu[2]};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    vel[i] = W * mu * x * (rcon[i] + mu * bdotr * bu[i]);
+    vel[i] = W * mu * x * (z.rcon[i] + mu * z.bdotr * bu[i]);
     prim[1 + i] = vel[i];
   }
   if constexpr (MHD) {
@@ -776,6 +827,25 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   if (p.fail_on_floors && (res.used_density_floor || res.used_energy_floor)) return 0.0;
   if (p.fail_on_ceilings && res.used_gamma_max) return 0.0;
   return 1.0;
+}
+
+// the whole solve with the reference's root find: regula_falsi on [0, 1] split at the
+// guess mu_io.  iters = -1 marks an unbracketed root.
+template <bool MHD, class M>
+__device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
+                                                const double X[3], double *cons,
+                                                double *prim, double &temp, double &mu_io,
+                                                double sig[3], int &iters) {
+  iters = 0;
+  C2PState z;
+  if (!c2p_setup<MHD>(p, g, X, cons, prim, z)) return 0.0;
+  double mu;
+  if (!regula_falsi(z.res, 0.0, 1.0, p.tol, mu_io, p.max_iter, mu, iters)) {
+    iters = -1;
+    return 0.0;
+  }
+  mu_io = mu;
+  return c2p_unwrap<MHD>(p, g, z, mu, cons, prim, temp, sig);
 }
 
 // ---- wavefront / block minimum (64-lane wavefronts) ----------------------------------

@@ -1,0 +1,720 @@
+// phx_mlp.hip -- learned con2prim bracket (no reference counterpart): a 4-64-64-64-2
+// ReLU network on BF16 MFMA with FP32 accumulation, its training kernels (forward,
+// backward, Adam) and the con2prim kernel that uses it.
+//
+// Network layout.  Every product is formed transposed, Z^T = W . H^T, with zones as the
+// MFMA N dimension: v_mfma_f32_16x16x16_bf16 leaves lane l holding rows 4(l/16)..+3 of
+// column l%16 of its 16x16 output tile, which is exactly the B-operand layout the next
+// layer needs (K rows 4(l/16)..+3 of column l%16).  Activations therefore stay in
+// registers from layer to layer; weights are the A operand, packed on the device into
+// per-lane fragments (lane l: row l%16, columns 4(l/16)..+3).  The weight-gradient
+// products need the other orientation and transpose 16x16 tiles through LDS.
+//
+// Features (scale-free conserved invariants of the con2prim set-up): log q, log r^2,
+// log b^2, log (r.b)^2 with q = tau/D, standardized.  Outputs: o1 = logit of the bracket
+// centre, o2 = log of the bracket half-width.  The solver evaluates the residual at the
+// bracket ends; a sign change runs the reference's regula falsi inside it, anything
+// else falls back to the plain solve on [0, 1] with the centre as guess.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "phx.h"
+#include "phx_device.hpp"
+#include "phx_internal.hpp"
+#include "phx_mlp.h"
+
+namespace phx {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// parameter vector: W1[64][4] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2]
+constexpr int kIn = 4, kHid = 64, kOut = 2;
+constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
+constexpr int oW3 = oB2 + kHid, oB3 = oW3 + kHid * kHid, oW4 = oB3 + kHid;
+constexpr int oB4 = oW4 + kOut * kHid, kNParam = oB4 + kOut;
+static_assert(kNParam == PHX_MLP_NPARAM, "parameter count");
+
+// fragment counts: forward A = W (M = out, K = in); backward A = W^T (M = in, K = out)
+constexpr int kFwdFrags = 4 * 1 + 4 * 4 + 4 * 4 + 1 * 4;  // L1, L2, L3, L4
+constexpr int kBwdFrags = 4 * 4 + 4 * 4 + 4 * 1;          // L2, L3, L4 (W^T)
+constexpr int fwdBase[4] = {0, 4, 20, 36};
+
+struct MlpDev {
+  const uint64_t *fwd;   // [kFwdFrags][64] packed bf16x4
+  const uint64_t *bwd;   // [kBwdFrags][64]
+  const float *param;    // [kNParam] fp32 master (biases read from here)
+  float fmean[4], fstd[4];
+  float width_scale, width_min;
+};
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+  bf16x4 v;
+  v[0] = (short)f2bf(a);
+  v[1] = (short)f2bf(b);
+  v[2] = (short)f2bf(c);
+  v[3] = (short)f2bf(d);
+  return v;
+}
+__device__ __forceinline__ bf16x4 as_bf16x4(uint64_t u) {
+  bf16x4 v;
+  memcpy(&v, &u, 8);
+  return v;
+}
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// ---- forward for one tile of 16 zones -------------------------------------------------
+// x: B fragment of the standardized features (lanes 0..15: features 0..3 of zone l; others
+// zero).  h1..h3 (optional): post-activation tiles, 4 neuron tiles each.  Returns the
+// output tile (lanes 0..15 hold o1, o2 of zone l in elements 0, 1).
+__device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, bf16x4 x, f32x4 *h1,
+                                                  f32x4 *h2, f32x4 *h3) {
+  const int lane = threadIdx.x & 63;
+  const int r0 = 4 * (lane >> 4);
+  f32x4 a[4], bcur[4];
+  // layer 1: 64 x 16(4) . 16 x 16
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(as_bf16x4(m.fwd[(fwdBase[0] + mt) * 64 + lane]), x, acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.param[oB1 + mt * 16 + r0 + r], 0.f);
+    a[mt] = acc;
+    if (h1) h1[mt] = acc;
+  }
+  // layers 2 and 3: 64 x 64 . 64 x 16
+#pragma unroll
+  for (int layer = 2; layer <= 3; ++layer) {
+    const int base = fwdBase[layer - 1];
+    const int ob = layer == 2 ? oB2 : oB3;
+    bf16x4 bin[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) bin[ks] = pack4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        acc = mfma(as_bf16x4(m.fwd[(base + mt * 4 + ks) * 64 + lane]), bin[ks], acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.param[ob + mt * 16 + r0 + r], 0.f);
+      bcur[mt] = acc;
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      a[mt] = bcur[mt];
+      if (layer == 2 && h2) h2[mt] = bcur[mt];
+      if (layer == 3 && h3) h3[mt] = bcur[mt];
+    }
+  }
+  // layer 4: 16(2) x 64 . 64 x 16
+  f32x4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks)
+    o = mfma(as_bf16x4(m.fwd[(fwdBase[3] + ks) * 64 + lane]),
+             pack4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]), o);
+  if (r0 == 0) {
+    o[0] += m.param[oB4 + 0];
+    o[1] += m.param[oB4 + 1];
+  }
+  return o;
+}
+
+// standardized features of one zone from its con2prim set-up
+__device__ __forceinline__ void zone_features(const MlpDev &m, const Residual &r, float f[4]) {
+  const double raw[4] = {log(fmax(r.q, 0.0) + 1e-12), log(fmax(r.rsq, 0.0) + 1e-12),
+                         log(fmax(r.bsq, 0.0) + 1e-12), log(fmax(r.rbsq, 0.0) + 1e-24)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f[i] = ((float)raw[i] - m.fmean[i]) / m.fstd[i];
+}
+
+// wave-cooperative prediction for the 64 zones of this wavefront (all lanes must call it):
+// returns o1, o2 of this lane's zone
+__device__ __forceinline__ void mlp_predict_wave(const MlpDev &m, const float f[4], float &o1,
+                                                 float &o2) {
+  const int lane = threadIdx.x & 63;
+  float r1 = 0.f, r2 = 0.f;
+#pragma unroll
+  for (int zt = 0; zt < 4; ++zt) {
+    // B fragment: lanes 0..15 carry the features of zone zt*16 + lane
+    const int src = zt * 16 + (lane & 15);
+    const float g0 = __shfl(f[0], src, 64), g1 = __shfl(f[1], src, 64);
+    const float g2 = __shfl(f[2], src, 64), g3 = __shfl(f[3], src, 64);
+    const bf16x4 x = lane < 16 ? pack4(g0, g1, g2, g3) : pack4(0.f, 0.f, 0.f, 0.f);
+    const f32x4 o = mlp_tile_forward(m, x, nullptr, nullptr, nullptr);
+    const float t1 = __shfl(o[0], lane & 15, 64), t2 = __shfl(o[1], lane & 15, 64);
+    if ((lane >> 4) == zt) {
+      r1 = t1;
+      r2 = t2;
+    }
+  }
+  o1 = r1;
+  o2 = r2;
+}
+
+// ---- kernels: packing, training, Adam, batch inference ------------------------------------
+__global__ void k_mlp_pack(const float *__restrict__ param, uint64_t *__restrict__ fwd,
+                           uint64_t *__restrict__ bwd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = t & 63, frag = t >> 6;
+  const int row = lane & 15, c0 = 4 * (lane >> 4);
+  auto W = [&](int layer, int o, int i) -> float {
+    const int nin = layer == 1 ? kIn : kHid, nout = layer == 4 ? kOut : kHid;
+    if (o >= nout || i >= nin) return 0.f;
+    const int off = layer == 1 ? oW1 : layer == 2 ? oW2 : layer == 3 ? oW3 : oW4;
+    return param[off + o * nin + i];
+  };
+  auto pack = [&](float a, float b, float c, float d) -> uint64_t {
+    return (uint64_t)f2bf(a) | ((uint64_t)f2bf(b) << 16) | ((uint64_t)f2bf(c) << 32) |
+           ((uint64_t)f2bf(d) << 48);
+  };
+  if (frag < kFwdFrags) {
+    int layer, mt, ks;
+    if (frag < 4) { layer = 1; mt = frag; ks = 0; }
+    else if (frag < 20) { layer = 2; mt = (frag - 4) / 4; ks = (frag - 4) % 4; }
+    else if (frag < 36) { layer = 3; mt = (frag - 20) / 4; ks = (frag - 20) % 4; }
+    else { layer = 4; mt = 0; ks = frag - 36; }
+    const int o = mt * 16 + row, i = ks * 16 + c0;
+    fwd[frag * 64 + lane] = pack(W(layer, o, i), W(layer, o, i + 1), W(layer, o, i + 2), W(layer, o, i + 3));
+  } else if (frag < kFwdFrags + kBwdFrags) {
+    const int bf = frag - kFwdFrags;
+    int layer, mt, ks;  // A = W^T: M = in (mt), K = out (ks)
+    if (bf < 16) { layer = 2; mt = bf / 4; ks = bf % 4; }
+    else if (bf < 32) { layer = 3; mt = (bf - 16) / 4; ks = (bf - 16) % 4; }
+    else { layer = 4; mt = bf - 32; ks = 0; }
+    const int i = mt * 16 + row, o = ks * 16 + c0;
+    bwd[bf * 64 + lane] = pack(W(layer, o, i), W(layer, o + 1, i), W(layer, o + 2, i), W(layer, o + 3, i));
+  }
+}
+
+// transpose a 16x16 tile held in C layout (lane: column l%16, rows 4(l/16)+r) into the
+// A/B-fragment layout (lane: row l%16, columns 4(l/16)+e), through LDS
+__device__ __forceinline__ bf16x4 tile_transpose(uint16_t *lds, const f32x4 v) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, r0 = 4 * (lane >> 4);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) lds[(r0 + r) * 16 + col] = f2bf(v[r]);
+  __syncthreads();
+  bf16x4 out;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[e] = (short)lds[col * 16 + r0 + e];
+  __syncthreads();
+  return out;
+}
+
+// One wavefront per block; each block walks `chunk` samples in tiles of 16, accumulates
+// the loss gradient of every parameter and adds it to grad (fp32 atomics) once.
+// Loss per sample: (o1 - logit mu*)^2 + lam (o2 - log(|sigmoid(o1) - mu*| + 1e-12))^2,
+// averaged over the n samples of the step.
+__global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
+                                                  const float *__restrict__ feat,
+                                                  const float *__restrict__ target,
+                                                  float lam, float *__restrict__ grad,
+                                                  float *__restrict__ loss) {
+  __shared__ uint16_t tl[256];
+  const int lane = threadIdx.x;
+  const int col = lane & 15, r0 = 4 * (lane >> 4);
+  const long s0 = (long)blockIdx.x * chunk;
+  const float inv_n = 1.0f / (float)n;
+  // weight-gradient accumulators (C layout tiles): dW1 4 tiles (out) x 1 (in 16),
+  // dW2, dW3 16 tiles, dW4 1 (out 16) x 4 (in); bias gradients per lane
+  f32x4 gW1[4], gW2[16], gW3[16], gW4[4];
+  float gb1[4][4] = {}, gb2[4][4] = {}, gb3[4][4] = {}, gb4[2] = {0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 16; ++t) gW2[t] = gW3[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) gW1[t] = gW4[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  float lsum = 0.f;
+  for (int base = 0; base < chunk; base += 16) {
+    const long s = s0 + base + col;
+    const bool have = s < n;
+    // B fragment of the features: lanes 0..15 = features 0..3 of sample s
+    bf16x4 x = pack4(0.f, 0.f, 0.f, 0.f);
+    if (lane < 16 && have) x = pack4(feat[s], feat[n + s], feat[2 * n + s], feat[3 * n + s]);
+    f32x4 h1[4], h2[4], h3[4];
+    f32x4 o = mlp_tile_forward(m, x, h1, h2, h3);
+    // loss gradient (lanes 0..15 own the outputs of sample s)
+    f32x4 dO = {0.f, 0.f, 0.f, 0.f};
+    if (lane < 16 && have) {
+      const float mu = fminf(fmaxf(target[s], 1e-12f), 1.0f - 1e-7f);
+      const float t1 = logf(mu / (1.0f - mu));
+      const float c = 1.0f / (1.0f + expf(-o[0]));
+      const float t2 = logf(fabsf(c - mu) + 1e-12f);
+      const float e1 = o[0] - t1, e2 = o[1] - t2;
+      lsum += (e1 * e1 + lam * e2 * e2) * inv_n;
+      dO[0] = 2.0f * e1 * inv_n;
+      dO[1] = 2.0f * lam * e2 * inv_n;
+    }
+    // backward through layer 4: dH3^T = W4^T . dO^T
+    const bf16x4 bdo = pack4(dO[0], dO[1], dO[2], dO[3]);
+    f32x4 d3[4], d2[4], d1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(as_bf16x4(m.bwd[(32 + mt) * 64 + lane]), bdo, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = h3[mt][r] > 0.f ? acc[r] : 0.f;
+      d3[mt] = acc;
+    }
+    // layer 3 and 2 backward: dH_{l-1}^T = W_l^T . dZ_l^T, masked by the activation
+#pragma unroll
+    for (int layer = 3; layer >= 2; --layer) {
+      const int base = layer == 3 ? 16 : 0;
+      const f32x4 *dz = layer == 3 ? d3 : d2;
+      const f32x4 *hp = layer == 3 ? h2 : h1;
+      f32x4 *dout = layer == 3 ? d2 : d1;
+      bf16x4 bin[4];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) bin[ks] = pack4(dz[ks][0], dz[ks][1], dz[ks][2], dz[ks][3]);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+          acc = mfma(as_bf16x4(m.bwd[(base + mt * 4 + ks) * 64 + lane]), bin[ks], acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = hp[mt][r] > 0.f ? acc[r] : 0.f;
+        dout[mt] = acc;
+      }
+    }
+    // weight gradients dW = dZ^T . H (K = the 16 samples), operands transposed via LDS
+    {
+      const bf16x4 a4 = tile_transpose(tl, dO);  // A: rows = outputs, K = samples
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) gW4[nt] = mfma(a4, tile_transpose(tl, h3[nt]), gW4[nt]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const bf16x4 a3 = tile_transpose(tl, d3[mt]);
+      const bf16x4 a2 = tile_transpose(tl, d2[mt]);
+      const bf16x4 a1 = tile_transpose(tl, d1[mt]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        gW3[mt * 4 + nt] = mfma(a3, tile_transpose(tl, h2[nt]), gW3[mt * 4 + nt]);
+        gW2[mt * 4 + nt] = mfma(a2, tile_transpose(tl, h1[nt]), gW2[mt * 4 + nt]);
+      }
+      // layer 1 input tile: the features as a C-layout tile (rows = features)
+      f32x4 xin = {0.f, 0.f, 0.f, 0.f};
+      if (lane < 16) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xin[r] = bf2f((uint16_t)x[r]);
+      }
+      gW1[mt] = mfma(a1, tile_transpose(tl, xin), gW1[mt]);
+      // bias gradients: sum over the 16 samples of this tile (per lane first)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        gb1[mt][r] += d1[mt][r];
+        gb2[mt][r] += d2[mt][r];
+        gb3[mt][r] += d3[mt][r];
+      }
+    }
+    gb4[0] += dO[0];
+    gb4[1] += dO[1];
+  }
+  // reduce bias partials over the 16 sample columns (lanes sharing l/16)
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        gb1[mt][r] += __shfl_xor(gb1[mt][r], off, 64);
+        gb2[mt][r] += __shfl_xor(gb2[mt][r], off, 64);
+        gb3[mt][r] += __shfl_xor(gb3[mt][r], off, 64);
+      }
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) {
+    gb4[0] += __shfl_xor(gb4[0], off, 64);
+    gb4[1] += __shfl_xor(gb4[1], off, 64);
+    lsum += __shfl_xor(lsum, off, 64);
+  }
+  // scatter: C layout of dW tiles (lane: column = in index col, rows = out r0 + r)
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int o = mt * 16 + r0 + r;
+      if (col < kIn) atomicAdd(&grad[oW1 + o * kIn + col], gW1[mt][r]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        atomicAdd(&grad[oW2 + o * kHid + nt * 16 + col], gW2[mt * 4 + nt][r]);
+        atomicAdd(&grad[oW3 + o * kHid + nt * 16 + col], gW3[mt * 4 + nt][r]);
+      }
+      if (col == 0) {
+        atomicAdd(&grad[oB1 + o], gb1[mt][r]);
+        atomicAdd(&grad[oB2 + o], gb2[mt][r]);
+        atomicAdd(&grad[oB3 + o], gb3[mt][r]);
+      }
+    }
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int o = r0 + r;
+      if (o < kOut) atomicAdd(&grad[oW4 + o * kHid + nt * 16 + col], gW4[nt][r]);
+    }
+  if (lane == 0) {
+    atomicAdd(&grad[oB4 + 0], gb4[0]);
+    atomicAdd(&grad[oB4 + 1], gb4[1]);
+    atomicAdd(loss, lsum);
+  }
+}
+
+__global__ void k_mlp_adam(float *__restrict__ param, float *__restrict__ grad,
+                           float *__restrict__ m1, float *__restrict__ m2, float lr, float b1,
+                           float b2, float eps, float c1, float c2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kNParam) return;
+  const float g = grad[t];
+  const float a = b1 * m1[t] + (1.f - b1) * g;
+  const float v = b2 * m2[t] + (1.f - b2) * g * g;
+  m1[t] = a;
+  m2[t] = v;
+  param[t] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+  grad[t] = 0.f;
+}
+
+__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev m, long n,
+                                                    const float *__restrict__ feat,
+                                                    float *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const long s = (long)blockIdx.x * 16 + (lane & 15);
+  bf16x4 x = pack4(0.f, 0.f, 0.f, 0.f);
+  if (lane < 16 && s < n) x = pack4(feat[s], feat[n + s], feat[2 * n + s], feat[3 * n + s]);
+  const f32x4 o = mlp_tile_forward(m, x, nullptr, nullptr, nullptr);
+  if (lane < 16 && s < n) {
+    out[s] = o[0];
+    out[n + s] = o[1];
+  }
+}
+
+// ---- con2prim with the learned bracket (batched states) --------------------------------
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_con2prim_batch_learned(
+    DevParams p, MlpDev m, long n, const double *__restrict__ coords, double *__restrict__ cons,
+    double *__restrict__ prim, double *__restrict__ mu, double *__restrict__ fail,
+    int32_t *__restrict__ iters, int32_t *__restrict__ evals, int32_t *__restrict__ fallback) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool have = s < n;  // every lane stays for the wave-wide network evaluation
+  const long sc = have ? s : n - 1;
+  double X[3] = {0.0, 0.0, 0.0};
+  if (coords) {
+    X[0] = coords[sc];
+    X[1] = coords[n + sc];
+  }
+  const M g(p.a, X[0], X[1], X[2]);
+  double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + sc];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = 0.0;
+  C2PState z;
+  const bool ok = c2p_setup<MHD>(p, g, X, cs, pr, z);
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  if (ok) zone_features(m, z.res, f);
+  float o1, o2;
+  mlp_predict_wave(m, f, o1, o2);
+  if (!have) return;
+  double st = 0.0, root = mu[s], tt = 0.0;
+  int it = 0, ev = 0, fb = 0;
+  if (ok) {
+    const double c = 1.0 / (1.0 + exp(-(double)o1));
+    const double w = fmax((double)m.width_scale * exp((double)o2), (double)m.width_min);
+    const double lo = fmax(c - w, 0.0), hi = fmin(c + w, 1.0);
+    const double ylo = z.res(lo), yhi = z.res(hi);
+    ev = 2;
+    bool good = true;
+    if (ylo * yhi <= 0.0) {
+      root = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
+    } else {
+      fb = 1;
+      good = regula_falsi(z.res, 0.0, 1.0, p.tol, c, p.max_iter, root, it);
+      ev += 2;
+      if (!good) it = -1;
+    }
+    if (good) {
+      ev += it;
+      st = c2p_unwrap<MHD>(p, g, z, root, cs, pr, tt, sig);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) cons[v * n + s] = cs[v];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) prim[v * n + s] = pr[v];
+  mu[s] = root;
+  fail[s] = st;
+  if (iters) iters[s] = it;
+  if (evals) evals[s] = ev;
+  if (fallback) fallback[s] = fb;
+}
+
+// features of batched states (training data and tests)
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_c2p_features(DevParams p, MlpDev m, long n,
+                                                      const double *__restrict__ coords,
+                                                      const double *__restrict__ cons,
+                                                      float *__restrict__ feat) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double X[3] = {0.0, 0.0, 0.0};
+  if (coords) {
+    X[0] = coords[s];
+    X[1] = coords[n + s];
+  }
+  const M g(p.a, X[0], X[1], X[2]);
+  double cs[L::ncons], pr[L::nprim];
+#pragma unroll
+  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + s];
+  C2PState z;
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  if (c2p_setup<MHD>(p, g, X, cs, pr, z)) zone_features(m, z.res, f);
+  for (int i = 0; i < 4; ++i) feat[i * n + s] = f[i];
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+// ================================ host side ==============================================
+struct phx_mlp {
+  int device = 0;
+  float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
+  uint64_t *fwd = nullptr, *bwd = nullptr;
+  float fmean[4] = {0, 0, 0, 0}, fstd[4] = {1, 1, 1, 1};
+  float width_scale = 4.0f, width_min = 1e-9f;
+  long step = 0;
+};
+
+static MlpDev dev_view(const phx_mlp *m) {
+  MlpDev d;
+  d.fwd = m->fwd;
+  d.bwd = m->bwd;
+  d.param = m->param;
+  for (int i = 0; i < 4; ++i) {
+    d.fmean[i] = m->fmean[i];
+    d.fstd[i] = m->fstd[i];
+  }
+  d.width_scale = m->width_scale;
+  d.width_min = m->width_min;
+  return d;
+}
+
+static void repack(phx_mlp *m, hipStream_t s) {
+  const int threads = (kFwdFrags + kBwdFrags) * 64;
+  hipLaunchKernelGGL(k_mlp_pack, dim3((threads + 255) / 256), dim3(256), 0, s, m->param, m->fwd,
+                     m->bwd);
+}
+
+extern "C" {
+
+int phx_mlp_create(phx_ctx *ctx, phx_mlp **out, uint64_t seed) {
+  if (!ctx || !out) return PHX_ERR_INVALID;
+  phx_mlp *m = new (std::nothrow) phx_mlp();
+  if (!m) return PHX_ERR_NOMEM;
+  m->device = ctx->device;
+  bool okm = hipMalloc(&m->param, sizeof(float) * kNParam) == hipSuccess &&
+             hipMalloc(&m->grad, sizeof(float) * kNParam) == hipSuccess &&
+             hipMalloc(&m->m1, sizeof(float) * kNParam) == hipSuccess &&
+             hipMalloc(&m->m2, sizeof(float) * kNParam) == hipSuccess &&
+             hipMalloc(&m->loss, sizeof(float)) == hipSuccess &&
+             hipMalloc(&m->fwd, sizeof(uint64_t) * kFwdFrags * 64) == hipSuccess &&
+             hipMalloc(&m->bwd, sizeof(uint64_t) * kBwdFrags * 64) == hipSuccess;
+  if (!okm) {
+    phx_mlp_destroy(m);
+    return PHX_ERR_NOMEM;
+  }
+  // He-uniform initialization from a 64-bit LCG (deterministic per seed), zero biases
+  std::vector<float> w(kNParam, 0.f);
+  uint64_t st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  auto uni = [&]() {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return ((st >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+  };
+  auto fill = [&](int off, int nout, int nin) {
+    const double lim = std::sqrt(6.0 / nin);
+    for (int k = 0; k < nout * nin; ++k) w[off + k] = (float)(lim * uni());
+  };
+  fill(oW1, kHid, kIn);
+  fill(oW2, kHid, kHid);
+  fill(oW3, kHid, kHid);
+  fill(oW4, kOut, kHid);
+  if (phx_mlp_set_params(ctx, m, w.data()) != PHX_OK) {
+    phx_mlp_destroy(m);
+    return PHX_ERR_DEVICE;
+  }
+  *out = m;
+  return PHX_OK;
+}
+
+int phx_mlp_destroy(phx_mlp *m) {
+  if (!m) return PHX_OK;
+  (void)hipSetDevice(m->device);
+  for (void *p : {(void *)m->param, (void *)m->grad, (void *)m->m1, (void *)m->m2,
+                  (void *)m->loss, (void *)m->fwd, (void *)m->bwd})
+    if (p) (void)hipFree(p);
+  delete m;
+  return PHX_OK;
+}
+
+int phx_mlp_set_params(phx_ctx *ctx, phx_mlp *m, const float *host_params) {
+  if (!ctx || !m || !host_params) return PHX_ERR_INVALID;
+  if (hipMemcpy(m->param, host_params, sizeof(float) * kNParam, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->grad, 0, sizeof(float) * kNParam) != hipSuccess ||
+      hipMemset(m->m1, 0, sizeof(float) * kNParam) != hipSuccess ||
+      hipMemset(m->m2, 0, sizeof(float) * kNParam) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "mlp upload failed");
+  m->step = 0;
+  repack(m, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->fail(PHX_ERR_DEVICE, "mlp pack failed");
+  return PHX_OK;
+}
+
+int phx_mlp_get_params(phx_ctx *ctx, const phx_mlp *m, float *host_params) {
+  if (!ctx || !m || !host_params) return PHX_ERR_INVALID;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipMemcpy(host_params, m->param, sizeof(float) * kNParam, hipMemcpyDeviceToHost) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "mlp download failed");
+  return PHX_OK;
+}
+
+int phx_mlp_set_normalization(phx_ctx *ctx, phx_mlp *m, const float mean[4], const float std_[4],
+                              float width_scale, float width_min) {
+  if (!ctx || !m || !mean || !std_) return PHX_ERR_INVALID;
+  for (int i = 0; i < 4; ++i) {
+    if (!(std_[i] > 0.f)) return ctx->fail(PHX_ERR_INVALID, "feature std must be positive");
+    m->fmean[i] = mean[i];
+    m->fstd[i] = std_[i];
+  }
+  if (!(width_scale > 0.f) || !(width_min >= 0.f)) return ctx->fail(PHX_ERR_INVALID, "bad width");
+  m->width_scale = width_scale;
+  m->width_min = width_min;
+  return PHX_OK;
+}
+
+int phx_mlp_gradient(phx_ctx *ctx, phx_mlp *m, int64_t n, const float *features,
+                     const float *target_mu, float lam, float *loss_out, phx_stream stream) {
+  if (!ctx || !m || n <= 0 || !features || !target_mu) return PHX_ERR_INVALID;
+  hipStream_t s = ctx->stream_of(stream);
+  constexpr int chunk = 256;
+  const long blocks = (n + chunk - 1) / chunk;
+  (void)hipMemsetAsync(m->loss, 0, sizeof(float), s);
+  hipLaunchKernelGGL(k_mlp_train, dim3((unsigned)blocks), dim3(64), 0, s, dev_view(m), (long)n,
+                     chunk, features, target_mu, lam, m->grad, m->loss);
+  if (loss_out) (void)hipMemcpyAsync(loss_out, m->loss, sizeof(float), hipMemcpyDeviceToDevice, s);
+  return ctx->check_launch();
+}
+
+int phx_mlp_read_gradient(phx_ctx *ctx, phx_mlp *m, float *host_grad) {
+  if (!ctx || !m || !host_grad) return PHX_ERR_INVALID;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipMemcpy(host_grad, m->grad, sizeof(float) * kNParam, hipMemcpyDeviceToHost) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "gradient download failed");
+  return PHX_OK;
+}
+
+int phx_mlp_zero_gradient(phx_ctx *ctx, phx_mlp *m) {
+  if (!ctx || !m) return PHX_ERR_INVALID;
+  if (hipMemset(m->grad, 0, sizeof(float) * kNParam) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "memset failed");
+  return PHX_OK;
+}
+
+int phx_mlp_adam_step(phx_ctx *ctx, phx_mlp *m, float lr, phx_stream stream) {
+  if (!ctx || !m || !(lr > 0.f)) return PHX_ERR_INVALID;
+  hipStream_t s = ctx->stream_of(stream);
+  m->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kNParam + 255) / 256), dim3(256), 0, s, m->param, m->grad,
+                     m->m1, m->m2, lr, b1, b2, 1e-8f, c1, c2);
+  repack(m, s);
+  return ctx->check_launch();
+}
+
+int phx_mlp_train_step(phx_ctx *ctx, phx_mlp *m, int64_t n, const float *features,
+                       const float *target_mu, float lr, float lam, float *loss_out,
+                       phx_stream stream) {
+  int rc = phx_mlp_gradient(ctx, m, n, features, target_mu, lam, loss_out, stream);
+  if (rc) return rc;
+  return phx_mlp_adam_step(ctx, m, lr, stream);
+}
+
+int phx_mlp_forward(phx_ctx *ctx, const phx_mlp *m, int64_t n, const float *features,
+                    float *out, phx_stream stream) {
+  if (!ctx || !m || n < 0 || (n > 0 && (!features || !out))) return PHX_ERR_INVALID;
+  if (n == 0) return PHX_OK;
+  hipStream_t s = ctx->stream_of(stream);
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, s, dev_view(m),
+                     (long)n, features, out);
+  return ctx->check_launch();
+}
+
+int phx_con2prim_features(phx_ctx *ctx, const phx_fluid_params *p, const phx_mlp *m, int64_t n,
+                          const double *coords, const double *cons, float *features,
+                          phx_stream stream) {
+  if (!ctx || !m) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (n <= 0) return n == 0 ? PHX_OK : ctx->fail(PHX_ERR_INVALID, "negative batch");
+  if (!cons || !features || (p->geom.kind != PHX_GEOM_MINKOWSKI && !coords))
+    return ctx->fail(PHX_ERR_INVALID, "null field");
+  const DevParams dp = make_dev_params(p);
+  hipStream_t s = ctx->stream_of(stream);
+  const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+  const MlpDev md = dev_view(m);
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_c2p_features<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
+    else hipLaunchKernelGGL((k_c2p_features<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_c2p_features<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
+    else hipLaunchKernelGGL((k_c2p_features<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
+  }
+  return ctx->check_launch();
+}
+
+int phx_con2prim_batch_learned(phx_ctx *ctx, const phx_fluid_params *p, const phx_mlp *m,
+                               int64_t n, const double *coords, double *cons, double *prim,
+                               double *mu, double *fail, int32_t *iters, int32_t *evals,
+                               int32_t *fallback, phx_stream stream) {
+  if (!ctx || !m) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (n <= 0) return n == 0 ? PHX_OK : ctx->fail(PHX_ERR_INVALID, "negative batch");
+  if (!cons || !prim || !mu || !fail || (p->geom.kind != PHX_GEOM_MINKOWSKI && !coords))
+    return ctx->fail(PHX_ERR_INVALID, "null field");
+  const DevParams dp = make_dev_params(p);
+  hipStream_t s = ctx->stream_of(stream);
+  const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+  const MlpDev md = dev_view(m);
+#define LARGS dp, md, (long)n, coords, cons, prim, mu, fail, iters, evals, fallback
+  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
+    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch_learned<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, LARGS);
+    else hipLaunchKernelGGL((k_con2prim_batch_learned<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, LARGS);
+  } else {
+    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch_learned<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, LARGS);
+    else hipLaunchKernelGGL((k_con2prim_batch_learned<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, LARGS);
+  }
+#undef LARGS
+  return ctx->check_launch();
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+"""CPU reference of the learned-bracket network (test infrastructure).
+
+Restates the library's forward pass, loss and backward pass in numpy, rounding every MFMA
+operand to BF16 (round-to-nearest-even) exactly where the device kernels do and
+accumulating in float64, so device gradients can be checked against it.  With
+``bf16=False`` it is the plain float64 network, which the finite-difference test pins.
+"""
+import numpy as np
+
+NIN, NHID, NOUT = 4, 64, 2
+SHAPES = [("W1", (NHID, NIN)), ("b1", (NHID,)), ("W2", (NHID, NHID)), ("b2", (NHID,)),
+          ("W3", (NHID, NHID)), ("b3", (NHID,)), ("W4", (NOUT, NHID)), ("b4", (NOUT,))]
+NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
+
+
+def unpack(w):
+    out, o = {}, 0
+    for name, sh in SHAPES:
+        k = int(np.prod(sh))
+        out[name] = np.asarray(w[o:o + k], np.float64).reshape(sh)
+        o += k
+    return out
+
+
+def pack(d):
+    return np.concatenate([np.asarray(d[name], np.float64).ravel() for name, _ in SHAPES])
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    a = np.ascontiguousarray(np.asarray(x, np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    special = (u & 0x7F800000) == 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    r = np.where(special, (u >> 16) << 16, r).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def forward(w, x, bf=True):
+    """x: (n, 4) standardized features.  Returns outputs (n, 2) and the activations."""
+    p = unpack(w)
+    q = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    W = {k: (q(p[k]) if k.startswith("W") else p[k]) for k in p}
+    h0 = q(x)
+    z1 = h0 @ W["W1"].T + p["b1"]
+    h1 = np.maximum(z1, 0)
+    z2 = q(h1) @ W["W2"].T + p["b2"]
+    h2 = np.maximum(z2, 0)
+    z3 = q(h2) @ W["W3"].T + p["b3"]
+    h3 = np.maximum(z3, 0)
+    o = q(h3) @ W["W4"].T + p["b4"]
+    return o, (h0, h1, h2, h3)
+
+
+def loss_and_grad(w, x, mu, lam, bf=True):
+    """Loss (o1 - logit mu)^2 + lam (o2 - log(|sigmoid o1 - mu| + 1e-12))^2, mean over n;
+    the second target is held fixed (no gradient through it)."""
+    n = x.shape[0]
+    p = unpack(w)
+    q = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    o, (h0, h1, h2, h3) = forward(w, x, bf)
+    m = np.clip(mu, 1e-12, 1 - 1e-7)
+    t1 = np.log(m / (1 - m))
+    c = 1 / (1 + np.exp(-o[:, 0]))
+    t2 = np.log(np.abs(c - m) + 1e-12)
+    e1, e2 = o[:, 0] - t1, o[:, 1] - t2
+    L = np.sum(e1 * e1 + lam * e2 * e2) / n
+    dO = np.stack([2 * e1 / n, 2 * lam * e2 / n], axis=1)
+    W4, W3, W2 = q(p["W4"]), q(p["W3"]), q(p["W2"])
+    d3 = (q(dO) @ W4) * (h3 > 0)
+    d2 = (q(d3) @ W3) * (h2 > 0)
+    d1 = (q(d2) @ W2) * (h1 > 0)
+    g = {"W4": q(dO).T @ q(h3), "b4": dO.sum(0),
+         "W3": q(d3).T @ q(h2), "b3": d3.sum(0),
+         "W2": q(d2).T @ q(h1), "b2": d2.sum(0),
+         "W1": q(d1).T @ h0, "b1": d1.sum(0)}
+    return L, pack(g)
+
+
+def init(seed=0):
+    rng = np.random.default_rng(seed)
+    d = {}
+    for name, sh in SHAPES:
+        if name.startswith("W"):
+            d[name] = rng.uniform(-1, 1, sh) * np.sqrt(6.0 / sh[1])
+        else:
+            d[name] = rng.normal(0, 0.1, sh)
+    return pack(d)

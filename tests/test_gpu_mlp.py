@@ -1,0 +1,166 @@
+"""The learned con2prim bracket on the GPU: MFMA forward and gradient against the CPU
+reference (tests/_mlp_ref.py), training on solver-produced pairs, and the learned solve
+against the plain solver and the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import _mlp_ref as R
+import _oracle as O
+from phoebus_amd import _abi as A
+from phoebus_amd.api import Mlp, Phx, fluid_params
+from phoebus_amd.device import DeviceArray, synchronize
+from phoebus_amd.workloads import batched_con2prim_states
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def phx():
+    ctx = Phx(0)
+    yield ctx
+    ctx.close()
+
+
+def test_mfma_forward_matches_reference(phx):
+    say("[gpu] mlp forward vs reference")
+    rng = np.random.default_rng(11)
+    w = R.init(5)
+    net = Mlp(phx)
+    net.set_params(w.astype(np.float32))
+    n = 1000
+    x = rng.normal(size=(n, 4)).astype(np.float32)
+    d_x = DeviceArray.from_host(np.ascontiguousarray(x.T))
+    d_o = DeviceArray((2, n), np.float32)
+    net.forward(n, d_x, d_o)
+    synchronize()
+    o_gpu = d_o.get().T
+    o_ref, _ = R.forward(w.astype(np.float32).astype(np.float64), x.astype(np.float64))
+    scale = np.max(np.abs(o_ref))
+    err = np.max(np.abs(o_gpu - o_ref))
+    assert err < 2e-3 * scale, (err, scale)  # fp32 accumulation, bf16 re-rounding of hiddens
+    net.close()
+
+
+def test_mfma_gradient_matches_reference(phx):
+    say("[gpu] mlp gradient vs reference")
+    rng = np.random.default_rng(12)
+    w = R.init(6).astype(np.float32)
+    net = Mlp(phx)
+    net.set_params(w)
+    n = 2048
+    x = rng.normal(size=(n, 4)).astype(np.float32)
+    mu = rng.uniform(0.02, 0.98, n).astype(np.float32)
+    d_x = DeviceArray.from_host(np.ascontiguousarray(x.T))
+    d_mu = DeviceArray.from_host(mu)
+    d_loss = DeviceArray((1,), np.float32)
+    net.zero_gradient()
+    net.gradient(n, d_x, d_mu, 0.3, d_loss)
+    synchronize()
+    g_gpu = net.read_gradient()
+    L_ref, g_ref = R.loss_and_grad(w.astype(np.float64), x.astype(np.float64),
+                                   mu.astype(np.float64), 0.3)
+    assert abs(d_loss.get()[0] - L_ref) < 1e-3 * abs(L_ref)
+    o = 0
+    for name, sh in R.SHAPES:
+        k = int(np.prod(sh))
+        gg, gr = g_gpu[o:o + k], g_ref[o:o + k]
+        scale = np.max(np.abs(gr)) + 1e-30
+        assert np.max(np.abs(gg - gr)) < 2e-2 * scale, (name, np.max(np.abs(gg - gr)), scale)
+        assert np.corrcoef(gg, gr)[0, 1] > 0.999, name
+        o += k
+    net.close()
+
+
+def _solver_pairs(phx, p, rng, n):
+    coords, prim = batched_con2prim_states(rng, n)
+    d_coords = DeviceArray.from_host(coords)
+    d_prim = DeviceArray.from_host(prim)
+    d_cons = DeviceArray((8, n))
+    phx.prim_to_con_batch(p, n, d_coords, d_prim, d_cons)
+    d_out = DeviceArray((10, n))
+    d_mu = DeviceArray.from_host(np.zeros(n))
+    d_fail = DeviceArray((n,))
+    d_it = DeviceArray((n,), np.int32)
+    cons0 = d_cons.get()
+    phx.con2prim_batch(p, n, d_coords, d_cons, d_out, d_mu, d_fail, d_it)
+    synchronize()
+    return dict(coords=coords, cons=cons0, prim=d_out.get(), mu=d_mu.get(), fail=d_fail.get(),
+                iters=d_it.get(), d_coords=d_coords)
+
+
+def train_network(phx, p, rng, n_train=1 << 16, steps=400, batch=8192, lr=3e-3, lam=0.2):
+    """Plain-solver pairs -> features -> Adam on the device kernels."""
+    data = _solver_pairs(phx, p, rng, n_train)
+    keep = data["fail"] == 1.0
+    net = Mlp(phx, seed=3)
+    net.set_normalization([0, 0, 0, 0], [1, 1, 1, 1])
+    d_cons = DeviceArray.from_host(data["cons"])
+    d_feat = DeviceArray((4, n_train), np.float32)
+    net.features(p, n_train, data["d_coords"], d_cons, d_feat)
+    synchronize()
+    raw = d_feat.get()[:, keep]
+    mean, std = raw.mean(axis=1), raw.std(axis=1) + 1e-6
+    net.set_normalization(mean, std)
+    net.features(p, n_train, data["d_coords"], d_cons, d_feat)
+    synchronize()
+    feat = d_feat.get()[:, keep]
+    mu = data["mu"][keep].astype(np.float32)
+    m = feat.shape[1]
+    d_loss = DeviceArray((1,), np.float32)
+    losses = []
+    for step in range(steps):
+        idx = rng.integers(0, m, batch)
+        fb = DeviceArray.from_host(np.ascontiguousarray(feat[:, idx]))
+        tb = DeviceArray.from_host(mu[idx])
+        net.train_step(batch, fb, tb, lr, lam, d_loss)
+        if step % 50 == 0 or step == steps - 1:
+            synchronize()
+            losses.append(float(d_loss.get()[0]))
+    return net, losses
+
+
+def test_training_cuts_iterations_and_keeps_results(phx):
+    say("[gpu] mlp training on solver pairs + learned con2prim")
+    p = fluid_params(gamma=4 / 3, mhd=True, geom=A.GEOM_SPHERICAL_KS, a=0.9375, c2p_tol=1e-10,
+                     c2p_max_iter=100)
+    net, losses = train_network(phx, p, np.random.default_rng(777))
+    assert losses[-1] < 0.5 * losses[0], losses
+    # disjoint evaluation set
+    n = 1 << 15
+    ev = _solver_pairs(phx, p, np.random.default_rng(20261202), n)
+    d_cons = DeviceArray.from_host(ev["cons"])
+    d_out = DeviceArray((10, n))
+    d_mu = DeviceArray.from_host(np.zeros(n))
+    d_fail = DeviceArray((n,))
+    d_it = DeviceArray((n,), np.int32)
+    d_ev = DeviceArray((n,), np.int32)
+    d_fb = DeviceArray((n,), np.int32)
+    net.con2prim_batch(p, n, ev["d_coords"], d_cons, d_out, d_mu, d_fail, d_it, d_ev, d_fb)
+    synchronize()
+    fail, it, fb, mu = d_fail.get(), d_it.get(), d_fb.get(), d_mu.get()
+    ok = ev["fail"] == 1.0
+    assert np.array_equal(fail, ev["fail"])
+    plain, learned = ev["iters"][ok].mean(), it[ok].mean()
+    say(f"    plain {plain:.3f} learned {learned:.3f} fallback {fb[ok].mean():.4f}")
+    assert learned * 1.5 < plain
+    assert fb[ok].mean() < 0.05
+    # same root to the solver tolerance, same primitives to what that tolerance allows
+    assert np.max(np.abs(mu[ok] - ev["mu"][ok])) <= 4 * p.c2p_tol
+    pr, pp = d_out.get()[:, ok], ev["prim"][:, ok]
+    scale = np.maximum(np.abs(pp), np.max(np.abs(pp[1:4]), axis=0))
+    assert np.all(np.abs(pr - pp) <= 1e-6 * scale)
+    # and the oracle agrees with the learned result on a sample
+    sel = np.flatnonzero(ok)[:512]
+    op = O.default_params(gamma=4 / 3, mhd=True, geom=O.GEOM_SPHKS, a=0.9375, tol=1e-10,
+                          max_iter=100)
+    ref = O.c2p_batch(op, np.ascontiguousarray(ev["coords"][:, sel]),
+                      np.ascontiguousarray(ev["cons"][:, sel]))
+    assert np.max(np.abs(ref["mu"] - mu[sel])) <= 4 * p.c2p_tol
+    net.close()
