@@ -10,6 +10,13 @@
 // per-lane fragments (lane l: row l%16, columns 4(l/16)..+3).  The weight-gradient
 // products need the other orientation and transpose 16x16 tiles through LDS.
 //
+// Precision.  A plain BF16 forward pass resolves the root to a few percent, which buys a
+// bracket only ~100x narrower than [0, 1].  The forward products therefore run in split
+// BF16: weights and activations are carried as hi + lo BF16 pairs (lo = the BF16 rounding
+// of the remainder) and each product is hi.hi + hi.lo + lo.hi on MFMA with FP32
+// accumulation (about 16 significant bits, 3x the MFMA work of a plain pass).  The
+// backward pass uses the hi parts only.
+//
 // Features (scale-free conserved invariants of the con2prim set-up): log q, log r^2,
 // log b^2, log (r.b)^2 with q = tau/D, standardized.  Outputs: o1 = logit of the bracket
 // centre, o2 = log of the bracket half-width.  The solver evaluates the residual at the
@@ -46,7 +53,7 @@ constexpr int kBwdFrags = 4 * 4 + 4 * 4 + 4 * 1;          // L2, L3, L4 (W^T)
 constexpr int fwdBase[4] = {0, 4, 20, 36};
 
 struct MlpDev {
-  const uint64_t *fwd;   // [kFwdFrags][64] packed bf16x4
+  const uint64_t *fwd;   // [2][kFwdFrags][64] packed bf16x4: hi parts, then lo parts
   const uint64_t *bwd;   // [kBwdFrags][64]
   const float *param;    // [kNParam] fp32 master (biases read from here)
   float fmean[4], fstd[4];
@@ -79,10 +86,31 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // ---- forward for one tile of 16 zones -------------------------------------------------
+struct Split {
+  bf16x4 hi, lo;
+};
+__device__ __forceinline__ Split split4(float a, float b, float c, float d) {
+  Split s;
+  s.hi = pack4(a, b, c, d);
+  s.lo = pack4(a - bf2f((uint16_t)s.hi[0]), b - bf2f((uint16_t)s.hi[1]),
+               c - bf2f((uint16_t)s.hi[2]), d - bf2f((uint16_t)s.hi[3]));
+  return s;
+}
+// split-BF16 product accumulate: W . a with W = (whi, wlo) fragments of one A tile
+__device__ __forceinline__ f32x4 mfma3(const MlpDev &m, int frag, int lane, const Split &a,
+                                      f32x4 acc) {
+  const bf16x4 wh = as_bf16x4(m.fwd[frag * 64 + lane]);
+  const bf16x4 wl = as_bf16x4(m.fwd[(kFwdFrags + frag) * 64 + lane]);
+  acc = mfma(wh, a.hi, acc);
+  acc = mfma(wh, a.lo, acc);
+  acc = mfma(wl, a.hi, acc);
+  return acc;
+}
+
 // x: B fragment of the standardized features (lanes 0..15: features 0..3 of zone l; others
-// zero).  h1..h3 (optional): post-activation tiles, 4 neuron tiles each.  Returns the
-// output tile (lanes 0..15 hold o1, o2 of zone l in elements 0, 1).
-__device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, bf16x4 x, f32x4 *h1,
+// zero), as a hi/lo pair.  h1..h3 (optional): post-activation tiles, 4 neuron tiles each.
+// Returns the output tile (lanes 0..15 hold o1, o2 of zone l in elements 0, 1).
+__device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &x, f32x4 *h1,
                                                   f32x4 *h2, f32x4 *h3) {
   const int lane = threadIdx.x & 63;
   const int r0 = 4 * (lane >> 4);
@@ -91,7 +119,7 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, bf16x4 x, f32
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma(as_bf16x4(m.fwd[(fwdBase[0] + mt) * 64 + lane]), x, acc);
+    acc = mfma3(m, fwdBase[0] + mt, lane, x, acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.param[oB1 + mt * 16 + r0 + r], 0.f);
     a[mt] = acc;
@@ -102,15 +130,14 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, bf16x4 x, f32
   for (int layer = 2; layer <= 3; ++layer) {
     const int base = fwdBase[layer - 1];
     const int ob = layer == 2 ? oB2 : oB3;
-    bf16x4 bin[4];
+    Split bin[4];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) bin[ks] = pack4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]);
+    for (int ks = 0; ks < 4; ++ks) bin[ks] = split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-        acc = mfma(as_bf16x4(m.fwd[(base + mt * 4 + ks) * 64 + lane]), bin[ks], acc);
+      for (int ks = 0; ks < 4; ++ks) acc = mfma3(m, base + mt * 4 + ks, lane, bin[ks], acc);
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.param[ob + mt * 16 + r0 + r], 0.f);
       bcur[mt] = acc;
@@ -126,8 +153,7 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, bf16x4 x, f32
   f32x4 o = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks)
-    o = mfma(as_bf16x4(m.fwd[(fwdBase[3] + ks) * 64 + lane]),
-             pack4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]), o);
+    o = mfma3(m, fwdBase[3] + ks, lane, split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]), o);
   if (r0 == 0) {
     o[0] += m.param[oB4 + 0];
     o[1] += m.param[oB4 + 1];
@@ -155,7 +181,7 @@ __device__ __forceinline__ void mlp_predict_wave(const MlpDev &m, const float f[
     const int src = zt * 16 + (lane & 15);
     const float g0 = __shfl(f[0], src, 64), g1 = __shfl(f[1], src, 64);
     const float g2 = __shfl(f[2], src, 64), g3 = __shfl(f[3], src, 64);
-    const bf16x4 x = lane < 16 ? pack4(g0, g1, g2, g3) : pack4(0.f, 0.f, 0.f, 0.f);
+    const Split x = lane < 16 ? split4(g0, g1, g2, g3) : split4(0.f, 0.f, 0.f, 0.f);
     const f32x4 o = mlp_tile_forward(m, x, nullptr, nullptr, nullptr);
     const float t1 = __shfl(o[0], lane & 15, 64), t2 = __shfl(o[1], lane & 15, 64);
     if ((lane >> 4) == zt) {
@@ -190,7 +216,13 @@ __global__ void k_mlp_pack(const float *__restrict__ param, uint64_t *__restrict
     else if (frag < 36) { layer = 3; mt = (frag - 20) / 4; ks = (frag - 20) % 4; }
     else { layer = 4; mt = 0; ks = frag - 36; }
     const int o = mt * 16 + row, i = ks * 16 + c0;
-    fwd[frag * 64 + lane] = pack(W(layer, o, i), W(layer, o, i + 1), W(layer, o, i + 2), W(layer, o, i + 3));
+    float w4[4], l4[4];
+    for (int e = 0; e < 4; ++e) {
+      w4[e] = W(layer, o, i + e);
+      l4[e] = w4[e] - bf2f(f2bf(w4[e]));
+    }
+    fwd[frag * 64 + lane] = pack(w4[0], w4[1], w4[2], w4[3]);
+    fwd[(kFwdFrags + frag) * 64 + lane] = pack(l4[0], l4[1], l4[2], l4[3]);
   } else if (frag < kFwdFrags + kBwdFrags) {
     const int bf = frag - kFwdFrags;
     int layer, mt, ks;  // A = W^T: M = in (mt), K = out (ks)
@@ -244,10 +276,11 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
     const long s = s0 + base + col;
     const bool have = s < n;
     // B fragment of the features: lanes 0..15 = features 0..3 of sample s
-    bf16x4 x = pack4(0.f, 0.f, 0.f, 0.f);
-    if (lane < 16 && have) x = pack4(feat[s], feat[n + s], feat[2 * n + s], feat[3 * n + s]);
+    Split xs = split4(0.f, 0.f, 0.f, 0.f);
+    if (lane < 16 && have) xs = split4(feat[s], feat[n + s], feat[2 * n + s], feat[3 * n + s]);
+    const bf16x4 x = xs.hi;
     f32x4 h1[4], h2[4], h3[4];
-    f32x4 o = mlp_tile_forward(m, x, h1, h2, h3);
+    f32x4 o = mlp_tile_forward(m, xs, h1, h2, h3);
     // loss gradient (lanes 0..15 own the outputs of sample s)
     f32x4 dO = {0.f, 0.f, 0.f, 0.f};
     if (lane < 16 && have) {
@@ -394,8 +427,8 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev m, long n,
                                                     float *__restrict__ out) {
   const int lane = threadIdx.x;
   const long s = (long)blockIdx.x * 16 + (lane & 15);
-  bf16x4 x = pack4(0.f, 0.f, 0.f, 0.f);
-  if (lane < 16 && s < n) x = pack4(feat[s], feat[n + s], feat[2 * n + s], feat[3 * n + s]);
+  Split x = split4(0.f, 0.f, 0.f, 0.f);
+  if (lane < 16 && s < n) x = split4(feat[s], feat[n + s], feat[2 * n + s], feat[3 * n + s]);
   const f32x4 o = mlp_tile_forward(m, x, nullptr, nullptr, nullptr);
   if (lane < 16 && s < n) {
     out[s] = o[0];
@@ -536,7 +569,7 @@ int phx_mlp_create(phx_ctx *ctx, phx_mlp **out, uint64_t seed) {
              hipMalloc(&m->m1, sizeof(float) * kNParam) == hipSuccess &&
              hipMalloc(&m->m2, sizeof(float) * kNParam) == hipSuccess &&
              hipMalloc(&m->loss, sizeof(float)) == hipSuccess &&
-             hipMalloc(&m->fwd, sizeof(uint64_t) * kFwdFrags * 64) == hipSuccess &&
+             hipMalloc(&m->fwd, sizeof(uint64_t) * 2 * kFwdFrags * 64) == hipSuccess &&
              hipMalloc(&m->bwd, sizeof(uint64_t) * kBwdFrags * 64) == hipSuccess;
   if (!okm) {
     phx_mlp_destroy(m);

@@ -1,8 +1,9 @@
 """CPU reference of the learned-bracket network (test infrastructure).
 
 Restates the library's forward pass, loss and backward pass in numpy, rounding every MFMA
-operand to BF16 (round-to-nearest-even) exactly where the device kernels do and
-accumulating in float64, so device gradients can be checked against it.  With
+operand to BF16 (round-to-nearest-even) exactly where the device kernels do -- split
+hi + lo pairs in the forward products, hi parts in the backward ones -- and accumulating
+in float64, so device outputs and gradients can be checked against it.  With
 ``bf16=False`` it is the plain float64 network, which the finite-difference test pins.
 """
 import numpy as np
@@ -36,19 +37,35 @@ def bf16(x):
     return r.view(np.float32).astype(np.float64)
 
 
+def _split(a):
+    """hi + lo BF16 pair of float32 values: hi = bf16(a), lo = bf16(a - hi)."""
+    a32 = np.asarray(a, np.float32)
+    hi = bf16(a32)
+    lo = bf16((a32.astype(np.float64) - hi).astype(np.float32))
+    return hi, lo
+
+
+def _mm(a, W, bf):
+    """a @ W.T as the device forms it: split BF16 (hi.hi + lo.hi + hi.lo) or exact."""
+    if not bf:
+        return np.asarray(a, np.float64) @ np.asarray(W, np.float64).T
+    ah, al = _split(a)
+    wh, wl = _split(W)
+    return ah @ wh.T + al @ wh.T + ah @ wl.T
+
+
 def forward(w, x, bf=True):
-    """x: (n, 4) standardized features.  Returns outputs (n, 2) and the activations."""
+    """x: (n, 4) standardized features.  Returns outputs (n, 2) and the activations
+    (h0 = the BF16 hi part of the features, as the weight gradient of layer 1 uses it)."""
     p = unpack(w)
-    q = bf16 if bf else (lambda a: np.asarray(a, np.float64))
-    W = {k: (q(p[k]) if k.startswith("W") else p[k]) for k in p}
-    h0 = q(x)
-    z1 = h0 @ W["W1"].T + p["b1"]
+    h0 = bf16(x) if bf else np.asarray(x, np.float64)
+    z1 = _mm(x, p["W1"], bf) + p["b1"]
     h1 = np.maximum(z1, 0)
-    z2 = q(h1) @ W["W2"].T + p["b2"]
+    z2 = _mm(h1, p["W2"], bf) + p["b2"]
     h2 = np.maximum(z2, 0)
-    z3 = q(h2) @ W["W3"].T + p["b3"]
+    z3 = _mm(h2, p["W3"], bf) + p["b3"]
     h3 = np.maximum(z3, 0)
-    o = q(h3) @ W["W4"].T + p["b4"]
+    o = _mm(h3, p["W4"], bf) + p["b4"]
     return o, (h0, h1, h2, h3)
 
 

@@ -44,7 +44,7 @@ def test_mfma_forward_matches_reference(phx):
     o_ref, _ = R.forward(w.astype(np.float32).astype(np.float64), x.astype(np.float64))
     scale = np.max(np.abs(o_ref))
     err = np.max(np.abs(o_gpu - o_ref))
-    assert err < 2e-3 * scale, (err, scale)  # fp32 accumulation, bf16 re-rounding of hiddens
+    assert err < 1e-4 * scale, (err, scale)  # split-BF16 products, FP32 accumulation
     net.close()
 
 
