@@ -11,9 +11,11 @@
  * b2[64] W3[64][64] b3[64] W4[2][64] b4[2] (row = output neuron).
  * Features: (log(q + 1e-12), log(r^2 + 1e-12), log(b^2 + 1e-12), log((r.b)^2 + 1e-24))
  * standardized by (x - mean) / std, with q = tau/D, r_i = S_i/D, b^i = B^i/sqrt(D) as in
- * con2prim_robust.hpp:327-361.  Outputs: o1 = logit of the bracket centre c,
- * o2 = log of the bracket half-width; the bracket is c -+ max(width_scale * exp(o2),
- * width_min) clipped to [0, 1].
+ * con2prim_robust.hpp:327-361.  Outputs are relative to the prior z0 = max(1 + q - b^2/2, 1)
+ * (an estimate of hW = 1/mu from the energy equation, between hW/Gamma and hW for b = 0,
+ * evaluated in fp32 from the standardized features): o1 = log(c z0) for the bracket centre c, o2 = log of the
+ * half-width relative to c; the bracket is c (1 -+ max(width_scale * exp(o2), width_min))
+ * clipped to [0, 1].
  * All array arguments are device pointers.
  */
 #ifndef PHX_MLP_H_
@@ -48,8 +50,9 @@ int phx_con2prim_features(phx_ctx *ctx, const phx_fluid_params *p, const phx_mlp
 int phx_mlp_forward(phx_ctx *ctx, const phx_mlp *mlp, int64_t n, const float *features,
                     float *out, phx_stream stream);
 
-/* Training.  Loss per sample (averaged over n): (o1 - logit mu*)^2
- * + lam (o2 - log(|sigmoid(o1) - mu*| + 1e-12))^2, with mu* the plain solver's root.
+/* Training.  Loss per sample (averaged over n), with t = log(mu* z0) and mu* the plain
+ * solver's root: (o1 - t)^2 + lam (o2 - log(|1 - exp(t - o1)| + 1e-12))^2, the second
+ * target held fixed.
  * phx_mlp_gradient accumulates the loss gradient into the context-free gradient buffer
  * of the network (and the loss into *loss_out, device, nullable); phx_mlp_adam_step
  * applies Adam (beta 0.9, 0.999) and clears the gradient; phx_mlp_train_step does both. */

@@ -1086,6 +1086,46 @@ double orc_estimate_dt(const orc_params *p, const orc_block *b, const double *ce
   return cfl * min_dt;
 }
 
+/* Physical boundaries for conserved variables (bc_vars = conserved), one face after the
+ * other in the order x1 inner, x1 outer, x2 inner, ...: outflow copies the first interior
+ * layer scaled by ratio(detgamma(ghost), detgamma(ref)) (phoebus_boundaries.cpp:118-150);
+ * periodic copies the opposite interior layers (a block that is its own neighbour).  The
+ * ghost slab of face d spans the entire block in the other dimensions. */
+void orc_apply_bcs(const orc_params *p, const orc_block *b, int nvar, double *q,
+                   const int kinds[6]) {
+  int n[3];
+  orc_block_dims(b, n);
+  const long nc = (long)n[0] * n[1] * n[2];
+  for (int d = 0; d < b->ndim; ++d) {
+    const int is = b->ng, ie = b->ng + b->nx[d] - 1, nint = b->nx[d];
+    for (int side = 0; side < 2; ++side) {
+      const int periodic = kinds[2 * d + side] == ORC_BC_PERIODIC;
+      const int g0 = side == 0 ? 0 : ie + 1, g1 = side == 0 ? is - 1 : n[d] - 1;
+      for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+          for (int i = 0; i < n[0]; ++i) {
+            int idx[3] = {i, j, k};
+            if (idx[d] < g0 || idx[d] > g1) continue;
+            int ref[3] = {i, j, k};
+            if (periodic)
+              ref[d] = side == 0 ? idx[d] + nint : idx[d] - nint;
+            else
+              ref[d] = side == 0 ? is : ie;
+            double gratio = 1.0;
+            if (!periodic) {
+              orc_point_geom g, gr;
+              orc_geom_point(&p->geom, xc(b, 0, idx[0]), xc(b, 1, idx[1]), xc(b, 2, idx[2]), &g);
+              orc_geom_point(&p->geom, xc(b, 0, ref[0]), xc(b, 1, ref[1]), xc(b, 2, ref[2]), &gr);
+              gratio = orc_ratio(g.detgamma, gr.detgamma);
+            }
+            const long c = ((long)idx[2] * n[1] + idx[1]) * n[0] + idx[0];
+            const long cr = ((long)ref[2] * n[1] + ref[1]) * n[0] + ref[0];
+            for (int l = 0; l < nvar; ++l) q[l * nc + c] = gratio * q[l * nc + cr];
+          }
+    }
+  }
+}
+
 /* ---- batched con2prim on spherical Kerr-Schild points (r, theta, phi = 0) ------------- */
 void orc_c2p_batch(const orc_params *p, long n, const double *coords, double *cons,
                    double *prim, double *mu, double *fail, int *iters) {

@@ -170,6 +170,12 @@ void orc_source_terms_block(const orc_params *p, const orc_block *b, const doubl
 double orc_estimate_dt(const orc_params *p, const orc_block *b, const double *cell_sig,
                        const double *face_sig, double cfl);
 
+/* phoebus_boundaries.cpp:118-150 (outflow, conserved) and single-block periodic fills,
+ * applied face by face; kinds[2*d + side] */
+enum { ORC_BC_OUTFLOW = 0, ORC_BC_PERIODIC = 1 };
+void orc_apply_bcs(const orc_params *p, const orc_block *b, int nvar, double *q,
+                   const int kinds[6]);
+
 /* batched con2prim over independent states with spherical-KS geometry at (r, theta)
  * per state: cons[8][n] in, prim[nprim][n] out */
 void orc_c2p_batch(const orc_params *p, long n, const double *coords /* [2][n] */,

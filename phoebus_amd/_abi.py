@@ -1,4 +1,4 @@
-"""ctypes binding of libphx.so (include/phx.h, include/phx_runtime.h).
+"""ctypes binding of libphx.so (include/phx.h, phx_runtime.h, phx_mlp.h, phx_driver.h).
 
 The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading fails loudly
 when it has not been built; there is no CPU fallback.
@@ -14,6 +14,8 @@ PHX_OK, PHX_ERR_INVALID, PHX_ERR_UNSUPPORTED, PHX_ERR_DEVICE, PHX_ERR_NOMEM = 0,
 GEOM_MINKOWSKI, GEOM_SPHERICAL_KS = 0, 1
 RECON_CONSTANT, RECON_LINEAR, RECON_WENO5Z, RECON_MP5 = 0, 1, 2, 3
 RIEMANN_LLF, RIEMANN_HLL = 0, 1
+BC_OUTFLOW, BC_PERIODIC = 0, 1
+DRIVER_PRIM, DRIVER_CONS, DRIVER_FAIL, DRIVER_C2P_MU = 0, 1, 2, 3
 (FLOOR_CONSTANT_RHO_SIE, FLOOR_EXPX1_RHO_SIE, FLOOR_EXPX1_RHO_U, FLOOR_X1_RHO_SIE,
  FLOOR_R_RHO_SIE) = range(5)
 
@@ -101,6 +103,14 @@ def lib():
         "phx_mlp_train_step": (i32, [vp, vp, C.c_int64, vp, vp, C.c_float, C.c_float, vp, vp]),
         "phx_con2prim_batch_learned": (i32, [vp, P(FluidParams), vp, C.c_int64, vp, vp, vp, vp, vp,
                                              vp, vp, vp, vp]),
+        "phx_driver_create": (i32, [vp, P(FluidParams), P(Block), P(i32), dbl, dbl, P(vp)]),
+        "phx_driver_destroy": (i32, [vp]),
+        "phx_driver_set_prim": (i32, [vp, vp]),
+        "phx_driver_initialize": (i32, [vp]),
+        "phx_driver_step": (i32, [vp]),
+        "phx_driver_execute": (i32, [vp, dbl, i32]),
+        "phx_driver_time": (i32, [vp, P(dbl), P(dbl), P(C.c_int64)]),
+        "phx_driver_get": (i32, [vp, i32, vp]),
         "phx_rt_device_count": (i32, [P(i32)]),
         "phx_rt_set_device": (i32, [i32]),
         "phx_rt_malloc": (i32, [P(vp), C.c_size_t]),

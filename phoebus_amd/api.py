@@ -241,3 +241,53 @@ class Mlp:
         self.phx._chk(self.L.phx_con2prim_batch_learned(
             self.phx.h, C.byref(p), self.h, n, _p(coords), _p(cons), _p(prim), _p(mu), _p(fail),
             _p(iters), _p(evals), _p(fallback), stream), "phx_con2prim_batch_learned")
+
+
+class Driver:
+    """One-block RK2 driver (include/phx_driver.h): the fluid task sequence of a phoebus
+    cycle on device fields the driver owns."""
+
+    def __init__(self, phx, p, b, bc=(A.BC_OUTFLOW,) * 6, cfl=0.4, dt_init_fact=1.0):
+        self.phx, self.L, self.p, self.b = phx, phx.L, p, b
+        h = C.c_void_p()
+        phx._chk(self.L.phx_driver_create(phx.h, C.byref(p), C.byref(b), (C.c_int * 6)(*bc),
+                                          cfl, dt_init_fact, C.byref(h)), "phx_driver_create")
+        self.h = h.value
+        self.shape = block_shape(b)
+
+    def close(self):
+        if self.h:
+            self.L.phx_driver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_prim(self, prim):
+        prim = np.ascontiguousarray(prim, np.float64)
+        assert prim.shape == (NPRIM[bool(self.p.mhd)],) + self.shape
+        self.phx._chk(self.L.phx_driver_set_prim(self.h, prim.ctypes.data), "phx_driver_set_prim")
+
+    def initialize(self):
+        self.phx._chk(self.L.phx_driver_initialize(self.h), "phx_driver_initialize")
+
+    def step(self):
+        self.phx._chk(self.L.phx_driver_step(self.h), "phx_driver_step")
+
+    def execute(self, tlim, max_steps=1 << 30):
+        self.phx._chk(self.L.phx_driver_execute(self.h, tlim, max_steps), "phx_driver_execute")
+
+    def time(self):
+        t, dt, n = C.c_double(), C.c_double(), C.c_int64()
+        self.phx._chk(self.L.phx_driver_time(self.h, C.byref(t), C.byref(dt), C.byref(n)),
+                      "phx_driver_time")
+        return t.value, dt.value, n.value
+
+    def get(self, what):
+        nv = {A.DRIVER_PRIM: NPRIM[bool(self.p.mhd)], A.DRIVER_CONS: NCONS[bool(self.p.mhd)]}
+        out = np.zeros((nv.get(what, 1),) + self.shape)
+        self.phx._chk(self.L.phx_driver_get(self.h, what, out.ctypes.data), "phx_driver_get")
+        return out if what in nv else out[0]

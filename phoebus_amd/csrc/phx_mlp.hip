@@ -18,10 +18,18 @@
 // backward pass uses the hi parts only.
 //
 // Features (scale-free conserved invariants of the con2prim set-up): log q, log r^2,
-// log b^2, log (r.b)^2 with q = tau/D, standardized.  Outputs: o1 = logit of the bracket
-// centre, o2 = log of the bracket half-width.  The solver evaluates the residual at the
-// bracket ends; a sign change runs the reference's regula falsi inside it, anything
-// else falls back to the plain solve on [0, 1] with the centre as guess.
+// log b^2, log (r.b)^2 with q = tau/D, r = S/D, b = B/sqrt(D), standardized.
+//
+// Outputs, relative to a physical prior.  With mu = 1/(h W), the energy equation gives
+// (tau + D)/D = hW - P/(rho W) + b^2 (1 + v^2)/2 - (b.v)^2/2, so z0 = 1 + q - b^2/2 =
+// hW - P/(rho W) + (b^2 v^2 - (b.v)^2)/2 >= max(hW/Gamma_ad, 1): for b = 0, mu z0 lies in
+// [1/Gamma_ad, 1], and log(mu z0) is a slowly varying function of the features (the
+// max(., 1) only guards round-off).  o1 = log(mu z0) of
+// the bracket centre, c = exp(o1)/z0; o2 = log of the bracket half-width relative to c.
+// Learning the residual instead of logit(mu) leaves the network ~10x less to resolve.
+// The solver evaluates the residual at the bracket ends; a sign change runs the
+// reference's regula falsi inside it, anything else falls back to the plain solve on
+// [0, 1] with the centre as guess.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -169,6 +177,14 @@ __device__ __forceinline__ void zone_features(const MlpDev &m, const Residual &r
   for (int i = 0; i < 4; ++i) f[i] = ((float)raw[i] - m.fmean[i]) / m.fstd[i];
 }
 
+// the prior z0 = max(1 + q - b^2/2, 1) from standardized features (the same fp32 function
+// of the features in training and in the solve)
+__device__ __forceinline__ float prior_z0(const MlpDev &m, float f0, float f2) {
+  const float q = expf(f0 * m.fstd[0] + m.fmean[0]) - 1e-12f;
+  const float bsq = expf(f2 * m.fstd[2] + m.fmean[2]) - 1e-12f;
+  return fmaxf(1.0f + q - 0.5f * bsq, 1.0f);
+}
+
 // wave-cooperative prediction for the 64 zones of this wavefront (all lanes must call it):
 // returns o1, o2 of this lane's zone
 __device__ __forceinline__ void mlp_predict_wave(const MlpDev &m, const float f[4], float &o1,
@@ -251,7 +267,8 @@ __device__ __forceinline__ bf16x4 tile_transpose(uint16_t *lds, const f32x4 v) {
 
 // One wavefront per block; each block walks `chunk` samples in tiles of 16, accumulates
 // the loss gradient of every parameter and adds it to grad (fp32 atomics) once.
-// Loss per sample: (o1 - logit mu*)^2 + lam (o2 - log(|sigmoid(o1) - mu*| + 1e-12))^2,
+// Loss per sample, with t = log(mu* z0): (o1 - t)^2 + lam (o2 - log(|1 - exp(t - o1)| +
+// 1e-12))^2 (the second target, the log relative miss of the centre, held fixed),
 // averaged over the n samples of the step.
 __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
                                                   const float *__restrict__ feat,
@@ -284,10 +301,9 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
     // loss gradient (lanes 0..15 own the outputs of sample s)
     f32x4 dO = {0.f, 0.f, 0.f, 0.f};
     if (lane < 16 && have) {
-      const float mu = fminf(fmaxf(target[s], 1e-12f), 1.0f - 1e-7f);
-      const float t1 = logf(mu / (1.0f - mu));
-      const float c = 1.0f / (1.0f + expf(-o[0]));
-      const float t2 = logf(fabsf(c - mu) + 1e-12f);
+      const float mu = fminf(fmaxf(target[s], 1e-30f), 1.0f);
+      const float t1 = logf(mu * prior_z0(m, feat[s], feat[2 * n + s]));
+      const float t2 = logf(fabsf(1.0f - expf(fminf(fmaxf(t1 - o[0], -30.f), 30.f))) + 1e-12f);
       const float e1 = o[0] - t1, e2 = o[1] - t2;
       lsum += (e1 * e1 + lam * e2 * e2) * inv_n;
       dO[0] = 2.0f * e1 * inv_n;
@@ -468,8 +484,8 @@ __global__ __launch_bounds__(256) void k_con2prim_batch_learned(
   double st = 0.0, root = mu[s], tt = 0.0;
   int it = 0, ev = 0, fb = 0;
   if (ok) {
-    const double c = 1.0 / (1.0 + exp(-(double)o1));
-    const double w = fmax((double)m.width_scale * exp((double)o2), (double)m.width_min);
+    const double c = exp((double)o1) / (double)prior_z0(m, f[0], f[2]);
+    const double w = c * fmax((double)m.width_scale * exp((double)o2), (double)m.width_min);
     const double lo = fmax(c - w, 0.0), hi = fmin(c + w, 1.0);
     const double ylo = z.res(lo), yhi = z.res(hi);
     ev = 2;

@@ -46,6 +46,10 @@ class DeviceArray:
         assert other.nbytes == self.nbytes
         _chk(lib().phx_rt_memcpy_dtod(self.ptr, other.ptr, self.nbytes), "device copy")
 
+    def view(self, offset, shape):
+        """A non-owning view of `shape` elements starting `offset` elements in."""
+        return DeviceView(self, offset, shape)
+
     def free(self):
         if self.ptr:
             lib().phx_rt_free(self.ptr)
@@ -56,6 +60,22 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView(DeviceArray):
+    """Part of a DeviceArray (keeps its parent alive, frees nothing)."""
+
+    def __init__(self, parent, offset, shape):
+        self.parent = parent
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = parent.dtype
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        off = int(offset) * self.dtype.itemsize
+        assert 0 <= off and off + self.nbytes <= parent.nbytes, "view outside its array"
+        self.ptr = parent.ptr + off
+
+    def free(self):
+        self.ptr = None
 
 
 def synchronize():

@@ -69,17 +69,30 @@ def forward(w, x, bf=True):
     return o, (h0, h1, h2, h3)
 
 
-def loss_and_grad(w, x, mu, lam, bf=True):
-    """Loss (o1 - logit mu)^2 + lam (o2 - log(|sigmoid o1 - mu| + 1e-12))^2, mean over n;
-    the second target is held fixed (no gradient through it)."""
+def prior_z0(x, mean, std):
+    """z0 = max(1 + q - b^2/2, 1) from standardized features, in fp32 as the device forms it."""
+    x = np.asarray(x, np.float32)
+    mean, std = np.asarray(mean, np.float32), np.asarray(std, np.float32)
+    qv = np.exp(x[:, 0] * std[0] + mean[0]) - np.float32(1e-12)
+    bsq = np.exp(x[:, 2] * std[2] + mean[2]) - np.float32(1e-12)
+    return np.maximum(np.float32(1) + qv - np.float32(0.5) * bsq, np.float32(1)).astype(np.float64)
+
+
+def targets(o1, mu, z0):
+    """t = log(mu z0) and the fixed width target log(|1 - exp(t - o1)| + 1e-12)."""
+    t1 = np.log(np.clip(mu, 1e-30, 1.0) * z0)
+    t2 = np.log(np.abs(1 - np.exp(np.clip(t1 - o1, -30, 30))) + 1e-12)
+    return t1, t2
+
+
+def loss_and_grad(w, x, mu, lam, mean=(0, 0, 0, 0), std=(1, 1, 1, 1), bf=True):
+    """Loss (o1 - t)^2 + lam (o2 - log(|1 - exp(t - o1)| + 1e-12))^2 with t = log(mu z0),
+    mean over n; the second target is held fixed (no gradient through it)."""
     n = x.shape[0]
     p = unpack(w)
     q = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     o, (h0, h1, h2, h3) = forward(w, x, bf)
-    m = np.clip(mu, 1e-12, 1 - 1e-7)
-    t1 = np.log(m / (1 - m))
-    c = 1 / (1 + np.exp(-o[:, 0]))
-    t2 = np.log(np.abs(c - m) + 1e-12)
+    t1, t2 = targets(o[:, 0], mu, prior_z0(x, mean, std))
     e1, e2 = o[:, 0] - t1, o[:, 1] - t2
     L = np.sum(e1 * e1 + lam * e2 * e2) / n
     dO = np.stack([2 * e1 / n, 2 * lam * e2 / n], axis=1)

@@ -311,3 +311,108 @@ def geom_gradlnalpha(params, X):
 def threads():
     """OpenMP threads the oracle uses."""
     return lib().orc_num_threads()
+
+
+BC_OUTFLOW, BC_PERIODIC = 0, 1
+
+
+def apply_bcs(params, b, q, kinds):
+    """orc_apply_bcs in place on q[nvar][k][j][i]."""
+    assert q.flags.c_contiguous and q.dtype == np.float64
+    lib().orc_apply_bcs(C.byref(params), C.byref(b), q.shape[0], ptr(q), (C.c_int * 6)(*kinds))
+
+
+class Driver:
+    """The one-block RK2 task sequence of a phoebus cycle on the oracle's block functions
+    (phoebus_driver.cpp:147-202 for the start, :204-738 per stage; rk2 containers "base"
+    and "1" with beta = 1, 1/2): the CPU counterpart of include/phx_driver.h, step for
+    step and container for container, so that results compare bit for bit in Minkowski."""
+
+    BETA = (1.0, 0.5)
+
+    def __init__(self, params, b, kinds, cfl, dt_init_fact=1.0):
+        self.p, self.b, self.kinds, self.cfl = params, b, tuple(kinds), cfl
+        self.dt_init_fact = dt_init_fact
+        sh = block_shape(b)
+        nc, npr = ncons(params.mhd), nprim(params.mhd)
+        self.cons = [np.zeros((nc,) + sh), np.zeros((nc,) + sh)]
+        self.prim = np.zeros((npr,) + sh)
+        self.temp = np.zeros(sh)
+        self.mu = np.zeros(sh)
+        self.fail = np.zeros(sh)
+        self.cell_sig = np.zeros((3,) + sh)
+        self.face_sig = None
+        self.flux = np.zeros((3, nc) + sh)
+        self.iters = np.zeros(sh, np.int32)
+        self.time, self.dt, self.ncycle = 0.0, 0.0, 0
+        self.has_face_speeds = False
+        self.tlim = 1e300
+        self.n = [sh[2], sh[1], sh[0]]
+
+    def _c2p(self, container):
+        lo = (0, 0, 0)
+        hi = (self.n[0] - 1, self.n[1] - 1, self.n[2] - 1)
+        lib().orc_con2prim_block(C.byref(self.p), C.byref(self.b), (C.c_int * 3)(*lo),
+                                 (C.c_int * 3)(*hi), ptr(self.cons[container]), ptr(self.prim),
+                                 ptr(self.temp), ptr(self.mu), ptr(self.cell_sig),
+                                 ptr(self.fail), ptr(self.iters))
+
+    def _dt(self):
+        fs = self.face_sig if self.has_face_speeds else None
+        d = lib().orc_estimate_dt(C.byref(self.p), C.byref(self.b), ptr(self.cell_sig),
+                                  ptr(fs) if fs is not None else None, C.c_double(self.cfl))
+        self.has_face_speeds = True
+        return d
+
+    def initialize(self, prim0):
+        self.prim[...] = prim0
+        lo = (0, 0, 0)
+        hi = (self.n[0] - 1, self.n[1] - 1, self.n[2] - 1)
+        sig = np.zeros_like(self.cell_sig)
+        lib().orc_prim_to_con_block(C.byref(self.p), C.byref(self.b), (C.c_int * 3)(*lo),
+                                    (C.c_int * 3)(*hi), ptr(self.prim), ptr(self.cons[0]),
+                                    ptr(sig))
+        apply_bcs(self.p, self.b, self.cons[0], self.kinds)
+        self._c2p(0)
+        self.cons[1][...] = self.cons[0]
+        self.dt = self.dt_init_fact * self._dt()
+
+    def stage(self, stage):
+        sc0, sc1 = (0, 1) if stage == 1 else (1, 0)
+        nc = ncons(self.p.mhd)
+        sh = block_shape(self.b)
+        src = None
+        if self.p.geom.kind != GEOM_MINKOWSKI:
+            src = source_terms_block(self.p, self.b, self.prim)
+        flux = np.zeros((self.b.ndim, nc) + sh)
+        fsig = np.zeros((self.b.ndim,) + sh)
+        lib().orc_calculate_fluxes(C.byref(self.p), C.byref(self.b), ptr(self.prim), ptr(flux),
+                                   ptr(fsig))
+        if self.p.mhd and self.b.ndim > 1:
+            emf = np.zeros((3,) + sh)
+            lib().orc_flux_ct(C.byref(self.p), C.byref(self.b), ptr(flux), ptr(emf))
+        self.face_sig = np.zeros((3,) + sh)
+        self.face_sig[:self.b.ndim] = fsig
+        lib().orc_flux_divergence_update(
+            C.byref(self.p), C.byref(self.b), ptr(flux), ptr(src) if src is not None else None,
+            ptr(self.cons[sc0]), ptr(self.cons[0]), C.c_double(self.BETA[stage - 1]),
+            C.c_double(self.dt), ptr(self.cons[sc1]))
+        self._c2p(sc1)
+        apply_bcs(self.p, self.b, self.cons[sc1], self.kinds)
+        self._c2p(sc1)
+
+    def step(self):
+        if self.time + self.dt > self.tlim:
+            self.dt = self.tlim - self.time
+        for s in (1, 2):
+            self.stage(s)
+        self.time += self.dt
+        self.ncycle += 1
+        self.dt = min(self._dt(), 2.0 * self.dt)
+
+    def execute(self, tlim, max_steps=1 << 30):
+        self.tlim = tlim
+        n = 0
+        while n < max_steps and self.time < self.tlim:
+            self.step()
+            n += 1

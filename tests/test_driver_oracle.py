@@ -1,0 +1,142 @@
+"""CPU checks of the one-block RK2 sequence on the oracle: the boundary fills, and the
+1-D shock tube (512 zones, Gamma = 5/3, WENO5 + HLL, cfl 0.4, t = 0.4) against the exact
+special-relativistic Riemann solution."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import _sr_riemann as S
+from phoebus_amd.workloads import shock_tube_prims
+
+
+def test_exact_riemann_known_answer():
+    # Marti & Mueller, Living Rev. Relativity 6 (2003) 7, problem 1: p* = 1.448,
+    # v* = 0.714, rho*_L = 2.639, rho*_R = 5.071, shock speed 0.828
+    _, _, _, st = S.solution(5 / 3, (10.0, 13.33, 0.0), (1.0, 1e-6, 0.0), 0.5, 0.4,
+                             np.array([0.5]))
+    assert abs(st["p_star"] - 1.448) < 1e-3
+    assert abs(st["v_star"] - 0.714) < 1e-3
+    assert abs(st["rho_star_l"] - 2.639) < 1e-3
+    assert abs(st["rho_star_r"] - 5.071) < 1e-3
+    assert abs(st["shock"] - 0.828) < 1e-3
+
+
+def test_exact_riemann_jump_conditions():
+    # across the right shock: D = rho W, and T^{01}, T^{11} fluxes match in the shock frame
+    g = 5 / 3
+    _, _, _, st = S.solution(g, (1.0, 1.0, 0.0), (0.125, 0.1, 0.0), 0.5, 0.4, np.array([0.5]))
+    Vs = st["shock"]
+
+    def cons_flux(rho, p, v):
+        h = 1 + g / (g - 1) * p / rho
+        W = 1 / np.sqrt(1 - v * v)
+        U = np.array([rho * W, rho * h * W * W * v, rho * h * W * W - p])
+        F = np.array([rho * W * v, rho * h * W * W * v * v + p, rho * h * W * W * v])
+        return U, F
+
+    Ua, Fa = cons_flux(0.125, 0.1, 0.0)
+    Ub, Fb = cons_flux(st["rho_star_r"], st["p_star"], st["v_star"])
+    np.testing.assert_allclose(Fb - Fa, Vs * (Ub - Ua), rtol=1e-9, atol=1e-12)
+
+
+def _block(nx):
+    return O.make_block(1, (nx,), ng=4, xmin=(0.0, 0.0, 0.0), dx=(1.0 / nx, 1.0, 1.0))
+
+
+def test_outflow_and_periodic_fills():
+    p = O.default_params(mhd=False)
+    b = O.make_block(2, (6, 5), ng=4, dx=(0.1, 0.2, 1.0))
+    sh = O.block_shape(b)
+    rng = np.random.default_rng(4)
+    q0 = rng.normal(size=(5,) + sh)
+    q = q0.copy()
+    O.apply_bcs(p, b, q, (O.BC_OUTFLOW, O.BC_OUTFLOW, O.BC_PERIODIC, O.BC_PERIODIC, 0, 0))
+    ii, jj = slice(4, 10), slice(4, 9)
+    # x1: ghosts equal the first/last interior column (on all rows, ghost rows included
+    # before the x2 fill)
+    np.testing.assert_array_equal(q[:, :, jj, 0:4], np.repeat(q0[:, :, jj, 4:5], 4, axis=3))
+    np.testing.assert_array_equal(q[:, :, jj, 10:14], np.repeat(q0[:, :, jj, 9:10], 4, axis=3))
+    # x2 periodic: ghost rows copy the opposite interior rows, corners included
+    np.testing.assert_array_equal(q[:, :, 0:4, :], q[:, :, 5:9, :])
+    np.testing.assert_array_equal(q[:, :, 9:13, :], q[:, :, 4:8, :])
+    np.testing.assert_array_equal(q[:, :, jj, ii], q0[:, :, jj, ii])
+
+
+def test_outflow_rescales_by_sqrt_gamma_in_kerr_schild():
+    p = O.default_params(mhd=False, geom=O.GEOM_SPHKS, a=0.9)
+    b = O.make_block(1, (8,), ng=4, xmin=(3.0, 1.2, 0.0), dx=(0.25, 1.0, 1.0))
+    sh = O.block_shape(b)
+    q = np.ones((5,) + sh)
+    O.apply_bcs(p, b, q, (O.BC_OUTFLOW,) * 6)
+    # closed form sqrt(gamma) = rho^2 sin(theta) sqrt(1 + 2r/rho^2) for spherical KS
+    a, th = 0.9, 1.2 + 0.5
+    r = 3.0 + (np.arange(16) - 4 + 0.5) * 0.25
+    rho2 = r * r + a * a * np.cos(th) ** 2
+    sg = np.sqrt(rho2 * (rho2 + 2 * r)) * np.sin(th)
+    expect = np.where(np.arange(16) < 4, sg / sg[4], np.where(np.arange(16) > 11, sg / sg[11], 1))
+    np.testing.assert_allclose(q[0, 0, 0], expect, rtol=1e-13)
+
+
+@pytest.fixture(scope="module")
+def shock_tube_run():
+    nx = 512
+    p = O.default_params(gamma=5 / 3, mhd=False, recon=O.RECON_WENO5, riemann=O.RIEMANN_HLL,
+                         tol=1e-8, max_iter=100)
+    b = _block(nx)
+    d = O.Driver(p, b, (O.BC_OUTFLOW,) * 6, cfl=0.4)
+    prim0 = shock_tube_prims(nx)
+    d.initialize(prim0)
+    D0 = d.cons[0][0, 0, 0, 4:4 + nx].sum()
+    d.execute(0.4)
+    return d, D0, nx
+
+
+def test_shock_tube_reaches_tlim(shock_tube_run):
+    d, _, nx = shock_tube_run
+    assert d.time == pytest.approx(0.4, abs=0.0)
+    assert d.ncycle > 0.4 * nx / 0.4 * 0.5  # dt <= cfl dx / max speed
+    assert np.all(d.fail[0, 0, 4:4 + nx] == 1.0)
+
+
+def test_shock_tube_conserves_mass(shock_tube_run):
+    # no wave reaches the outflow boundaries by t = 0.4, so the interior sum of D is
+    # conserved to round-off by the telescoping flux divergence
+    d, D0, nx = shock_tube_run
+    D = d.cons[0][0, 0, 0, 4:4 + nx].sum()
+    assert abs(D - D0) < 1e-13 * D0
+
+
+def test_shock_tube_matches_exact_solution(shock_tube_run):
+    d, _, nx = shock_tube_run
+    x = (np.arange(nx) + 0.5) / nx
+    rho_ex, p_ex, v_ex, st = S.solution(5 / 3, (1.0, 1.0, 0.0), (0.125, 0.1, 0.0), 0.5, 0.4, x)
+    prim = d.prim[:, 0, 0, 4:4 + nx]
+    rho, W_v, P = prim[0], prim[1], prim[5]
+    v = W_v / np.sqrt(1 + W_v * W_v)
+    l1_rho = np.mean(np.abs(rho - rho_ex))
+    l1_v = np.mean(np.abs(v - v_ex))
+    assert l1_rho < 2e-3, l1_rho  # 1.18e-3 measured
+    assert l1_v < 2e-3, l1_v  # 0.93e-3 measured
+    # plateau between the rarefaction tail and the contact: p* and v* to 0.1 %
+    xi = (x - 0.5) / 0.4
+    plateau = (xi > 0.1) & (xi < 0.35)
+    assert np.max(np.abs(P[plateau] / st["p_star"] - 1)) < 1e-3
+    assert np.max(np.abs(v[plateau] / st["v_star"] - 1)) < 1e-3
+    # the shock sits within two zones of the exact position
+    jump = np.argmax(np.abs(np.diff(rho[x > 0.75]))) + np.sum(x <= 0.75)
+    assert abs(x[jump] - (0.5 + 0.4 * st["shock"])) < 2.0 / nx
+
+
+def test_shock_tube_converges_at_first_order():
+    # L1 errors of a solution with discontinuities fall ~ 1/N
+    err = []
+    for nx in (128, 256):
+        p = O.default_params(gamma=5 / 3, mhd=False, recon=O.RECON_WENO5,
+                             riemann=O.RIEMANN_HLL, tol=1e-8, max_iter=100)
+        d = O.Driver(p, _block(nx), (O.BC_OUTFLOW,) * 6, cfl=0.4)
+        d.initialize(shock_tube_prims(nx))
+        d.execute(0.4)
+        x = (np.arange(nx) + 0.5) / nx
+        rho_ex = S.solution(5 / 3, (1.0, 1.0, 0.0), (0.125, 0.1, 0.0), 0.5, 0.4, x)[0]
+        err.append(np.mean(np.abs(d.prim[0, 0, 0, 4:4 + nx] - rho_ex)))
+    assert err[1] < 0.65 * err[0], err
