@@ -113,6 +113,116 @@ def cpu_baseline(prim, cons, mu0, args):
                       f"(C99 restatement, OpenMP, median stage {t:.3f} s)"}
 
 
+def c3_learned_vs_plain(phx, args, log):
+    """Batched con2prim on spherical Kerr-Schild (a = 0.9375, Gamma = 4/3, tol 1e-10): the
+    plain solver and the learned-bracket solver on the same 2^26 states (seed SEED, in
+    chunks), after training the network on the device from 2^22 plain-solver pairs of a
+    disjoint seed.  Reports mean iterations, histograms, fallback rate, the largest
+    difference of the two roots, states/s of each, and the CPU oracle's root on a sample."""
+    from phoebus_amd import _abi as A
+    from phoebus_amd.api import fluid_params
+    from phoebus_amd.device import DeviceArray, Event, synchronize
+    from phoebus_amd.learned import train_bracket
+    from phoebus_amd.workloads import batched_con2prim_states
+
+    p = fluid_params(gamma=4.0 / 3.0, mhd=True, geom=A.GEOM_SPHERICAL_KS, a=0.9375,
+                     c2p_tol=1e-10, c2p_max_iter=100)
+    t0 = time.perf_counter()
+    net, losses = train_bracket(phx, p, np.random.default_rng(SEED + 1), args.c3_pairs,
+                                args.c3_train_steps)
+    t_train = time.perf_counter() - t0
+    log(f"c3: trained {args.c3_train_steps} steps on {args.c3_pairs} pairs in {t_train:.1f} s")
+    rng = np.random.default_rng(SEED)
+    chunk = min(args.c3_states, 1 << 22)
+    n_total = (args.c3_states // chunk) * chunk
+    hp = np.zeros(128, np.int64)
+    hl = np.zeros(128, np.int64)
+    fb_count = ok_count = 0
+    ms_plain = ms_learned = 0.0
+    max_dmu = 0.0
+    fail_mismatch = 0
+    sample = None
+    d_coords = DeviceArray((2, chunk))
+    d_prim = DeviceArray((10, chunk))
+    d_cons = DeviceArray((8, chunk))
+    d_cons2 = DeviceArray((8, chunk))
+    d_out = DeviceArray((10, chunk))
+    d_mu = DeviceArray((chunk,))
+    d_mu2 = DeviceArray((chunk,))
+    d_fail = DeviceArray((chunk,))
+    d_fail2 = DeviceArray((chunk,))
+    d_it = DeviceArray((chunk,), np.int32)
+    d_it2 = DeviceArray((chunk,), np.int32)
+    d_fb = DeviceArray((chunk,), np.int32)
+    e0, e1, e2 = Event(), Event(), Event()
+    for c0 in range(0, n_total, chunk):
+        coords, prim = batched_con2prim_states(rng, chunk)
+        d_coords.upload(coords)
+        d_prim.upload(prim)
+        phx.prim_to_con_batch(p, chunk, d_coords, d_prim, d_cons)
+        d_cons2.fill_from(d_cons)
+        if sample is None:
+            sample = (coords[:, :2048].copy(), d_cons.get()[:, :2048].copy())
+        d_mu.zero()
+        d_mu2.zero()
+        synchronize()
+        e0.record()
+        phx.con2prim_batch(p, chunk, d_coords, d_cons, d_out, d_mu, d_fail, d_it)
+        e1.record()
+        net.con2prim_batch(p, chunk, d_coords, d_cons2, d_out, d_mu2, d_fail2, d_it2, None, d_fb)
+        e2.record()
+        synchronize()
+        ms_plain += e0.elapsed_ms(e1)
+        ms_learned += e1.elapsed_ms(e2)
+        fail, fail2 = d_fail.get(), d_fail2.get()
+        ok = fail == 1.0
+        fail_mismatch += int(np.sum(fail != fail2))
+        ip, il = d_it.get()[ok], d_it2.get()[ok]
+        hp += np.bincount(np.clip(ip, 0, 127), minlength=128)
+        hl += np.bincount(np.clip(il, 0, 127), minlength=128)
+        fb_count += int(np.sum(d_fb.get()[ok]))
+        ok_count += int(np.sum(ok))
+        max_dmu = max(max_dmu, float(np.max(np.abs(d_mu.get()[ok] - d_mu2.get()[ok]))))
+    if sample is not None:
+        mean_p = float(np.dot(np.arange(128), hp) / max(ok_count, 1))
+    mean_l = float(np.dot(np.arange(128), hl) / max(ok_count, 1))
+    out = {"workload": f"batched con2prim, {n_total} states, spherical KS a=0.9375, "
+                       "Gamma=4/3, tol 1e-10, cold start",
+           "mean_iters_plain": mean_p, "mean_iters_learned": mean_l,
+           "iters_ratio": mean_p / mean_l if mean_l > 0 else None,
+           "fallback_rate": fb_count / max(ok_count, 1),
+           "max_abs_dmu_learned_vs_plain": max_dmu,
+           "fail_flag_mismatches": fail_mismatch,
+           "states_per_s_plain": n_total / (ms_plain * 1e-3),
+           "states_per_s_learned": n_total / (ms_learned * 1e-3),
+           "hist_plain": np.trim_zeros(hp, "b").tolist(),
+           "hist_learned": np.trim_zeros(hl, "b").tolist(),
+           "train": {"pairs": args.c3_pairs, "steps": args.c3_train_steps,
+                     "seconds": t_train, "loss_first_last": [losses[0], losses[-1]]}}
+    if not args.no_cpu and sample is not None:
+        # FP64 oracle on a sample of the first chunk: its root against the learned one
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        op = O.default_params(gamma=4.0 / 3.0, mhd=True, geom=O.GEOM_SPHKS, a=0.9375,
+                              tol=1e-10, max_iter=100)
+        coords_s, cons_s = sample
+        ref = O.c2p_batch(op, np.ascontiguousarray(coords_s), np.ascontiguousarray(cons_s))
+        d_c = DeviceArray.from_host(coords_s)
+        d_u = DeviceArray.from_host(cons_s)
+        m = coords_s.shape[1]
+        d_o, d_m, d_f = DeviceArray((10, m)), DeviceArray.from_host(np.zeros(m)), DeviceArray((m,))
+        net.con2prim_batch(p, m, d_c, d_u, d_o, d_m, d_f)
+        synchronize()
+        okm = ref["fail"] == 1.0
+        out["oracle_sample"] = {"states": m,
+                                "max_abs_dmu_vs_oracle": float(np.max(np.abs(
+                                    d_m.get()[okm] - ref["mu"][okm]))),
+                                "fail_flags_equal": bool(np.array_equal(d_f.get(),
+                                                                        ref["fail"]))}
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +232,10 @@ def main():
     ap.add_argument("--cpu-nx", type=int, default=NX)
     ap.add_argument("--cpu-stages", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the batched con2prim leg")
+    ap.add_argument("--c3-states", type=int, default=1 << 26)
+    ap.add_argument("--c3-pairs", type=int, default=1 << 22)
+    ap.add_argument("--c3-train-steps", type=int, default=6000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
@@ -211,6 +325,7 @@ def main():
     dom = max(launch_ms, key=lambda k: launch_ms[k])
     achieved = kb[dom] / (launch_ms[dom] * 1e-3) / 1e9
     warm_iters = iters.get()[I]
+    cons_h, mu0_h = st.cons.get(), mu0.get()
     value = nx ** 3 / (ms_per_step * 1e-3)
     out = {
         "metric": "zone-updates/s of GRMHD RK stage; con2prim mean iters, MLP-guess vs plain",
@@ -244,8 +359,12 @@ def main():
                      "fail_count": int(np.sum(st.fail.get()[I] == 0.0)),
                      "mlp_guess": None},
     }
+    if not args.no_c3:
+        del st, u_out, prim_out, iters
+        out["con2prim"]["mlp_guess"] = c3_learned_vs_plain(
+            phx, args, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prim_h, st.cons.get(), mu0.get(), args)
+        out["cpu_baseline"] = cpu_baseline(prim_h, cons_h, mu0_h, args)
     print(json.dumps(out), flush=True)
     phx.close()
 

@@ -1,0 +1,75 @@
+"""Training of the learned con2prim bracket on the device (include/phx_mlp.h).
+
+Pairs come from the plain solver itself: random primitive states (the batched con2prim
+workload's distribution) -> prim_to_con -> plain con2prim -> (features, root mu).  The
+network trains with the library's own MFMA forward/backward and Adam kernels on shuffled
+mini-batches that stay resident in device memory; the learning rate follows a cosine
+schedule.
+"""
+import numpy as np
+
+from .device import DeviceArray, synchronize
+from .workloads import batched_con2prim_states
+
+
+def solver_pairs(phx, p, rng, n):
+    """n states of the batched workload and the plain solver's results on them."""
+    coords, prim = batched_con2prim_states(rng, n)
+    d_coords = DeviceArray.from_host(coords)
+    d_prim = DeviceArray.from_host(prim)
+    d_cons = DeviceArray((8, n))
+    phx.prim_to_con_batch(p, n, d_coords, d_prim, d_cons)
+    del d_prim
+    cons0 = d_cons.get()
+    d_out = DeviceArray((10, n))
+    d_mu = DeviceArray.from_host(np.zeros(n))
+    d_fail = DeviceArray((n,))
+    d_it = DeviceArray((n,), np.int32)
+    phx.con2prim_batch(p, n, d_coords, d_cons, d_out, d_mu, d_fail, d_it)
+    synchronize()
+    return dict(coords=coords, cons=cons0, prim=d_out.get(), mu=d_mu.get(),
+                fail=d_fail.get(), iters=d_it.get(), d_coords=d_coords)
+
+
+def features(net, p, data, n):
+    d_cons = DeviceArray.from_host(data["cons"])
+    d_f = DeviceArray((4, n), np.float32)
+    net.features(p, n, data["d_coords"], d_cons, d_f)
+    synchronize()
+    return d_f.get()
+
+
+def train_bracket(phx, p, rng, n_pairs=1 << 22, steps=6000, batch=16384, lr=2e-3, lam=0.05,
+                  width_scale=4.0, width_min=1e-7, seed=3, log=None):
+    """A trained Mlp (phoebus_amd.api.Mlp) and the loss history."""
+    from .api import Mlp
+
+    data = solver_pairs(phx, p, rng, n_pairs)
+    keep = data["fail"] == 1.0
+    net = Mlp(phx, seed=seed)
+    net.set_normalization([0, 0, 0, 0], [1, 1, 1, 1])
+    raw = features(net, p, data, n_pairs)[:, keep]
+    mean, std = raw.mean(axis=1), raw.std(axis=1) + 1e-6
+    net.set_normalization(mean, std, width_scale, width_min)
+    f = features(net, p, data, n_pairs)[:, keep]
+    mu = data["mu"][keep].astype(np.float32)
+    nb = max(1, f.shape[1] // batch)
+    perm = rng.permutation(f.shape[1])[:nb * batch]
+    fb = np.ascontiguousarray(f[:, perm].reshape(4, nb, batch).transpose(1, 0, 2))
+    d_f = DeviceArray.from_host(fb)
+    d_mu = DeviceArray.from_host(mu[perm])
+    d_loss = DeviceArray((1,), np.float32)
+    losses = []
+    for s in range(steps):
+        lr_s = lr * 0.5 * (1.0 + np.cos(np.pi * s / steps)) + 1e-5
+        b = s % nb
+        net.train_step(batch, d_f.view(b * 4 * batch, (4, batch)),
+                       d_mu.view(b * batch, (batch,)), lr_s, lam, d_loss)
+        if s % 500 == 0 or s == steps - 1:
+            synchronize()
+            losses.append(float(d_loss.get()[0]))
+            if log:
+                log(f"step {s} loss {losses[-1]:.5f}")
+    synchronize()
+    net.norm = (mean, std)
+    return net, losses

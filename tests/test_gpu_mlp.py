@@ -11,7 +11,7 @@ import _oracle as O
 from phoebus_amd import _abi as A
 from phoebus_amd.api import Mlp, Phx, fluid_params
 from phoebus_amd.device import DeviceArray, synchronize
-from phoebus_amd.workloads import batched_con2prim_states
+from phoebus_amd.learned import solver_pairs, train_bracket
 
 pytestmark = pytest.mark.gpu
 
@@ -78,63 +78,16 @@ def test_mfma_gradient_matches_reference(phx):
     net.close()
 
 
-def _solver_pairs(phx, p, rng, n):
-    coords, prim = batched_con2prim_states(rng, n)
-    d_coords = DeviceArray.from_host(coords)
-    d_prim = DeviceArray.from_host(prim)
-    d_cons = DeviceArray((8, n))
-    phx.prim_to_con_batch(p, n, d_coords, d_prim, d_cons)
-    d_out = DeviceArray((10, n))
-    d_mu = DeviceArray.from_host(np.zeros(n))
-    d_fail = DeviceArray((n,))
-    d_it = DeviceArray((n,), np.int32)
-    cons0 = d_cons.get()
-    phx.con2prim_batch(p, n, d_coords, d_cons, d_out, d_mu, d_fail, d_it)
-    synchronize()
-    return dict(coords=coords, cons=cons0, prim=d_out.get(), mu=d_mu.get(), fail=d_fail.get(),
-                iters=d_it.get(), d_coords=d_coords)
-
-
-def train_network(phx, p, rng, n_train=1 << 16, steps=400, batch=8192, lr=3e-3, lam=0.2):
-    """Plain-solver pairs -> features -> Adam on the device kernels."""
-    data = _solver_pairs(phx, p, rng, n_train)
-    keep = data["fail"] == 1.0
-    net = Mlp(phx, seed=3)
-    net.set_normalization([0, 0, 0, 0], [1, 1, 1, 1])
-    d_cons = DeviceArray.from_host(data["cons"])
-    d_feat = DeviceArray((4, n_train), np.float32)
-    net.features(p, n_train, data["d_coords"], d_cons, d_feat)
-    synchronize()
-    raw = d_feat.get()[:, keep]
-    mean, std = raw.mean(axis=1), raw.std(axis=1) + 1e-6
-    net.set_normalization(mean, std)
-    net.features(p, n_train, data["d_coords"], d_cons, d_feat)
-    synchronize()
-    feat = d_feat.get()[:, keep]
-    mu = data["mu"][keep].astype(np.float32)
-    m = feat.shape[1]
-    d_loss = DeviceArray((1,), np.float32)
-    losses = []
-    for step in range(steps):
-        idx = rng.integers(0, m, batch)
-        fb = DeviceArray.from_host(np.ascontiguousarray(feat[:, idx]))
-        tb = DeviceArray.from_host(mu[idx])
-        net.train_step(batch, fb, tb, lr, lam, d_loss)
-        if step % 50 == 0 or step == steps - 1:
-            synchronize()
-            losses.append(float(d_loss.get()[0]))
-    return net, losses
-
-
 def test_training_cuts_iterations_and_keeps_results(phx):
     say("[gpu] mlp training on solver pairs + learned con2prim")
     p = fluid_params(gamma=4 / 3, mhd=True, geom=A.GEOM_SPHERICAL_KS, a=0.9375, c2p_tol=1e-10,
                      c2p_max_iter=100)
-    net, losses = train_network(phx, p, np.random.default_rng(777))
-    assert losses[-1] < 0.5 * losses[0], losses
+    net, losses = train_bracket(phx, p, np.random.default_rng(777), n_pairs=1 << 20,
+                                steps=4000)
+    assert losses[-1] < 0.2 * losses[0], losses
     # disjoint evaluation set
     n = 1 << 15
-    ev = _solver_pairs(phx, p, np.random.default_rng(20261202), n)
+    ev = solver_pairs(phx, p, np.random.default_rng(20261202), n)
     d_cons = DeviceArray.from_host(ev["cons"])
     d_out = DeviceArray((10, n))
     d_mu = DeviceArray.from_host(np.zeros(n))
