@@ -233,9 +233,10 @@ def main():
     ap.add_argument("--cpu-stages", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched con2prim leg")
+    ap.add_argument("--no-tune", action="store_true", help="default flux tile shapes")
     ap.add_argument("--c3-states", type=int, default=1 << 26)
     ap.add_argument("--c3-pairs", type=int, default=1 << 22)
-    ap.add_argument("--c3-train-steps", type=int, default=6000)
+    ap.add_argument("--c3-train-steps", type=int, default=30000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
@@ -281,6 +282,13 @@ def main():
     sp = C.c_void_p()
     phx.L.phx_rt_stream_create(C.byref(sp))
     s = sp.value
+
+    # pick the y/z flux tile shapes for this block and reconstruction (times every
+    # admissible shape on these fields; results do not depend on it)
+    if not args.no_tune:
+        phx.autotune(p, b, f_stage, reps=3, stream=s)
+        phx.L.phx_rt_stream_synchronize(s)
+    tiles = {d: list(phx.tile_shape(p, b, d)) for d in (1, 2)}
 
     def stage():
         # the warm-start guess of each repeated stage is the input state's own root
@@ -343,7 +351,8 @@ def main():
         "config": {"workload": f"single 3-D GRMHD meshblock {nx}^3 + {NG} ghosts, Minkowski, "
                                "Gamma=13/9, WENO5-Z + HLL + FluxCT + divergence/RK update + "
                                "robust con2prim (tol 1e-10) + CFL, one RK stage",
-                   "zones": nx ** 3, "recon": "weno5z", "riemann": "hll"},
+                   "zones": nx ** 3, "recon": "weno5z", "riemann": "hll",
+                   "flux_tiles_yz": tiles},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": kb[dom],

@@ -58,10 +58,23 @@ enum {
 
 /* Analytic metric selection (reference: compile-time Geometry::CoordSysMeshBlock,
  * cmake/Geometry.cmake:17-41).  Minkowski uses Cartesian X; spherical Kerr-Schild uses
- * X = (r, theta, phi) and spin a (spherical_kerr_schild.hpp:42-243). */
+ * X = (r, theta, phi) and spin a (spherical_kerr_schild.hpp:42-243); FMKS uses
+ * X = (log r, x2 in [0, 1], phi): spherical Kerr-Schild under the McKinney-Gammie-Ryan
+ * transformation (mckinney_gammie_ryan.hpp:45-209, modified_system.hpp:52-248) with the
+ * parameters of fmks.cpp:37-52 (coordinates/derefine_poles, hslope, poly_xt,
+ * poly_alpha, smooth, hexp_*; x0 = min(x1min, max(0, x1min - nghost dx1))).  FMKS metric
+ * derivatives are central finite differences as geometry_utils.hpp:94-187. */
+enum { PHX_GEOM_FMKS = 2 };
+typedef struct {
+  int derefine_poles;
+  double h, xt, alpha, x0, smooth;
+  double hexp_br, hexp_nsq, hexp_csq;
+} phx_mgr;
+
 typedef struct {
   int kind;
   double a;
+  phx_mgr mgr; /* FMKS only */
 } phx_geometry;
 
 /* fixup::Bounds (fixup.hpp:72-327): floors of floor_type with (rho0, sie0, rho_exp,
@@ -210,6 +223,20 @@ int phx_con2prim_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
 int phx_prim_to_con_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
                           const double *coords, const double *prim, double *cons,
                           phx_stream stream);
+
+/* ---- tile-shape autotuning (no reference counterpart: Kokkos picks team sizes) -----
+ * phx_autotune times every admissible y/z flux tile shape (ti cells along i by ts along
+ * the sweep) on the caller's fields -- overwriting f->flux and f->face_sig -- and keeps
+ * the fastest in the context for this (recon, riemann, mhd, geometry, block size);
+ * phx_calculate_fluxes and phx_fluid_stage then use it.  Results do not depend on the
+ * shape.  reps = timed launches per candidate.  phx_tile_shape reports the shape in use
+ * for dir 1 (y) or 2 (z); phx_set_tile_shape fixes one (0 = default). */
+int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                 const phx_block_fields *f, int reps, phx_stream stream);
+int phx_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, int dir,
+                   int *ti, int *ts);
+int phx_set_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, int dir,
+                       int ti, int ts);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,8 @@
  */
 #include "phx_oracle.h"
 
+#define ORC_PI 3.14159265358979323846 /* M_PI */
+
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -155,23 +157,9 @@ void orc_recon(int type, const double *q, double *ql, double *qr) {
 /* ---- geometry ------------------------------------------------------------------------ */
 static void zero4(double g[4][4]) { memset(g, 0, 16 * sizeof(double)); }
 
-void orc_geom_point(const orc_geom *g, double X1, double X2, double X3, orc_point_geom *o) {
-  (void)X3;
-  memset(o, 0, sizeof(*o));
-  if (g->kind == ORC_GEOM_MINKOWSKI) { /* minkowski.hpp:35-115 */
-    o->alpha = 1.0;
-    o->gcov[0][0] = -1.0;
-    for (int d = 0; d < 3; ++d) {
-      o->gcov[d + 1][d + 1] = 1.0;
-      o->gcon[d][d] = 1.0;
-      o->gam[d][d] = 1.0;
-    }
-    o->detgamma = 1.0;
-    o->detg = 1.0;
-    return;
-  }
-  /* spherical_kerr_schild.hpp:42-243 */
-  const double a = g->a, a2 = a * a;
+/* spherical Kerr-Schild at (r = |X1|, theta = X2): spherical_kerr_schild.hpp:42-243 */
+static void ks_point(double a, double X1, double X2, orc_point_geom *o) {
+  const double a2 = a * a;
   const double r = fabs(X1), th = X2, cth = cos(th), sth = sin(th);
   const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
   o->alpha = sqrt(orc_ratio(rho2, 2 * r + rho2));
@@ -197,15 +185,8 @@ void orc_geom_point(const orc_geom *g, double X1, double X2, double X3, orc_poin
   o->detg = sqrt(fabs(rho2 * rho2 * sth2));
 }
 
-void orc_geom_gcon4(const orc_geom *g, double X1, double X2, double X3, double gi[4][4]) {
-  (void)X3;
-  zero4(gi);
-  if (g->kind == ORC_GEOM_MINKOWSKI) {
-    gi[0][0] = -1;
-    gi[1][1] = gi[2][2] = gi[3][3] = 1;
-    return;
-  }
-  const double a = g->a, a2 = a * a;
+static void ks_gcon4(double a, double X1, double X2, double gi[4][4]) {
+  const double a2 = a * a;
   const double r = fabs(X1), cth = cos(X2), sth = sin(X2);
   const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
   gi[0][1] = gi[1][0] = orc_ratio(2 * r, rho2); /* spherical_kerr_schild.hpp:86-102 */
@@ -216,10 +197,175 @@ void orc_geom_gcon4(const orc_geom *g, double X1, double X2, double X3, double g
   gi[3][3] = orc_ratio(1, rho2 * sth2);
 }
 
+/* mckinney_gammie_ryan.hpp:66-108 (operator()) and :143-197 (r_, th_, thG_, thJ_) */
+static double mgr_norm(const orc_mgr *m) {
+  return 0.5 * ORC_PI * 1. / (1. + 1. / (m->alpha + 1.) * 1. / pow(m->xt, m->alpha));
+}
+static double mgr_thG(const orc_mgr *m, double X2) {
+  return ORC_PI * X2 + ((1. - m->h) / 2.) * sin(2. * ORC_PI * X2);
+}
+static void mgr_thJ(const orc_mgr *m, double X2, double *y, double *thJ) {
+  *y = 2. * X2 - 1.;
+  *thJ = mgr_norm(m) * *y * (1. + pow(*y / m->xt, m->alpha) / (m->alpha + 1.)) + 0.5 * ORC_PI;
+}
+static double mgr_r(const orc_mgr *m, double X1) {
+  const double dr = X1 - m->hexp_br;
+  return exp(X1 + (dr > 0.) * m->hexp_csq * pow(dr, m->hexp_nsq));
+}
+static double mgr_th(const orc_mgr *m, double X1, double X2, double *y) {
+  const double thG = mgr_thG(m, X2);
+  double th;
+  if (m->derefine_poles) {
+    double thJ;
+    mgr_thJ(m, X2, y, &thJ);
+    th = thG + exp(m->smooth * (m->x0 - X1)) * (thJ - thG);
+  } else {
+    *y = 0;
+    th = thG;
+  }
+  const double SINGSMALL = 1.e-20; /* keep theta off the poles */
+  if (fabs(th) < SINGSMALL) th = th >= 0. ? SINGSMALL : -SINGSMALL;
+  if (fabs(ORC_PI - th) < SINGSMALL) th = th >= ORC_PI ? ORC_PI + SINGSMALL : ORC_PI - SINGSMALL;
+  return th;
+}
+
+void orc_mgr_transform(const orc_mgr *m, double X1, double X2, double X3, double C[3],
+                       double Jcov[3][3], double Jcon[3][3]) {
+  double y;
+  C[0] = mgr_r(m, X1);
+  C[1] = mgr_th(m, X1, X2, &y);
+  C[2] = X3;
+  const double dr = X1 - m->hexp_br;
+  const double drdX1 = exp(X1 + (dr > 0.) * m->hexp_csq * pow(dr, m->hexp_nsq)) *
+                       (1. + (dr > 0.) * m->hexp_csq * m->hexp_nsq * pow(dr, m->hexp_nsq - 1.));
+  const double dthGdX2 = ORC_PI + ORC_PI * (1 - m->h) * cos(2 * ORC_PI * X2);
+  double dthdX1, dthdX2;
+  if (m->derefine_poles) {
+    double thJ;
+    mgr_thJ(m, X2, &y, &thJ);
+    const double thG = mgr_thG(m, X2);
+    const double dthJdX2 = mgr_norm(m) * (1 + pow(y / m->xt, m->alpha)) * 2.;
+    const double e = exp(m->smooth * (m->x0 - X1));
+    dthdX1 = -m->smooth * (thJ - thG) * e;
+    dthdX2 = dthGdX2 + e * (dthJdX2 - dthGdX2);
+  } else {
+    dthdX1 = 0.0;
+    dthdX2 = dthGdX2;
+  }
+  memset(Jcov, 0, 9 * sizeof(double));
+  memset(Jcon, 0, 9 * sizeof(double));
+  Jcov[0][0] = drdX1;
+  Jcov[1][0] = dthdX1;
+  Jcov[1][1] = dthdX2;
+  Jcov[2][2] = 1.;
+  Jcon[0][0] = orc_ratio(1., drdX1);
+  Jcon[1][0] = -orc_ratio(dthdX1, drdX1 * dthdX2);
+  Jcon[1][1] = orc_ratio(1., dthdX2);
+  Jcon[2][2] = 1.;
+}
+
+static double det3(const double A[3][3]) { /* linear_algebra.hpp:50-54 */
+  return (A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] +
+          A[0][2] * A[1][0] * A[2][1] - A[0][2] * A[1][1] * A[2][0] -
+          A[0][1] * A[1][0] * A[2][2] - A[0][0] * A[1][2] * A[2][1]);
+}
+/* S2ST_ of modified_system.hpp:239-245: the spatial Jacobian as a 4x4 with J[0][0] = 1 */
+static double s2st(const double A[3][3], int mu, int nu) {
+  if (mu == 0 || nu == 0) return mu == nu ? 1 : 0;
+  return A[mu - 1][nu - 1];
+}
+
+void orc_geom_point(const orc_geom *g, double X1, double X2, double X3, orc_point_geom *o) {
+  memset(o, 0, sizeof(*o));
+  if (g->kind == ORC_GEOM_MINKOWSKI) { /* minkowski.hpp:35-115 */
+    o->alpha = 1.0;
+    o->gcov[0][0] = -1.0;
+    for (int d = 0; d < 3; ++d) {
+      o->gcov[d + 1][d + 1] = 1.0;
+      o->gcon[d][d] = 1.0;
+      o->gam[d][d] = 1.0;
+    }
+    o->detgamma = 1.0;
+    o->detg = 1.0;
+    return;
+  }
+  if (g->kind == ORC_GEOM_SPHKS) {
+    ks_point(g->a, X1, X2, o);
+    return;
+  }
+  /* FMKS: modified_system.hpp:63-168 over the KS point at (r, theta) */
+  double C[3], Jcov[3][3], Jcon[3][3];
+  orc_mgr_transform(&g->mgr, X1, X2, X3, C, Jcov, Jcon);
+  orc_point_geom k;
+  memset(&k, 0, sizeof(k));
+  ks_point(g->a, C[0], C[1], &k);
+  o->alpha = k.alpha;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o->beta[i] += Jcon[i][j] * k.beta[j];
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int lam = 0; lam < 4; ++lam)
+        for (int kap = 0; kap < 4; ++kap)
+          o->gcov[mu][nu] += k.gcov[lam][kap] * s2st(Jcov, lam, mu) * s2st(Jcov, kap, nu);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int kk = 0; kk < 3; ++kk)
+        for (int l = 0; l < 3; ++l) {
+          o->gam[i][j] += k.gam[kk][l] * Jcov[kk][i] * Jcov[l][j];
+          o->gcon[i][j] += k.gcon[kk][l] * Jcon[i][kk] * Jcon[j][l];
+        }
+  const double detJ = det3((const double(*)[3])Jcov);
+  o->detgamma = fabs(k.detgamma * detJ);
+  o->detg = fabs(k.detg * detJ);
+}
+
+void orc_geom_gcon4(const orc_geom *g, double X1, double X2, double X3, double gi[4][4]) {
+  zero4(gi);
+  if (g->kind == ORC_GEOM_MINKOWSKI) {
+    gi[0][0] = -1;
+    gi[1][1] = gi[2][2] = gi[3][3] = 1;
+    return;
+  }
+  if (g->kind == ORC_GEOM_SPHKS) {
+    ks_gcon4(g->a, X1, X2, gi);
+    return;
+  }
+  double C[3], Jcov[3][3], Jcon[3][3], g0[4][4];
+  orc_mgr_transform(&g->mgr, X1, X2, X3, C, Jcov, Jcon);
+  zero4(g0);
+  ks_gcon4(g->a, C[0], C[1], g0);
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int lam = 0; lam < 4; ++lam)
+        for (int kap = 0; kap < 4; ++kap)
+          gi[mu][nu] += g0[lam][kap] * s2st(Jcon, mu, lam) * s2st(Jcon, nu, kap);
+}
+
+/* geometry_utils.hpp:150-187 (SetMetricGradientByFD): central differences of g_{mu nu}
+ * with step max(|X_d|, 1) * eps^(1/3) */
+static void fd_dg(const orc_geom *g, double X1, double X2, double X3, double dg[4][4][4]) {
+  const double EPS = pow(DBL_EPSILON, 1.0 / 3.0);
+  for (int d = 1; d < 4; ++d) {
+    double L[3] = {X1, X2, X3}, R[3] = {X1, X2, X3};
+    const double dx = smax(fabs(L[d - 1]), 1.0) * EPS;
+    L[d - 1] -= dx;
+    R[d - 1] += dx;
+    orc_point_geom gl, gr;
+    orc_geom_point(g, R[0], R[1], R[2], &gr);
+    orc_geom_point(g, L[0], L[1], L[2], &gl);
+    for (int mu = 0; mu < 4; ++mu)
+      for (int nu = 0; nu < 4; ++nu)
+        dg[mu][nu][d] = orc_ratio(gr.gcov[mu][nu] - gl.gcov[mu][nu], 2 * dx);
+  }
+}
+
 void orc_geom_dg(const orc_geom *g, double X1, double X2, double X3, double dg[4][4][4]) {
-  (void)X3;
   memset(dg, 0, 64 * sizeof(double));
   if (g->kind == ORC_GEOM_MINKOWSKI) return;
+  if (g->kind == ORC_GEOM_FMKS) {
+    fd_dg(g, X1, X2, X3, dg);
+    return;
+  }
   /* spherical_kerr_schild.hpp:151-194: dg[a][b][c] = d_c g_ab */
   const double a = g->a, a2 = a * a;
   const double r = fabs(X1), th = X2, cth = cos(th), sth = sin(th);
@@ -245,9 +391,22 @@ void orc_geom_dg(const orc_geom *g, double X1, double X2, double X3, double dg[4
 }
 
 void orc_geom_gradlnalpha(const orc_geom *g, double X1, double X2, double X3, double da[4]) {
-  (void)X3;
   da[0] = da[1] = da[2] = da[3] = 0.0;
   if (g->kind == ORC_GEOM_MINKOWSKI) return;
+  if (g->kind == ORC_GEOM_FMKS) { /* geometry_utils.hpp:94-123 (SetGradLnAlphaByFD) */
+    const double EPS = pow(DBL_EPSILON, 1.0 / 3.0);
+    for (int d = 1; d < 4; ++d) {
+      double M[3] = {X1, X2, X3}, P[3] = {X1, X2, X3};
+      const double dx = smax(fabs(M[d - 1]), 1.0) * EPS;
+      P[d - 1] += dx;
+      M[d - 1] -= dx;
+      orc_point_geom gm, gp;
+      orc_geom_point(g, M[0], M[1], M[2], &gm);
+      orc_geom_point(g, P[0], P[1], P[2], &gp);
+      da[d] = orc_ratio(gp.alpha - gm.alpha, dx * (gm.alpha + gp.alpha));
+    }
+    return;
+  }
   const double a = g->a, a2 = a * a; /* spherical_kerr_schild.hpp:196-205 */
   const double r = fabs(X1), cth = cos(X2), sth = sin(X2);
   const double rho2 = r * r + a2 * cth * cth, rho4 = rho2 * rho2;

@@ -35,11 +35,20 @@ extern "C" {
 #endif
 
 /* ---- geometry ---------------------------------------------------------------------- */
-enum { ORC_GEOM_MINKOWSKI = 0, ORC_GEOM_SPHKS = 1 };
+enum { ORC_GEOM_MINKOWSKI = 0, ORC_GEOM_SPHKS = 1, ORC_GEOM_FMKS = 2 };
+
+/* FMKS = spherical Kerr-Schild under the McKinney-Gammie-Ryan transformation
+ * (mckinney_gammie_ryan.hpp:45-209, modified_system.hpp:52-248): r = exp(X1) (with the
+ * hyper-exponential stretch beyond hexp_br), theta(X1, X2) with derefined poles. */
+typedef struct {
+  int derefine_poles;
+  double h, xt, alpha, x0, smooth, hexp_br, hexp_nsq, hexp_csq;
+} orc_mgr;
 
 typedef struct {
   int kind;
   double a; /* spin for Kerr-Schild */
+  orc_mgr mgr;
 } orc_geom;
 
 /* metric quantities at one point (Analytic<...> interface, analytic_system.hpp:157-446) */
@@ -48,6 +57,9 @@ typedef struct {
 } orc_point_geom;
 
 void orc_geom_point(const orc_geom *g, double X1, double X2, double X3, orc_point_geom *out);
+/* the MGR transformation: C = (r, theta, phi), Jcov = d(r,th,ph)/dX, Jcon = its inverse */
+void orc_mgr_transform(const orc_mgr *m, double X1, double X2, double X3, double C[3],
+                       double Jcov[3][3], double Jcon[3][3]);
 void orc_geom_gcon4(const orc_geom *g, double X1, double X2, double X3, double gi[4][4]);
 void orc_geom_dg(const orc_geom *g, double X1, double X2, double X3, double dg[4][4][4]);
 void orc_geom_gradlnalpha(const orc_geom *g, double X1, double X2, double X3, double da[4]);

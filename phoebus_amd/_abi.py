@@ -11,7 +11,7 @@ MLP_NPARAM = 8770
 LIBPATH = os.path.join(HERE, "lib", "libphx.so")
 
 PHX_OK, PHX_ERR_INVALID, PHX_ERR_UNSUPPORTED, PHX_ERR_DEVICE, PHX_ERR_NOMEM = 0, -1, -2, -3, -4
-GEOM_MINKOWSKI, GEOM_SPHERICAL_KS = 0, 1
+GEOM_MINKOWSKI, GEOM_SPHERICAL_KS, GEOM_FMKS = 0, 1, 2
 RECON_CONSTANT, RECON_LINEAR, RECON_WENO5Z, RECON_MP5 = 0, 1, 2, 3
 RIEMANN_LLF, RIEMANN_HLL = 0, 1
 BC_OUTFLOW, BC_PERIODIC = 0, 1
@@ -22,8 +22,14 @@ DRIVER_PRIM, DRIVER_CONS, DRIVER_FAIL, DRIVER_C2P_MU = 0, 1, 2, 3
 DBL_MAX = 1.7976931348623157e308
 
 
+class Mgr(C.Structure):
+    _fields_ = [("derefine_poles", C.c_int), ("h", C.c_double), ("xt", C.c_double),
+                ("alpha", C.c_double), ("x0", C.c_double), ("smooth", C.c_double),
+                ("hexp_br", C.c_double), ("hexp_nsq", C.c_double), ("hexp_csq", C.c_double)]
+
+
 class Geometry(C.Structure):
-    _fields_ = [("kind", C.c_int), ("a", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("a", C.c_double), ("mgr", Mgr)]
 
 
 class Bounds(C.Structure):
@@ -103,6 +109,9 @@ def lib():
         "phx_mlp_train_step": (i32, [vp, vp, C.c_int64, vp, vp, C.c_float, C.c_float, vp, vp]),
         "phx_con2prim_batch_learned": (i32, [vp, P(FluidParams), vp, C.c_int64, vp, vp, vp, vp, vp,
                                              vp, vp, vp, vp]),
+        "phx_autotune": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), i32, vp]),
+        "phx_tile_shape": (i32, [vp, P(FluidParams), P(Block), i32, P(i32), P(i32)]),
+        "phx_set_tile_shape": (i32, [vp, P(FluidParams), P(Block), i32, i32, i32]),
         "phx_driver_create": (i32, [vp, P(FluidParams), P(Block), P(i32), dbl, dbl, P(vp)]),
         "phx_driver_destroy": (i32, [vp]),
         "phx_driver_set_prim": (i32, [vp, vp]),

@@ -14,10 +14,16 @@ NPRIM = {True: 10, False: 7}
 NCONS = {True: 8, False: 5}
 
 
+def mgr(derefine_poles=True, h=0.3, xt=0.82, alpha=14.0, x0=0.0, smooth=0.5, hexp_br=1e30,
+        hexp_nsq=4.0, hexp_csq=4.0):
+    """McKinney-Gammie-Ryan (FMKS) parameters; defaults of fmks.cpp:37-52."""
+    return A.Mgr(int(derefine_poles), h, xt, alpha, x0, smooth, hexp_br, hexp_nsq, hexp_csq)
+
+
 def fluid_params(gamma=5.0 / 3.0, cv=1.0, mhd=True, recon=A.RECON_WENO5Z,
                  riemann=A.RIEMANN_HLL, c2p_tol=1e-10, c2p_max_iter=100,
                  geom=A.GEOM_MINKOWSKI, a=0.0, floor_scale_fac=1.0, fail_on_floors=False,
-                 fail_on_ceilings=False, floors=None, ceilings=None):
+                 fail_on_ceilings=False, floors=None, ceilings=None, mgr_params=None):
     """FluidParams with floors and ceilings disabled unless given (fixup.hpp defaults)."""
     b = A.Bounds(A.FLOOR_CONSTANT_RHO_SIE, -A.DBL_MAX, -A.DBL_MAX, 0.0, 0.0, 0.0, 0.0,
                  A.DBL_MAX, A.DBL_MAX)
@@ -28,7 +34,7 @@ def fluid_params(gamma=5.0 / 3.0, cv=1.0, mhd=True, recon=A.RECON_WENO5Z,
         b.gam_max, b.sie_max = ceilings
     return A.FluidParams(gamma, cv, int(mhd), recon, riemann, c2p_tol, c2p_max_iter,
                          floor_scale_fac, int(fail_on_floors), int(fail_on_ceilings), b,
-                         A.Geometry(geom, a))
+                         A.Geometry(geom, a, mgr_params if mgr_params is not None else mgr()))
 
 
 def block(ndim, nx, ng=4, xmin=(0.0, 0.0, 0.0), dx=(1.0, 1.0, 1.0)):
@@ -126,6 +132,20 @@ class Phx:
                                                 C.byref(a), stream), "phx_fluid_stage")
 
     # ---- batched con2prim ----------------------------------------------------------------
+    def autotune(self, p, b, f, reps=3, stream=None):
+        return self._chk(self.L.phx_autotune(self.h, C.byref(p), C.byref(b), C.byref(f), reps,
+                                             stream), "phx_autotune")
+
+    def tile_shape(self, p, b, d):
+        ti, ts = C.c_int(), C.c_int()
+        self._chk(self.L.phx_tile_shape(self.h, C.byref(p), C.byref(b), d, C.byref(ti),
+                                        C.byref(ts)), "phx_tile_shape")
+        return ti.value, ts.value
+
+    def set_tile_shape(self, p, b, d, ti, ts):
+        return self._chk(self.L.phx_set_tile_shape(self.h, C.byref(p), C.byref(b), d, ti, ts),
+                         "phx_set_tile_shape")
+
     def con2prim_batch(self, p, n, coords, cons, prim, mu, fail, iters=None, stream=None):
         return self._chk(self.L.phx_con2prim_batch(self.h, C.byref(p), n, _p(coords), _p(cons),
                                                    _p(prim), _p(mu), _p(fail), _p(iters),

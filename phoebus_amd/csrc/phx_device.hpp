@@ -31,6 +31,12 @@ struct DevParams {
   int fail_on_floors, fail_on_ceilings;
   double a;                // Kerr-Schild spin
   DevBounds bnd;
+  // McKinney-Gammie-Ryan transformation (FMKS); norm = 0.5 pi / (1 + xt^-alpha/(alpha+1))
+  struct Mgr {
+    int derefine_poles;
+    double h, xt, alpha, x0, smooth, hexp_br, hexp_nsq, hexp_csq, norm;
+  } mgr;
+  double fd_eps;  // eps^(1/3), the finite-difference scale of the FMKS derivatives
 };
 
 // ---- block geometry as the kernels see it ---------------------------------------------
@@ -42,6 +48,17 @@ struct BlockGeom {
   double xmin[3], dx[3];
   int istart[3];     // parthenon's istart: ng in active dimensions, 0 otherwise
 };
+
+// The hardware hands consecutive workgroups of a launch to the 8 XCDs in turn, and each
+// XCD has its own L2.  Renumbering so that each XCD receives a contiguous range of
+// logical blocks keeps neighbouring tiles (which share cache lines at their edges and
+// along i) in one L2.  A bijection on [0, n) for any n.
+__device__ __forceinline__ unsigned xcd_remap(unsigned bx, unsigned n) {
+  constexpr unsigned kXcd = 8;
+  const unsigned per = n / kXcd, rem = n % kXcd;
+  const unsigned x = bx % kXcd, idx = bx / kXcd;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
 
 __device__ __forceinline__ double xc(const BlockGeom &b, int d, int idx) {
   return b.xmin[d] + (idx - b.istart[d] + 0.5) * b.dx[d];
@@ -182,6 +199,8 @@ __device__ __forceinline__ void reconstruct_edges(const double *q, double &ql, d
   }
 }
 
+constexpr double kPi = 3.14159265358979323846;  // M_PI
+
 // ---- metrics -----------------------------------------------------------------------------
 // A metric policy evaluates alpha, beta^i, g_{mu nu}, gamma^{ij}, sqrt(gamma), sqrt(-g) at
 // one point.  nz_cov / nz_con / nz_beta mark entries that can be nonzero; loops below
@@ -191,7 +210,7 @@ struct MinkowskiMetric {  // minkowski.hpp:35-115
   static constexpr __host__ __device__ bool nz_cov(int m, int n) { return m == n; }
   static constexpr __host__ __device__ bool nz_con(int m, int n) { return m == n; }
   static constexpr __host__ __device__ bool nz_beta(int) { return false; }
-  __device__ MinkowskiMetric(double, double, double, double) {}
+  __device__ MinkowskiMetric(const DevParams &, double, double, double) {}
   __device__ double alpha() const { return 1.0; }
   __device__ double beta(int) const { return 0.0; }
   __device__ double gcov(int m, int n) const { return m == n ? (m == 0 ? -1.0 : 1.0) : 0.0; }
@@ -211,7 +230,9 @@ struct SphericalKSMetric {  // spherical_kerr_schild.hpp:42-243, X = (r, theta, 
   static constexpr __host__ __device__ bool nz_beta(int i) { return i == 0; }
   double alpha_, beta0_, g00_, g01_, g03_, g11_, g13_, g22_, g33_;
   double c00_, c02_, c11_, c22_, detgamma_, detg_;
-  __device__ SphericalKSMetric(double a, double X1, double X2, double) {
+  __device__ SphericalKSMetric(const DevParams &p, double X1, double X2, double)
+      : SphericalKSMetric(p.a, X1, X2) {}
+  __device__ SphericalKSMetric(double a, double X1, double X2) {
     const double a2 = a * a;
     const double r = fabs(X1), cth = cos(X2), sth = sin(X2);
     const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
@@ -250,12 +271,104 @@ struct SphericalKSMetric {  // spherical_kerr_schild.hpp:42-243, X = (r, theta, 
   __device__ double detg() const { return detg_; }
 };
 
+// McKinney-Gammie-Ryan map X -> (r, theta, phi) and its Jacobian
+// (mckinney_gammie_ryan.hpp:66-108, 143-197): J = [[r1, 0, 0], [t1, t2, 0], [0, 0, 1]]
+// with r1 = dr/dX1, t1 = dtheta/dX1, t2 = dtheta/dX2.
+struct MgrMap {
+  double r, th, r1, t1, t2;
+  __device__ MgrMap(const DevParams::Mgr &m, double X1, double X2) {
+    const double dr = X1 - m.hexp_br;
+    const double on = dr > 0. ? 1. : 0.;
+    r = exp(X1 + on * m.hexp_csq * pow(dr, m.hexp_nsq));
+    r1 = exp(X1 + on * m.hexp_csq * pow(dr, m.hexp_nsq)) *
+         (1. + on * m.hexp_csq * m.hexp_nsq * pow(dr, m.hexp_nsq - 1.));
+    const double thG = kPi * X2 + ((1. - m.h) / 2.) * sin(2. * kPi * X2);
+    const double dthGdX2 = kPi + kPi * (1 - m.h) * cos(2 * kPi * X2);
+    if (m.derefine_poles) {
+      const double y = 2. * X2 - 1.;
+      const double py = pow(y / m.xt, m.alpha);
+      const double thJ = m.norm * y * (1. + py / (m.alpha + 1.)) + 0.5 * kPi;
+      const double e = exp(m.smooth * (m.x0 - X1));
+      th = thG + e * (thJ - thG);
+      t1 = -m.smooth * (thJ - thG) * e;
+      t2 = dthGdX2 + e * (m.norm * (1 + py) * 2. - dthGdX2);
+    } else {
+      th = thG;
+      t1 = 0.0;
+      t2 = dthGdX2;
+    }
+    constexpr double kSingSmall = 1.e-20;  // keep theta off the poles
+    if (fabs(th) < kSingSmall) th = th >= 0. ? kSingSmall : -kSingSmall;
+    if (fabs(kPi - th) < kSingSmall) th = th >= kPi ? kPi + kSingSmall : kPi - kSingSmall;
+  }
+};
+
+// FMKS (modified_system.hpp:63-168): spherical Kerr-Schild at (r, theta) contracted with
+// the MGR Jacobian, term by term in the reference's summation order (zero terms elided).
+struct FMKSMetric {
+  static constexpr __host__ __device__ bool nz_cov(int m, int n) {
+    return !((m + n == 2 && m != 1) || m + n == 5);  // g_{02} = g_{23} = 0
+  }
+  static constexpr __host__ __device__ bool nz_con(int, int) { return true; }
+  static constexpr __host__ __device__ bool nz_beta(int i) { return i < 2; }
+  double alpha_, b0_, b1_, g00_, g01_, g03_, g11_, g12_, g13_, g22_, g33_;
+  double c00_, c01_, c02_, c11_, c12_, c22_, detgamma_, detg_;
+  // Jcon = J^-1: [[1/r1, 0, 0], [-t1/(r1 t2), 1/t2, 0], [0, 0, 1]]
+  double j00_, j10_, j11_;
+  __device__ FMKSMetric(const DevParams &p, double X1, double X2, double) {
+    const MgrMap m(p.mgr, X1, X2);
+    const SphericalKSMetric k(p.a, m.r, m.th);
+    j00_ = ratio(1., m.r1);
+    j10_ = -ratio(m.t1, m.r1 * m.t2);
+    j11_ = ratio(1., m.t2);
+    alpha_ = k.alpha_;
+    b0_ = j00_ * k.beta0_;
+    b1_ = j10_ * k.beta0_;
+    g00_ = k.g00_;
+    g01_ = k.g01_ * m.r1;
+    g03_ = k.g03_;
+    g11_ = k.g11_ * m.r1 * m.r1 + k.g22_ * m.t1 * m.t1;
+    g12_ = k.g22_ * m.t1 * m.t2;
+    g13_ = k.g13_ * m.r1;
+    g22_ = k.g22_ * m.t2 * m.t2;
+    g33_ = k.g33_;
+    c00_ = k.c00_ * j00_ * j00_;
+    c01_ = k.c00_ * j00_ * j10_;
+    c02_ = k.c02_ * j00_;
+    c11_ = k.c00_ * j10_ * j10_ + k.c11_ * j11_ * j11_;
+    c12_ = k.c02_ * j10_;
+    c22_ = k.c22_;
+    const double detJ = m.r1 * m.t2;  // linear_algebra.hpp:50-54 on a triangular J
+    detgamma_ = fabs(k.detgamma_ * detJ);
+    detg_ = fabs(k.detg_ * detJ);
+  }
+  __device__ double alpha() const { return alpha_; }
+  __device__ double beta(int i) const { return i == 0 ? b0_ : i == 1 ? b1_ : 0.0; }
+  __device__ double gcov(int m, int n) const {
+    const int lo = m < n ? m : n, hi = m < n ? n : m;
+    if (lo == 0) return hi == 0 ? g00_ : hi == 1 ? g01_ : hi == 3 ? g03_ : 0.0;
+    if (lo == 1) return hi == 1 ? g11_ : hi == 2 ? g12_ : g13_;
+    if (lo == 2) return hi == 2 ? g22_ : 0.0;
+    return g33_;
+  }
+  __device__ double gcon(int m, int n) const {
+    const int lo = m < n ? m : n, hi = m < n ? n : m;
+    if (lo == 0) return hi == 0 ? c00_ : hi == 1 ? c01_ : c02_;
+    if (lo == 1) return hi == 1 ? c11_ : c12_;
+    return c22_;
+  }
+  __device__ double detgamma() const { return detgamma_; }
+  __device__ double detg() const { return detg_; }
+};
+
 template <int GEOM>
 struct MetricOf;
 template <>
 struct MetricOf<PHX_GEOM_MINKOWSKI> { using type = MinkowskiMetric; };
 template <>
 struct MetricOf<PHX_GEOM_SPHERICAL_KS> { using type = SphericalKSMetric; };
+template <>
+struct MetricOf<PHX_GEOM_FMKS> { using type = FMKSMetric; };
 
 // ---- bounds: fixup.hpp:91-134, 188-213 ------------------------------------------------
 __device__ __forceinline__ void get_floors(const DevBounds &b, double x1, double x2,

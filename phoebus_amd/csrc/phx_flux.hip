@@ -37,7 +37,7 @@ __device__ __forceinline__ void face_out(const DevParams &p, int dir, const Bloc
                                          long c, double *__restrict__ flux,
                                          double *__restrict__ fsig) {
   constexpr int NC = Layout<MHD>::ncons;
-  const M g(p.a, X[0], X[1], X[2]);
+  const M g(p, X[0], X[1], X[2]);
   double f[NC];
   const double s = riemann_flux<MHD, RIEMANN>(p, dir, g, X, ql, qr, f);
 #pragma unroll
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
   const int fk = b.ie[2] - b.is[2] + 1 + 2 * dk;
   const long total = (long)b.ni * fj * fk;  // full rows of the face range
   const int t = threadIdx.x;
-  long L = (long)blockIdx.x * (XB - 2 * H - 1) + t;
+  long L = (long)xcd_remap(blockIdx.x, gridDim.x) * (XB - 2 * H - 1) + t;
   const bool valid = L < total;
   if (!valid) L = total - 1;
   const int i = (int)(L % b.ni);
@@ -108,11 +108,13 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
 // Rows of ti_n consecutive i are read into LDS; each lane reconstructs its cell along the
 // own direction from LDS and publishes the upper edge for the lane one row up.  Lanes in
 // own-rows H+1 .. ts_n-H-1 emit faces; tiles along the own direction overlap by 2H+1
-// cells.  The host sizes ts_n to the whole own extent of the face range when it fits
-// (135 cells for a 128-zone WENO5 block), which keeps 92% of the lanes emitting; the
-// kernel is FP64-issue-bound, so the narrow rows cost little in load efficiency.
-constexpr int kTileThreads = 512;
+// cells.  The shape trades redundant halo work (short ts_n) against partial cache lines
+// (narrow ti_n: a row segment of ti_n doubles uses ti_n/16 of a 128-byte line); it is
+// chosen per block size and reconstruction by the autotuner (phx_autotune), else by the
+// default below.  Blocks are renumbered XCD by XCD so neighbouring i chunks share an L2.
+constexpr int kTileMaxThreads = 1024;
 constexpr int kTileMaxRows = 256;
+constexpr int kTileMaxLdsBytes = 64 * 1024;  // two blocks per CU
 
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
 __global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, in
   const int dt_ = b.ndim > T;
   const int nic = (b.ie[0] - b.is[0] + 3 + ti_n - 1) / ti_n;  // i chunks over [is-1, ie+1]
   const int nft = (b.ie[DIR] - b.is[DIR] + 3 + S - 1) / S;    // tiles over faces [s-1, e+1]
-  const int bx = blockIdx.x;
+  const int bx = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int ichunk = bx % nic;
   const int tile = (bx / nic) % nft;
   const int tr = bx / (nic * nft);
@@ -186,9 +188,29 @@ __global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, in
 }
 
 // ---- dispatch --------------------------------------------------------------------------
+TileShape resolve_tile_shape(int d, int recon, int mhd, const BlockGeom &bg, TileShape want) {
+  const int H = recon == PHX_RECON_CONSTANT ? 0 : recon == PHX_RECON_LINEAR ? 1 : 2;
+  const int NP = mhd ? 10 : 7;
+  // own-direction cells the face range needs: s-2-H .. e+1+H
+  const int need = bg.ie[d] - bg.is[d] + 4 + 2 * H;
+  const int cap = need <= kTileMaxRows ? need : kTileMaxRows;
+  TileShape t = want;
+  if (t.ts <= 0) t.ts = cap < 64 ? cap : 64;
+  if (t.ti <= 0) t.ti = 8;
+  if (t.ts > cap) t.ts = cap;
+  if (t.ts < 2 * H + 2) t.ts = 2 * H + 2 <= cap ? 2 * H + 2 : cap;
+  if (t.ti < 1) t.ti = 1;
+  while (t.ti > 1 && (t.ti * t.ts > kTileMaxThreads ||
+                      (long)NP * t.ti * t.ts * 8 > kTileMaxLdsBytes))
+    t.ti /= 2;
+  while (t.ti * t.ts > kTileMaxThreads || (long)NP * t.ti * t.ts * 8 > kTileMaxLdsBytes)
+    t.ts /= 2;
+  return t;
+}
+
 template <int RECON, int RIEMANN, bool MHD, int GEOM>
 static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const double *prim,
-                       double *flux, double *fsig, hipStream_t s) {
+                       double *flux, double *fsig, hipStream_t s, TileShape shape) {
   const int dj = bg.ndim > 1, dk = bg.ndim > 2;
   if (d == 0) {
     constexpr int H = ReconWidth<RECON>::H;
@@ -204,10 +226,8 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
   constexpr int NP = Layout<MHD>::nprim;
   const int T = d == 1 ? 2 : 1;
   const int dt_ = bg.ndim > T;
-  // own-direction cells the face range needs: s-2-H .. e+1+H
-  const int need = bg.ie[d] - bg.is[d] + 4 + 2 * H;
-  const int ts_n = need <= kTileMaxRows ? need : kTileMaxRows;
-  const int ti_n = kTileThreads / ts_n < 16 ? kTileThreads / ts_n : 16;
+  const TileShape ts = resolve_tile_shape(d, RECON, MHD, bg, shape);
+  const int ts_n = ts.ts, ti_n = ts.ti;
   const int S = ts_n - 2 * H - 1;
   const long nic = (bg.ie[0] - bg.is[0] + 3 + ti_n - 1) / ti_n;
   const long nft = (bg.ie[d] - bg.is[d] + 3 + S - 1) / S;
@@ -225,47 +245,48 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
 
 template <int RECON, int RIEMANN, bool MHD>
 static void launch_geom(int geom, int d, const DevParams &dp, const BlockGeom &bg,
-                        const double *prim, double *flux, double *fsig, hipStream_t s) {
-  if (geom == PHX_GEOM_MINKOWSKI)
-    launch_dir<RECON, RIEMANN, MHD, PHX_GEOM_MINKOWSKI>(d, dp, bg, prim, flux, fsig, s);
-  else
-    launch_dir<RECON, RIEMANN, MHD, PHX_GEOM_SPHERICAL_KS>(d, dp, bg, prim, flux, fsig, s);
+                        const double *prim, double *flux, double *fsig, hipStream_t s,
+                        TileShape sh) {
+  with_geom(geom, [&](auto G) {
+    launch_dir<RECON, RIEMANN, MHD, decltype(G)::value>(d, dp, bg, prim, flux, fsig, s, sh);
+  });
 }
 
 template <int RECON, int RIEMANN>
 static void launch_mhd(int mhd, int geom, int d, const DevParams &dp, const BlockGeom &bg,
-                       const double *prim, double *flux, double *fsig, hipStream_t s) {
+                       const double *prim, double *flux, double *fsig, hipStream_t s,
+                       TileShape sh) {
   if (mhd)
-    launch_geom<RECON, RIEMANN, true>(geom, d, dp, bg, prim, flux, fsig, s);
+    launch_geom<RECON, RIEMANN, true>(geom, d, dp, bg, prim, flux, fsig, s, sh);
   else
-    launch_geom<RECON, RIEMANN, false>(geom, d, dp, bg, prim, flux, fsig, s);
+    launch_geom<RECON, RIEMANN, false>(geom, d, dp, bg, prim, flux, fsig, s, sh);
 }
 
 template <int RECON>
 static void launch_riemann(int riemann, int mhd, int geom, int d, const DevParams &dp,
                            const BlockGeom &bg, const double *prim, double *flux,
-                           double *fsig, hipStream_t s) {
+                           double *fsig, hipStream_t s, TileShape sh) {
   if (riemann == PHX_RIEMANN_HLL)
-    launch_mhd<RECON, PHX_RIEMANN_HLL>(mhd, geom, d, dp, bg, prim, flux, fsig, s);
+    launch_mhd<RECON, PHX_RIEMANN_HLL>(mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
   else
-    launch_mhd<RECON, PHX_RIEMANN_LLF>(mhd, geom, d, dp, bg, prim, flux, fsig, s);
+    launch_mhd<RECON, PHX_RIEMANN_LLF>(mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
 }
 
 void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevParams &dp,
                    const BlockGeom &bg, const double *prim, double *flux, double *fsig,
-                   hipStream_t s) {
+                   hipStream_t s, TileShape sh) {
   switch (recon) {
   case PHX_RECON_CONSTANT:
-    launch_riemann<PHX_RECON_CONSTANT>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s);
+    launch_riemann<PHX_RECON_CONSTANT>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   case PHX_RECON_LINEAR:
-    launch_riemann<PHX_RECON_LINEAR>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s);
+    launch_riemann<PHX_RECON_LINEAR>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   case PHX_RECON_WENO5Z:
-    launch_riemann<PHX_RECON_WENO5Z>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s);
+    launch_riemann<PHX_RECON_WENO5Z>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   default:
-    launch_riemann<PHX_RECON_MP5>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s);
+    launch_riemann<PHX_RECON_MP5>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   }
 }

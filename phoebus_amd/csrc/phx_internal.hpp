@@ -3,7 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <tuple>
+#include <type_traits>
 
 #include "phx.h"
 
@@ -34,11 +37,39 @@ inline BlockGeom make_block_geom(const phx_block *b) {
   return g;
 }
 
+// Run f with the geometry (resp. MHD flag) as a compile-time constant:
+// with_geom(kind, [&](auto G) { ... decltype(G)::value ... }).
+template <int G>
+using GeomTag = std::integral_constant<int, G>;
+template <class F>
+inline void with_geom(int kind, F &&f) {
+  switch (kind) {
+  case PHX_GEOM_MINKOWSKI: f(GeomTag<PHX_GEOM_MINKOWSKI>()); break;
+  case PHX_GEOM_SPHERICAL_KS: f(GeomTag<PHX_GEOM_SPHERICAL_KS>()); break;
+  default: f(GeomTag<PHX_GEOM_FMKS>()); break;
+  }
+}
+template <class F>
+inline void with_mhd(int mhd, F &&f) {
+  if (mhd)
+    f(std::true_type());
+  else
+    f(std::false_type());
+}
+
+// shape of the y/z flux tiles: ti consecutive i by ts cells along the sweep direction
+// (0 = the built-in default); results do not depend on it
+struct TileShape {
+  int ti = 0, ts = 0;
+};
+
 // flux launches (phx_flux.hip): direction d of fluid.cpp:753-770 with the fused
 // reconstruction; writes flux[d][m][cells] and fsig[d][cells]
 void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevParams &dp,
                    const BlockGeom &bg, const double *prim, double *flux, double *fsig,
-                   hipStream_t s);
+                   hipStream_t s, TileShape shape = TileShape());
+// the shape launch_fluxes uses for direction d (after defaults and limits)
+TileShape resolve_tile_shape(int d, int recon, int mhd, const BlockGeom &bg, TileShape want);
 
 DevParams make_dev_params(const phx_fluid_params *p);
 int check_params(phx_ctx *ctx, const phx_fluid_params *p);
@@ -67,6 +98,17 @@ struct phx_ctx {
 
   int device = 0;
   hipStream_t stream = nullptr;  // default stream of the context
+  // autotuned y/z flux tile shapes per (recon, riemann, mhd, geom, ndim, nx, ny, nz, ng, d)
+  using TileKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
+  std::map<TileKey, phx::TileShape> tiles;
+  static TileKey tile_key(const phx_fluid_params *p, const phx_block *b, int d) {
+    return TileKey(p->recon, p->riemann, p->mhd, p->geom.kind, b->ndim, b->nx[0], b->nx[1],
+                   b->nx[2], b->ng, d);
+  }
+  phx::TileShape tile_for(const phx_fluid_params *p, const phx_block *b, int d) const {
+    const auto it = tiles.find(tile_key(p, b, d));
+    return it == tiles.end() ? phx::TileShape() : it->second;
+  }
   double *dt_partial = nullptr;  // [kDtPartials]
   std::string last_error;
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstdint>
 #include <cstdio>
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256) void k_con2prim(DevParams p, BlockGeom b, Rang
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
   const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  const M g(p.a, X[0], X[1], X[2]);
+  const M g(p, X[0], X[1], X[2]);
   double cs[L::ncons], pr[L::nprim], sig[3];
 #pragma unroll
   for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(256) void k_prim_to_con(DevParams p, BlockGeom b, R
   const int k = (int)(t / ((long)rg.n[0] * rg.n[1])) + rg.lo[2];
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
-  const M g(p.a, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
+  const M g(p, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
   double pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(1024) void k_dt_finish(const double *__restrict__ p
   if (threadIdx.x == 0) dt[0] = cfl * v;
 }
 
-// ---- geometric sources for spherical Kerr-Schild (fluid.cpp:566-657, tmunu.hpp) -------
+// ---- geometric sources for curved metrics (fluid.cpp:566-657, tmunu.hpp) ----------------
 struct KSDerivs {
   double dg[4][4][4];
   double da[4];
@@ -300,11 +301,64 @@ __device__ void ks_derivs(double a, double X1, double X2, KSDerivs &o) {
   o.gi[3][3] = ratio(1, rho2 * sth2);
 }
 
-template <bool MHD>
-__global__ __launch_bounds__(128) void k_sources_ks(DevParams p, BlockGeom b,
-                                                    const double *__restrict__ prim,
-                                                    double *__restrict__ src) {
+// FMKS derivatives by central differences (geometry_utils.hpp:94-187): step
+// max(|X_d|, 1) eps^(1/3) per direction, g_{mu nu} and alpha at the same two points; the
+// inverse spacetime metric from the Kerr-Schild one and the Jacobian inverse
+// (modified_system.hpp:96-112), in the reference's summation order.
+__device__ void fmks_derivs(const DevParams &p, double X1, double X2, double X3,
+                            KSDerivs &o) {
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y) {
+      o.gi[x][y] = 0.0;
+      for (int z = 0; z < 4; ++z) o.dg[x][y][z] = 0.0;
+    }
+  o.da[0] = 0.0;
+  const double X[3] = {X1, X2, X3};
+  for (int d = 1; d < 4; ++d) {
+    double L[3] = {X1, X2, X3}, R[3] = {X1, X2, X3};
+    const double dx = smax(fabs(X[d - 1]), 1.0) * p.fd_eps;
+    L[d - 1] -= dx;
+    R[d - 1] += dx;
+    const FMKSMetric gr(p, R[0], R[1], R[2]);
+    const FMKSMetric gl(p, L[0], L[1], L[2]);
+    for (int mu = 0; mu < 4; ++mu)
+      for (int nu = 0; nu < 4; ++nu)
+        o.dg[mu][nu][d] = ratio(gr.gcov(mu, nu) - gl.gcov(mu, nu), 2 * dx);
+    o.da[d] = ratio(gr.alpha() - gl.alpha(), dx * (gl.alpha() + gr.alpha()));
+  }
+  const MgrMap m(p.mgr, X1, X2);
+  KSDerivs k;
+  ks_derivs(p.a, m.r, m.th, k);  // only k.gi is used
+  double SJ[4][4] = {};
+  SJ[0][0] = 1.0;
+  SJ[1][1] = ratio(1., m.r1);
+  SJ[2][1] = -ratio(m.t1, m.r1 * m.t2);
+  SJ[2][2] = ratio(1., m.t2);
+  SJ[3][3] = 1.0;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu) {
+      double acc = 0.0;
+      for (int lam = 0; lam < 4; ++lam)
+        for (int kap = 0; kap < 4; ++kap) acc += k.gi[lam][kap] * SJ[mu][lam] * SJ[nu][kap];
+      o.gi[mu][nu] = acc;
+    }
+}
+
+template <int GEOM>
+__device__ __forceinline__ void geom_derivs(const DevParams &p, double X1, double X2,
+                                            double X3, KSDerivs &o) {
+  if constexpr (GEOM == PHX_GEOM_FMKS)
+    fmks_derivs(p, X1, X2, X3, o);
+  else
+    ks_derivs(p.a, X1, X2, o);
+}
+
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(128) void k_sources(DevParams p, BlockGeom b,
+                                                 const double *__restrict__ prim,
+                                                 double *__restrict__ src) {
   using L = Layout<MHD>;
+  using M = typename MetricOf<GEOM>::type;
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nk) return;
@@ -314,9 +368,9 @@ __global__ __launch_bounds__(128) void k_sources_ks(DevParams p, BlockGeom b,
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
   const double X1 = xc(b, 0, i), X2 = xc(b, 1, j), X3 = xc(b, 2, k);
-  const SphericalKSMetric g(p.a, X1, X2, X3);
+  const M g(p, X1, X2, X3);
   KSDerivs dv;
-  ks_derivs(p.a, X1, X2, dv);
+  geom_derivs<GEOM>(p, X1, X2, X3, dv);
   double pr[L::nprim];
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
   // tmunu.hpp:94-126
@@ -401,7 +455,7 @@ __global__ __launch_bounds__(256) void k_con2prim_batch(DevParams p, long n,
     X[0] = coords[s];
     X[1] = coords[n + s];
   }
-  const M g(p.a, X[0], X[1], X[2]);
+  const M g(p, X[0], X[1], X[2]);
   double cs[L::ncons], pr[L::nprim], sig[3];
 #pragma unroll
   for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + s];
@@ -428,7 +482,7 @@ __global__ __launch_bounds__(256) void k_prim_to_con_batch(DevParams p, long n,
   using L = Layout<MHD>;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const M g(p.a, coords ? coords[s] : 0.0, coords ? coords[n + s] : 0.0, 0.0);
+  const M g(p, coords ? coords[s] : 0.0, coords ? coords[n + s] : 0.0, 0.0);
   double pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + s];
@@ -470,6 +524,19 @@ DevParams make_dev_params(const phx_fluid_params *p) {
   d.bnd.sie_min_eos = p->bounds.sie_min_eos;
   d.bnd.gam_max = p->bounds.gam_max;
   d.bnd.sie_max = p->bounds.sie_max;
+  const phx_mgr &m = p->geom.mgr;
+  d.mgr.derefine_poles = m.derefine_poles;
+  d.mgr.h = m.h;
+  d.mgr.xt = m.xt;
+  d.mgr.alpha = m.alpha;
+  d.mgr.x0 = m.x0;
+  d.mgr.smooth = m.smooth;
+  d.mgr.hexp_br = m.hexp_br;
+  d.mgr.hexp_nsq = m.hexp_nsq;
+  d.mgr.hexp_csq = m.hexp_csq;
+  // mckinney_gammie_ryan.hpp:192-194 (GetNorm_), on the host in the same operation order
+  d.mgr.norm = 0.5 * M_PI * 1. / (1. + 1. / (m.alpha + 1.) * 1. / std::pow(m.xt, m.alpha));
+  d.fd_eps = std::pow(DBL_EPSILON, 1.0 / 3.0);  // geometry_utils.hpp:153
   return d;
 }
 
@@ -480,8 +547,12 @@ int check_params(phx_ctx *ctx, const phx_fluid_params *p) {
     return ctx->fail(PHX_ERR_INVALID, "invalid reconstruction");
   if (p->riemann != PHX_RIEMANN_LLF && p->riemann != PHX_RIEMANN_HLL)
     return ctx->fail(PHX_ERR_INVALID, "invalid Riemann solver");
-  if (p->geom.kind != PHX_GEOM_MINKOWSKI && p->geom.kind != PHX_GEOM_SPHERICAL_KS)
+  if (p->geom.kind != PHX_GEOM_MINKOWSKI && p->geom.kind != PHX_GEOM_SPHERICAL_KS &&
+      p->geom.kind != PHX_GEOM_FMKS)
     return ctx->fail(PHX_ERR_UNSUPPORTED, "unsupported geometry");
+  if (p->geom.kind == PHX_GEOM_FMKS &&
+      !(p->geom.mgr.xt > 0. && p->geom.mgr.alpha > -1. && p->geom.mgr.hexp_nsq >= 1.))
+    return ctx->fail(PHX_ERR_INVALID, "invalid FMKS transformation parameters");
   if (!(p->c2p_floor_scale_fac <= 1. && p->c2p_floor_scale_fac > 0.))
     return ctx->fail(PHX_ERR_INVALID, "c2p_floor_scale_fac must be in (0, 1]");
   if (p->bounds.floor_type < 0 || p->bounds.floor_type > PHX_FLOOR_R_RHO_SIE)
@@ -530,10 +601,14 @@ void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
     return;
   }
   const long n = (long)(bg.ie[0] - bg.is[0] + 1) * (bg.ie[1] - bg.is[1] + 1) * (bg.ie[2] - bg.is[2] + 1);
-  if (mhd)
-    hipLaunchKernelGGL(k_sources_ks<true>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, prim, src);
-  else
-    hipLaunchKernelGGL(k_sources_ks<false>, dim3(nblocks(n, 128)), dim3(128), 0, s, dp, bg, prim, src);
+  with_geom(geom, [&](auto G) {
+    constexpr int GEOM = decltype(G)::value;
+    if constexpr (GEOM != PHX_GEOM_MINKOWSKI)
+      with_mhd(mhd, [&](auto H) {
+        hipLaunchKernelGGL((k_sources<decltype(H)::value, GEOM>), dim3(nblocks(n, 128)), dim3(128),
+                           0, s, dp, bg, prim, src);
+      });
+  });
 }
 
 void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s) {
@@ -557,12 +632,9 @@ int phx_calculate_fluxes(phx_ctx *ctx, const phx_fluid_params *p, const phx_bloc
   const DevParams dp = make_dev_params(p);
   const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
-  const long nface = (long)(b->nx[0] + 2) * (b->nx[1] + (b->ndim > 1 ? 2 : 0)) *
-                     (b->nx[2] + (b->ndim > 2 ? 2 : 0));
-  (void)nface;
   for (int d = 0; d < b->ndim; ++d)
     launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
-                  f->face_sig, s);
+                  f->face_sig, s, ctx->tile_for(p, b, d));
   return ctx->check_launch();
 }
 
@@ -640,13 +712,11 @@ int phx_con2prim(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
   const dim3 grid(nblocks(n, 256)), blk(256);
 #define C2P_ARGS dp, bg, rg, f->cons, f->prim, f->temp, f->c2p_mu, f->cell_sig, f->fail, iters
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_con2prim<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, C2P_ARGS);
-    else hipLaunchKernelGGL((k_con2prim<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, C2P_ARGS);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_con2prim<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, C2P_ARGS);
-    else hipLaunchKernelGGL((k_con2prim<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, C2P_ARGS);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_con2prim<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, C2P_ARGS);
+    });
+  });
 #undef C2P_ARGS
   return ctx->check_launch();
 }
@@ -666,13 +736,11 @@ int phx_prim_to_con(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   hipStream_t s = ctx->stream_of(stream);
   const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
   const dim3 grid(nblocks(n, 256)), blk(256);
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
-    else hipLaunchKernelGGL((k_prim_to_con<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
-    else hipLaunchKernelGGL((k_prim_to_con<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_prim_to_con<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
+    });
+  });
   return ctx->check_launch();
 }
 
@@ -709,13 +777,11 @@ int phx_con2prim_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
   const DevParams dp = make_dev_params(p);
   hipStream_t s = ctx->stream_of(stream);
   const dim3 grid(nblocks(n, 256)), blk(256);
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
-    else hipLaunchKernelGGL((k_con2prim_batch<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
-    else hipLaunchKernelGGL((k_con2prim_batch<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_con2prim_batch<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, dp, (long)n, coords, cons, prim, mu, fail, iters);
+    });
+  });
   return ctx->check_launch();
 }
 
@@ -733,13 +799,11 @@ int phx_prim_to_con_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
   const DevParams dp = make_dev_params(p);
   hipStream_t s = ctx->stream_of(stream);
   const dim3 grid(nblocks(n, 256)), blk(256);
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con_batch<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
-    else hipLaunchKernelGGL((k_prim_to_con_batch<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_prim_to_con_batch<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
-    else hipLaunchKernelGGL((k_prim_to_con_batch<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_prim_to_con_batch<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, dp, (long)n, coords, prim, cons);
+    });
+  });
   return ctx->check_launch();
 }
 

@@ -54,6 +54,8 @@ constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid, oB2 = oW2 + kHi
 constexpr int oW3 = oB2 + kHid, oB3 = oW3 + kHid * kHid, oW4 = oB3 + kHid;
 constexpr int oB4 = oW4 + kOut * kHid, kNParam = oB4 + kOut;
 static_assert(kNParam == PHX_MLP_NPARAM, "parameter count");
+// compact copy of the biases (b1, b2, b3, b4), rebuilt with the packed weights
+constexpr int kB1 = 0, kB2 = kHid, kB3 = 2 * kHid, kB4 = 3 * kHid, kNBias = 3 * kHid + kOut;
 
 // fragment counts: forward A = W (M = out, K = in); backward A = W^T (M = in, K = out)
 constexpr int kFwdFrags = 4 * 1 + 4 * 4 + 4 * 4 + 1 * 4;  // L1, L2, L3, L4
@@ -63,7 +65,8 @@ constexpr int fwdBase[4] = {0, 4, 20, 36};
 struct MlpDev {
   const uint64_t *fwd;   // [2][kFwdFrags][64] packed bf16x4: hi parts, then lo parts
   const uint64_t *bwd;   // [kBwdFrags][64]
-  const float *param;    // [kNParam] fp32 master (biases read from here)
+  const float *bias;     // [kNBias] compact biases
+  const float *param;    // [kNParam] fp32 master
   float fmean[4], fstd[4];
   float width_scale, width_min;
 };
@@ -106,9 +109,9 @@ __device__ __forceinline__ Split split4(float a, float b, float c, float d) {
 }
 // split-BF16 product accumulate: W . a with W = (whi, wlo) fragments of one A tile
 __device__ __forceinline__ f32x4 mfma3(const MlpDev &m, int frag, int lane, const Split &a,
-                                      f32x4 acc) {
-  const bf16x4 wh = as_bf16x4(m.fwd[frag * 64 + lane]);
-  const bf16x4 wl = as_bf16x4(m.fwd[(kFwdFrags + frag) * 64 + lane]);
+                                      f32x4 acc, int ofs) {
+  const bf16x4 wh = as_bf16x4(m.fwd[ofs + frag * 64 + lane]);
+  const bf16x4 wl = as_bf16x4(m.fwd[ofs + (kFwdFrags + frag) * 64 + lane]);
   acc = mfma(wh, a.hi, acc);
   acc = mfma(wh, a.lo, acc);
   acc = mfma(wl, a.hi, acc);
@@ -117,9 +120,11 @@ __device__ __forceinline__ f32x4 mfma3(const MlpDev &m, int frag, int lane, cons
 
 // x: B fragment of the standardized features (lanes 0..15: features 0..3 of zone l; others
 // zero), as a hi/lo pair.  h1..h3 (optional): post-activation tiles, 4 neuron tiles each.
-// Returns the output tile (lanes 0..15 hold o1, o2 of zone l in elements 0, 1).
+// Returns the output tile (lanes 0..15 hold o1, o2 of zone l in elements 0, 1).  ofs is
+// added to every weight and bias index; it is 0 at run time (callers may make it opaque
+// to the compiler so that weight reads are not hoisted out of an enclosing loop).
 __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &x, f32x4 *h1,
-                                                  f32x4 *h2, f32x4 *h3) {
+                                                  f32x4 *h2, f32x4 *h3, int ofs = 0) {
   const int lane = threadIdx.x & 63;
   const int r0 = 4 * (lane >> 4);
   f32x4 a[4], bcur[4];
@@ -127,9 +132,9 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma3(m, fwdBase[0] + mt, lane, x, acc);
+    acc = mfma3(m, fwdBase[0] + mt, lane, x, acc, ofs);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.param[oB1 + mt * 16 + r0 + r], 0.f);
+    for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.bias[ofs + kB1 + mt * 16 + r0 + r], 0.f);
     a[mt] = acc;
     if (h1) h1[mt] = acc;
   }
@@ -137,7 +142,7 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &
 #pragma unroll
   for (int layer = 2; layer <= 3; ++layer) {
     const int base = fwdBase[layer - 1];
-    const int ob = layer == 2 ? oB2 : oB3;
+    const int ob = layer == 2 ? kB2 : kB3;
     Split bin[4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) bin[ks] = split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]);
@@ -145,9 +150,9 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &
     for (int mt = 0; mt < 4; ++mt) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) acc = mfma3(m, base + mt * 4 + ks, lane, bin[ks], acc);
+      for (int ks = 0; ks < 4; ++ks) acc = mfma3(m, base + mt * 4 + ks, lane, bin[ks], acc, ofs);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.param[ob + mt * 16 + r0 + r], 0.f);
+      for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.bias[ofs + ob + mt * 16 + r0 + r], 0.f);
       bcur[mt] = acc;
     }
 #pragma unroll
@@ -161,10 +166,10 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &
   f32x4 o = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks)
-    o = mfma3(m, fwdBase[3] + ks, lane, split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]), o);
+    o = mfma3(m, fwdBase[3] + ks, lane, split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]), o, ofs);
   if (r0 == 0) {
-    o[0] += m.param[oB4 + 0];
-    o[1] += m.param[oB4 + 1];
+    o[0] += m.bias[ofs + kB4 + 0];
+    o[1] += m.bias[ofs + kB4 + 1];
   }
   return o;
 }
@@ -191,14 +196,18 @@ __device__ __forceinline__ void mlp_predict_wave(const MlpDev &m, const float f[
                                                  float &o2) {
   const int lane = threadIdx.x & 63;
   float r1 = 0.f, r2 = 0.f;
-#pragma unroll
+  // one tile at a time: the weight fragments are re-read per tile (through an index
+  // offset the compiler cannot see is 0) rather than all 40 held in registers
+#pragma unroll 1
   for (int zt = 0; zt < 4; ++zt) {
+    int ofs = 0;
+    asm volatile("" : "+s"(ofs));
     // B fragment: lanes 0..15 carry the features of zone zt*16 + lane
     const int src = zt * 16 + (lane & 15);
     const float g0 = __shfl(f[0], src, 64), g1 = __shfl(f[1], src, 64);
     const float g2 = __shfl(f[2], src, 64), g3 = __shfl(f[3], src, 64);
     const Split x = lane < 16 ? split4(g0, g1, g2, g3) : split4(0.f, 0.f, 0.f, 0.f);
-    const f32x4 o = mlp_tile_forward(m, x, nullptr, nullptr, nullptr);
+    const f32x4 o = mlp_tile_forward(m, x, nullptr, nullptr, nullptr, ofs);
     const float t1 = __shfl(o[0], lane & 15, 64), t2 = __shfl(o[1], lane & 15, 64);
     if ((lane >> 4) == zt) {
       r1 = t1;
@@ -211,8 +220,15 @@ __device__ __forceinline__ void mlp_predict_wave(const MlpDev &m, const float f[
 
 // ---- kernels: packing, training, Adam, batch inference ------------------------------------
 __global__ void k_mlp_pack(const float *__restrict__ param, uint64_t *__restrict__ fwd,
-                           uint64_t *__restrict__ bwd) {
+                           uint64_t *__restrict__ bwd, float *__restrict__ bias) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (kFwdFrags + kBwdFrags) * 64) {
+    const int q = t - (kFwdFrags + kBwdFrags) * 64;
+    if (q < kNBias)
+      bias[q] = q < kB2 ? param[oB1 + q] : q < kB3 ? param[oB2 + q - kB2]
+                : q < kB4 ? param[oB3 + q - kB3] : param[oB4 + q - kB4];
+    return;
+  }
   const int lane = t & 63, frag = t >> 6;
   const int row = lane & 15, c0 = 4 * (lane >> 4);
   auto W = [&](int layer, int o, int i) -> float {
@@ -453,65 +469,102 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev m, long n,
 }
 
 // ---- con2prim with the learned bracket (batched states) --------------------------------
+// The packed forward weights (hi and lo, 40 KB) and the biases are copied into LDS once
+// per block of 256 states (an L2-resident read); every wavefront then evaluates the
+// network for its 64 states from LDS.  (A persistent grid that loads them once per CU
+// was tried: the loop makes the compiler keep the kernel arguments in scratch and hoist
+// loop invariants into ~270 VGPRs, which costs more than the per-block copy.)
+constexpr int kLearnedThreads = 256;
+
 template <bool MHD, int GEOM>
-__global__ __launch_bounds__(256) void k_con2prim_batch_learned(
+__global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
     DevParams p, MlpDev m, long n, const double *__restrict__ coords, double *__restrict__ cons,
     double *__restrict__ prim, double *__restrict__ mu, double *__restrict__ fail,
     int32_t *__restrict__ iters, int32_t *__restrict__ evals, int32_t *__restrict__ fallback) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
-  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool have = s < n;  // every lane stays for the wave-wide network evaluation
-  const long sc = have ? s : n - 1;
-  double X[3] = {0.0, 0.0, 0.0};
-  if (coords) {
-    X[0] = coords[sc];
-    X[1] = coords[n + sc];
-  }
-  const M g(p.a, X[0], X[1], X[2]);
-  double cs[L::ncons], pr[L::nprim], sig[3];
+  __shared__ uint64_t wfwd[2 * kFwdFrags * 64];
+  __shared__ float wbias[kNBias];
+  for (int q = threadIdx.x; q < 2 * kFwdFrags * 64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
+  for (int q = threadIdx.x; q < kNBias; q += kLearnedThreads) wbias[q] = m.bias[q];
+  __syncthreads();
+  MlpDev ml = m;
+  ml.fwd = wfwd;
+  ml.bias = wbias;
+  {
+    const long base = (long)blockIdx.x * kLearnedThreads;
+    const long s = base + threadIdx.x;
+    const bool have = s < n;  // every lane stays for the wave-wide network evaluation
+    const long sc = have ? s : n - 1;
+    // phase 1: features of this lane's state
+    float f[4] = {0.f, 0.f, 0.f, 0.f};
+    {
+      double X[3] = {0.0, 0.0, 0.0};
+      if (coords) {
+        X[0] = coords[sc];
+        X[1] = coords[n + sc];
+      }
+      const M g(p, X[0], X[1], X[2]);
+      double cs[L::ncons], pr[L::nprim];
 #pragma unroll
-  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + sc];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = 0.0;
-  C2PState z;
-  const bool ok = c2p_setup<MHD>(p, g, X, cs, pr, z);
-  float f[4] = {0.f, 0.f, 0.f, 0.f};
-  if (ok) zone_features(m, z.res, f);
-  float o1, o2;
-  mlp_predict_wave(m, f, o1, o2);
-  if (!have) return;
-  double st = 0.0, root = mu[s], tt = 0.0;
-  int it = 0, ev = 0, fb = 0;
-  if (ok) {
-    const double c = exp((double)o1) / (double)prior_z0(m, f[0], f[2]);
-    const double w = c * fmax((double)m.width_scale * exp((double)o2), (double)m.width_min);
-    const double lo = fmax(c - w, 0.0), hi = fmin(c + w, 1.0);
-    const double ylo = z.res(lo), yhi = z.res(hi);
-    ev = 2;
-    bool good = true;
-    if (ylo * yhi <= 0.0) {
-      root = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
-    } else {
-      fb = 1;
-      good = regula_falsi(z.res, 0.0, 1.0, p.tol, c, p.max_iter, root, it);
-      ev += 2;
-      if (!good) it = -1;
+      for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + sc];
+      C2PState z;
+      if (c2p_setup<MHD>(p, g, X, cs, pr, z)) zone_features(ml, z.res, f);
     }
-    if (good) {
-      ev += it;
-      st = c2p_unwrap<MHD>(p, g, z, root, cs, pr, tt, sig);
+    // phase 2: the network for the wave's 64 states
+    float o1, o2;
+    mlp_predict_wave(ml, f, o1, o2);
+    if (!have) return;
+    // phase 3: set-up again (bit for bit the same) and the bracketed solve.  Reading the
+    // inputs through pointers the compiler cannot relate to phase 1's keeps phase 1's
+    // set-up from staying live across the network.
+    const double *cin = cons, *xin = coords;
+    asm volatile("" : "+s"(cin), "+s"(xin));
+    double X[3] = {0.0, 0.0, 0.0};
+    if (xin) {
+      X[0] = xin[sc];
+      X[1] = xin[n + sc];
     }
+    const M g(p, X[0], X[1], X[2]);
+    double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+    for (int v = 0; v < L::ncons; ++v) cs[v] = cin[v * n + sc];
+#pragma unroll
+    for (int v = 0; v < L::nprim; ++v) pr[v] = 0.0;
+    C2PState z;
+    const bool ok = c2p_setup<MHD>(p, g, X, cs, pr, z);
+    double st = 0.0, root = mu[s], tt = 0.0;
+    int it = 0, ev = 0, fb = 0;
+    if (ok) {
+      const double c = exp((double)o1) / (double)prior_z0(ml, f[0], f[2]);
+      const double w = c * fmax((double)ml.width_scale * exp((double)o2), (double)ml.width_min);
+      const double lo = fmax(c - w, 0.0), hi = fmin(c + w, 1.0);
+      const double ylo = z.res(lo), yhi = z.res(hi);
+      ev = 2;
+      bool good = true;
+      if (ylo * yhi <= 0.0) {
+        root = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
+      } else {
+        fb = 1;
+        good = regula_falsi(z.res, 0.0, 1.0, p.tol, c, p.max_iter, root, it);
+        ev += 2;
+        if (!good) it = -1;
+      }
+      if (good) {
+        ev += it;
+        st = c2p_unwrap<MHD>(p, g, z, root, cs, pr, tt, sig);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) cons[v * n + s] = cs[v];
+#pragma unroll
+    for (int v = 0; v < L::nprim; ++v) prim[v * n + s] = pr[v];
+    mu[s] = root;
+    fail[s] = st;
+    if (iters) iters[s] = it;
+    if (evals) evals[s] = ev;
+    if (fallback) fallback[s] = fb;
   }
-#pragma unroll
-  for (int v = 0; v < 5; ++v) cons[v * n + s] = cs[v];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) prim[v * n + s] = pr[v];
-  mu[s] = root;
-  fail[s] = st;
-  if (iters) iters[s] = it;
-  if (evals) evals[s] = ev;
-  if (fallback) fallback[s] = fb;
 }
 
 // features of batched states (training data and tests)
@@ -529,7 +582,7 @@ __global__ __launch_bounds__(256) void k_c2p_features(DevParams p, MlpDev m, lon
     X[0] = coords[s];
     X[1] = coords[n + s];
   }
-  const M g(p.a, X[0], X[1], X[2]);
+  const M g(p, X[0], X[1], X[2]);
   double cs[L::ncons], pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + s];
@@ -548,6 +601,7 @@ struct phx_mlp {
   int device = 0;
   float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
   uint64_t *fwd = nullptr, *bwd = nullptr;
+  float *bias = nullptr;
   float fmean[4] = {0, 0, 0, 0}, fstd[4] = {1, 1, 1, 1};
   float width_scale = 4.0f, width_min = 1e-9f;
   long step = 0;
@@ -557,6 +611,7 @@ static MlpDev dev_view(const phx_mlp *m) {
   MlpDev d;
   d.fwd = m->fwd;
   d.bwd = m->bwd;
+  d.bias = m->bias;
   d.param = m->param;
   for (int i = 0; i < 4; ++i) {
     d.fmean[i] = m->fmean[i];
@@ -568,9 +623,9 @@ static MlpDev dev_view(const phx_mlp *m) {
 }
 
 static void repack(phx_mlp *m, hipStream_t s) {
-  const int threads = (kFwdFrags + kBwdFrags) * 64;
+  const int threads = (kFwdFrags + kBwdFrags) * 64 + kNBias;
   hipLaunchKernelGGL(k_mlp_pack, dim3((threads + 255) / 256), dim3(256), 0, s, m->param, m->fwd,
-                     m->bwd);
+                     m->bwd, m->bias);
 }
 
 extern "C" {
@@ -586,7 +641,8 @@ int phx_mlp_create(phx_ctx *ctx, phx_mlp **out, uint64_t seed) {
              hipMalloc(&m->m2, sizeof(float) * kNParam) == hipSuccess &&
              hipMalloc(&m->loss, sizeof(float)) == hipSuccess &&
              hipMalloc(&m->fwd, sizeof(uint64_t) * 2 * kFwdFrags * 64) == hipSuccess &&
-             hipMalloc(&m->bwd, sizeof(uint64_t) * kBwdFrags * 64) == hipSuccess;
+             hipMalloc(&m->bwd, sizeof(uint64_t) * kBwdFrags * 64) == hipSuccess &&
+             hipMalloc(&m->bias, sizeof(float) * kNBias) == hipSuccess;
   if (!okm) {
     phx_mlp_destroy(m);
     return PHX_ERR_NOMEM;
@@ -618,7 +674,7 @@ int phx_mlp_destroy(phx_mlp *m) {
   if (!m) return PHX_OK;
   (void)hipSetDevice(m->device);
   for (void *p : {(void *)m->param, (void *)m->grad, (void *)m->m1, (void *)m->m2,
-                  (void *)m->loss, (void *)m->fwd, (void *)m->bwd})
+                  (void *)m->loss, (void *)m->fwd, (void *)m->bwd, (void *)m->bias})
     if (p) (void)hipFree(p);
   delete m;
   return PHX_OK;
@@ -730,13 +786,11 @@ int phx_con2prim_features(phx_ctx *ctx, const phx_fluid_params *p, const phx_mlp
   hipStream_t s = ctx->stream_of(stream);
   const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
   const MlpDev md = dev_view(m);
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_c2p_features<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
-    else hipLaunchKernelGGL((k_c2p_features<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_c2p_features<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
-    else hipLaunchKernelGGL((k_c2p_features<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_c2p_features<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, dp, md, (long)n, coords, cons, features);
+    });
+  });
   return ctx->check_launch();
 }
 
@@ -752,16 +806,14 @@ int phx_con2prim_batch_learned(phx_ctx *ctx, const phx_fluid_params *p, const ph
     return ctx->fail(PHX_ERR_INVALID, "null field");
   const DevParams dp = make_dev_params(p);
   hipStream_t s = ctx->stream_of(stream);
-  const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+  const dim3 grid((unsigned)((n + kLearnedThreads - 1) / kLearnedThreads)), blk(kLearnedThreads);
   const MlpDev md = dev_view(m);
 #define LARGS dp, md, (long)n, coords, cons, prim, mu, fail, iters, evals, fallback
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch_learned<true, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, LARGS);
-    else hipLaunchKernelGGL((k_con2prim_batch_learned<false, PHX_GEOM_MINKOWSKI>), grid, blk, 0, s, LARGS);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_con2prim_batch_learned<true, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, LARGS);
-    else hipLaunchKernelGGL((k_con2prim_batch_learned<false, PHX_GEOM_SPHERICAL_KS>), grid, blk, 0, s, LARGS);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_con2prim_batch_learned<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, LARGS);
+    });
+  });
 #undef LARGS
   return ctx->check_launch();
 }

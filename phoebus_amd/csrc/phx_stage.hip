@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_update_con2prim(
     const bool nan_in = isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0;
     if (!nan_in) {
       const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-      const M g(p.a, X[0], X[1], X[2]);
+      const M g(p, X[0], X[1], X[2]);
 #pragma unroll
       for (int v = 0; v < L::nprim; ++v) pr[v] = prim_out[v * n + c];
       double tt = temp ? temp[c] : 0.0, m = mu[c];
@@ -121,7 +121,7 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
   for (int d = 0; d < b->ndim; ++d) {
     ctx->mark_begin(kFluxName[d], s);
     launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
-                  f->face_sig, s);
+                  f->face_sig, s, ctx->tile_for(p, b, d));
     ctx->mark_end(s);
   }
   if (p->mhd && b->ndim > 1) {
@@ -146,13 +146,11 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
   ctx->mark_begin("update_con2prim", s);
 #define ARGS dp, bg, f->flux, src, f->cons, a->u_base, a->beta, bdt, a->u_out, a->prim_out, \
              f->temp, f->c2p_mu, f->cell_sig, f->face_sig, f->fail, a->iters, partial
-  if (p->geom.kind == PHX_GEOM_MINKOWSKI) {
-    if (p->mhd) hipLaunchKernelGGL((k_update_con2prim<true, PHX_GEOM_MINKOWSKI>), dim3(nb), dim3(256), 0, s, ARGS);
-    else hipLaunchKernelGGL((k_update_con2prim<false, PHX_GEOM_MINKOWSKI>), dim3(nb), dim3(256), 0, s, ARGS);
-  } else {
-    if (p->mhd) hipLaunchKernelGGL((k_update_con2prim<true, PHX_GEOM_SPHERICAL_KS>), dim3(nb), dim3(256), 0, s, ARGS);
-    else hipLaunchKernelGGL((k_update_con2prim<false, PHX_GEOM_SPHERICAL_KS>), dim3(nb), dim3(256), 0, s, ARGS);
-  }
+  with_geom(p->geom.kind, [&](auto G) {
+    with_mhd(p->mhd, [&](auto H) {
+      hipLaunchKernelGGL((k_update_con2prim<decltype(H)::value, decltype(G)::value>), dim3(nb), dim3(256), 0, s, ARGS);
+    });
+  });
 #undef ARGS
   ctx->mark_end(s);
   if (a->dt_out) {

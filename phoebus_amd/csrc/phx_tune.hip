@@ -1,0 +1,103 @@
+// phx_tune.hip -- autotuner of the y/z flux tile shape (include/phx.h, phx_autotune).
+//
+// The y/z flux kernels stage ti x ts cells of every primitive in LDS.  Which shape is
+// fastest depends on the block extent (halo overhead 2H+1 of ts rows per tile), the
+// reconstruction order (H, register pressure), the variable count (LDS per block) and on
+// cache-line use along i.  The tuner times every admissible shape on the caller's own
+// fields with HIP events and keeps the fastest per (recon, riemann, mhd, geometry, block
+// size, direction) in the context; later flux and stage calls with the same key use it.
+// Results are bit-identical for every shape, so tuning changes time only.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "phx.h"
+#include "phx_internal.hpp"
+
+using namespace phx;
+
+extern "C" {
+
+int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                 const phx_block_fields *f, int reps, phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = check_block(ctx, b, p->recon))) return rc;
+  if (!f || !f->prim || !f->flux || !f->face_sig) return ctx->fail(PHX_ERR_INVALID, "null field");
+  if (reps < 1) reps = 3;
+  const DevParams dp = make_dev_params(p);
+  const BlockGeom bg = make_block_geom(b);
+  hipStream_t s = ctx->stream_of(stream);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "event create failed");
+  static const int kTi[] = {1, 2, 4, 8, 16, 32};
+  static const int kTs[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
+  for (int d = 1; d < b->ndim; ++d) {
+    std::vector<TileShape> cand;
+    for (int ti : kTi)
+      for (int ts : kTs) {
+        const TileShape want{ti, ts};
+        const TileShape r = resolve_tile_shape(d, p->recon, p->mhd, bg, want);
+        if (r.ti != ti || r.ts != ts) continue;  // not admissible as asked
+        bool dup = false;
+        for (const auto &c : cand) dup = dup || (c.ti == r.ti && c.ts == r.ts);
+        if (!dup) cand.push_back(r);
+      }
+    // the full own extent (one tile per column) is always a candidate
+    const TileShape full = resolve_tile_shape(d, p->recon, p->mhd, bg, TileShape{0, 1 << 20});
+    for (int ti : kTi) {
+      const TileShape r = resolve_tile_shape(d, p->recon, p->mhd, bg, TileShape{ti, full.ts});
+      bool dup = false;
+      for (const auto &c : cand) dup = dup || (c.ti == r.ti && c.ts == r.ts);
+      if (!dup) cand.push_back(r);
+    }
+    float best = 1e30f;
+    TileShape best_shape;
+    for (const auto &c : cand) {
+      launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
+                    f->face_sig, s, c);  // warm-up
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps; ++r)
+        launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim,
+                      f->flux, f->face_sig, s, c);
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const int lrc = ctx->check_launch();
+        return lrc ? lrc : ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_shape = c;
+      }
+    }
+    ctx->tiles[phx_ctx::tile_key(p, b, d)] = best_shape;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ctx->check_launch();
+}
+
+int phx_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, int dir,
+                   int *ti, int *ts) {
+  if (!ctx || !p || !b || !ti || !ts || dir < 1 || dir > 2) return PHX_ERR_INVALID;
+  const TileShape r =
+      resolve_tile_shape(dir, p->recon, p->mhd, make_block_geom(b), ctx->tile_for(p, b, dir));
+  *ti = r.ti;
+  *ts = r.ts;
+  return PHX_OK;
+}
+
+int phx_set_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, int dir,
+                       int ti, int ts) {
+  if (!ctx || !p || !b || dir < 1 || dir > 2 || ti < 0 || ts < 0) return PHX_ERR_INVALID;
+  ctx->tiles[phx_ctx::tile_key(p, b, dir)] = TileShape{ti, ts};
+  return PHX_OK;
+}
+
+}  // extern "C"

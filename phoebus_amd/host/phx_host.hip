@@ -156,8 +156,8 @@ __global__ void k_bc(BlockGeom b, DevParams p, int d, int side, int periodic, in
   const long cr = ((long)ref[2] * b.nj + ref[1]) * b.ni + ref[0];
   double gratio = 1.0;
   if (!periodic) {
-    const M g(p.a, xc(b, 0, idx[0]), xc(b, 1, idx[1]), xc(b, 2, idx[2]));
-    const M gr(p.a, xc(b, 0, ref[0]), xc(b, 1, ref[1]), xc(b, 2, ref[2]));
+    const M g(p, xc(b, 0, idx[0]), xc(b, 1, idx[1]), xc(b, 2, idx[2]));
+    const M gr(p, xc(b, 0, ref[0]), xc(b, 1, ref[1]), xc(b, 2, ref[2]));
     gratio = ratio(g.detgamma(), gr.detgamma());
   }
   for (int l = 0; l < nv; ++l) q[l * b.ncell + c] = gratio * q[l * b.ncell + cr];
@@ -174,12 +174,10 @@ TaskStatus ApplyBoundaryConditions(MeshBlockData *rc, int container, const Kind 
     for (int side = 0; side < 2; ++side) {
       const int periodic = kinds[2 * d + side] == Kind::periodic;
       const unsigned blocks = (unsigned)((total + 255) / 256);
-      if (rc->params().geom.kind == PHX_GEOM_MINKOWSKI)
-        hipLaunchKernelGGL(k_bc<PHX_GEOM_MINKOWSKI>, dim3(blocks), dim3(256), 0, s, bg, dp, d, side,
-                           periodic, rc->ncons(), rc->cons(container));
-      else
-        hipLaunchKernelGGL(k_bc<PHX_GEOM_SPHERICAL_KS>, dim3(blocks), dim3(256), 0, s, bg, dp, d,
+      with_geom(rc->params().geom.kind, [&](auto G) {
+        hipLaunchKernelGGL((k_bc<decltype(G)::value>), dim3(blocks), dim3(256), 0, s, bg, dp, d,
                            side, periodic, rc->ncons(), rc->cons(container));
+      });
     }
   }
   return hipGetLastError() == hipSuccess ? TaskStatus::complete : TaskStatus::fail;

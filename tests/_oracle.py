@@ -16,14 +16,26 @@ LIBPATH = os.path.join(ORACLE_DIR, "_build", "libphx_oracle.so")
 
 DBL_MAX = np.finfo(np.float64).max
 
-GEOM_MINKOWSKI, GEOM_SPHKS = 0, 1
+GEOM_MINKOWSKI, GEOM_SPHKS, GEOM_FMKS = 0, 1, 2
 RECON_CONSTANT, RECON_LINEAR, RECON_WENO5, RECON_MP5 = 0, 1, 2, 3
 RIEMANN_LLF, RIEMANN_HLL = 0, 1
 FLOOR_CONSTANT, FLOOR_EXPX1_RHO_SIE, FLOOR_EXPX1_RHO_U, FLOOR_X1_RHO_SIE, FLOOR_R_RHO_SIE = range(5)
 
 
+class Mgr(C.Structure):
+    _fields_ = [("derefine_poles", C.c_int), ("h", C.c_double), ("xt", C.c_double),
+                ("alpha", C.c_double), ("x0", C.c_double), ("smooth", C.c_double),
+                ("hexp_br", C.c_double), ("hexp_nsq", C.c_double), ("hexp_csq", C.c_double)]
+
+
+def mgr(derefine_poles=True, h=0.3, xt=0.82, alpha=14.0, x0=0.0, smooth=0.5, hexp_br=1e30,
+        hexp_nsq=4.0, hexp_csq=4.0):
+    """McKinney-Gammie-Ryan parameters (fmks.cpp:37-52 defaults)."""
+    return Mgr(int(derefine_poles), h, xt, alpha, x0, smooth, hexp_br, hexp_nsq, hexp_csq)
+
+
 class Geom(C.Structure):
-    _fields_ = [("kind", C.c_int), ("a", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("a", C.c_double), ("mgr", Mgr)]
 
 
 class Bounds(C.Structure):
@@ -100,11 +112,11 @@ def ptr(a):
 
 
 def default_params(gamma=5.0 / 3.0, mhd=True, recon=RECON_WENO5, riemann=RIEMANN_HLL,
-                   geom=GEOM_MINKOWSKI, a=0.0, tol=1e-10, max_iter=100):
+                   geom=GEOM_MINKOWSKI, a=0.0, tol=1e-10, max_iter=100, mgr_params=None):
     """Floors and ceilings disabled (fixup.hpp default Floors()/Ceilings(); EOS minima 0)."""
     b = Bounds(FLOOR_CONSTANT, -DBL_MAX, -DBL_MAX, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX)
     return Params(gamma - 1.0, 1.0, int(mhd), recon, riemann, tol, max_iter, 1.0, 0, 0, b,
-                  Geom(geom, a))
+                  Geom(geom, a, mgr_params if mgr_params is not None else mgr()))
 
 
 def nprim(mhd):
@@ -416,3 +428,12 @@ class Driver:
         while n < max_steps and self.time < self.tlim:
             self.step()
             n += 1
+
+
+def mgr_transform(m, X):
+    Cc = (C.c_double * 3)()
+    Jcov = ((C.c_double * 3) * 3)()
+    Jcon = ((C.c_double * 3) * 3)()
+    lib().orc_mgr_transform(C.byref(m), C.c_double(X[0]), C.c_double(X[1]), C.c_double(X[2]),
+                            Cc, Jcov, Jcon)
+    return np.array(Cc[:]), np.array([r[:] for r in Jcov]), np.array([r[:] for r in Jcon])

@@ -51,7 +51,8 @@ def test_library_is_gfx950_code_object():
 
 def test_struct_layouts_match_header():
     # sizes implied by include/phx.h on LP64
-    assert C.sizeof(A.Geometry) == 16
+    assert C.sizeof(A.Mgr) == 8 + 8 * 8  # int + pad, 8 doubles
+    assert C.sizeof(A.Geometry) == 16 + C.sizeof(A.Mgr)
     assert C.sizeof(A.Bounds) == 72
     assert C.sizeof(A.Block) == 4 + 12 + 4 + 4 + 48  # int, int[3], int, pad, double[6]
     assert C.sizeof(A.BlockFields) == 80

@@ -13,7 +13,7 @@ import pytest
 import _oracle as O
 import _physics as PH
 from phoebus_amd import _abi as A
-from phoebus_amd.api import BlockState, Phx, block, fluid_params
+from phoebus_amd.api import BlockState, Phx, block, fluid_params, mgr
 from phoebus_amd.device import DeviceArray, synchronize
 
 pytestmark = pytest.mark.gpu
@@ -45,6 +45,9 @@ def oparams(p):
     b = p.bounds
     op.bounds = O.Bounds(b.floor_type, b.rho0, b.sie0, b.rho_exp, b.sie_exp, b.rho_min_eos,
                          b.sie_min_eos, b.gam_max, b.sie_max)
+    m = p.geom.mgr
+    op.geom.mgr = O.Mgr(m.derefine_poles, m.h, m.xt, m.alpha, m.x0, m.smooth, m.hexp_br,
+                        m.hexp_nsq, m.hexp_csq)
     return op
 
 
@@ -70,34 +73,54 @@ def assert_match(gpu, ref, exact, what, rtol=RTOL_CURVED, atol_scale=None):
         assert np.all(err <= lim), f"{what}: max rel {relerr(gpu, ref)}"
 
 
+# FMKS test transformation: fmks.cpp defaults with the smooth region starting at X1 = 0.2
+FMKS_MGR = dict(x0=0.2)
+
+
+def geom_params(geom, **kw):
+    return fluid_params(geom=geom, a=0.9375, mgr_params=mgr(**FMKS_MGR), **kw)
+
+
+def geom_block(ndim, nx, geom):
+    if geom == A.GEOM_SPHERICAL_KS:
+        return block(ndim, nx, 4, xmin=(4.0, 0.6, 0.0), dx=(0.25, 0.05, 0.05))
+    if geom == A.GEOM_FMKS:  # X = (log r, x2, phi)
+        return block(ndim, nx, 4, xmin=(1.4, 0.3, 0.0), dx=(0.05, 0.02, 0.05))
+    return block(ndim, nx, 4, xmin=(0.0, 0.0, 0.0), dx=(0.1, 0.2, 0.3))
+
+
 # ---------------------------------------------------------------- batched states -------
-def ks_states(rng, n, mhd, gamma):
-    """States in spherical KS at r ~ U(1.5, 50), theta ~ acos U(-1,1) (away from the axis)."""
+def ks_states(rng, n, mhd, gamma, geom=A.GEOM_SPHERICAL_KS):
+    """States in spherical KS at r ~ U(1.5, 50), theta ~ acos U(-1,1) (away from the axis),
+    or FMKS at X1 = log r, X2 ~ U(0.05, 0.95)."""
     r = rng.uniform(1.5, 50.0, n)
-    th = np.arccos(rng.uniform(-0.98, 0.98, n))
-    coords = np.ascontiguousarray(np.stack([r, th]))
+    if geom == A.GEOM_FMKS:
+        coords = np.ascontiguousarray(np.stack([np.log(r), rng.uniform(0.05, 0.95, n)]))
+    else:
+        coords = np.ascontiguousarray(np.stack([r, np.arccos(rng.uniform(-0.98, 0.98, n))]))
     prims = PH.random_prims(rng, n, mhd, gamma, wmax=5.0)
+    op = oparams(geom_params(geom))
     # scale vector components into the coordinate basis at each point
     for s in range(n):
-        g = O.point_geom(O.default_params(geom=O.GEOM_SPHKS, a=0.9375), (r[s], th[s], 0.0))
+        g = O.point_geom(op, (coords[0, s], coords[1, s], 0.0))
         prims[:, s] = PH.to_coordinate_frame(prims[:, s:s + 1], g.np("gam"), mhd)[:, 0]
     return coords, np.ascontiguousarray(prims)
 
 
-@pytest.mark.parametrize("geom", [A.GEOM_MINKOWSKI, A.GEOM_SPHERICAL_KS])
+@pytest.mark.parametrize("geom", [A.GEOM_MINKOWSKI, A.GEOM_SPHERICAL_KS, A.GEOM_FMKS])
 @pytest.mark.parametrize("mhd", [False, True])
 def test_batch_p2c_c2p_parity(phx, geom, mhd):
     say(f"[gpu] batch p2c/c2p geom={geom} mhd={mhd}")
     rng = np.random.default_rng(100 + geom * 2 + mhd)
     n = 4096
     gamma = 4.0 / 3.0
-    p = fluid_params(gamma=gamma, mhd=mhd, geom=geom, a=0.9375, c2p_tol=1e-10)
+    p = geom_params(geom, gamma=gamma, mhd=mhd, c2p_tol=1e-10)
     op = oparams(p)
     if geom == A.GEOM_MINKOWSKI:
         coords = np.zeros((2, n))
         prims = np.ascontiguousarray(PH.random_prims(rng, n, mhd, gamma, wmax=5.0))
     else:
-        coords, prims = ks_states(rng, n, mhd, gamma)
+        coords, prims = ks_states(rng, n, mhd, gamma, geom)
     exact = geom == A.GEOM_MINKOWSKI
     # prim -> cons
     ref_cons = O.p2c_batch(op, coords, prims)
@@ -150,7 +173,7 @@ def random_block(rng, b, mhd, gamma, geom):
     sh = O.block_shape(oblock(b))
     n = int(np.prod(sh))
     pr = PH.random_prims(rng, n, mhd, gamma, wmax=3.0)
-    if geom == A.GEOM_SPHERICAL_KS:
+    if geom != A.GEOM_MINKOWSKI:
         pr[1:4] *= 0.2  # keep curvilinear components moderate
         if mhd:
             pr[5:8] *= 0.2
@@ -177,6 +200,8 @@ CASES = [
     (3, (10, 8, 6), True, A.RECON_MP5, A.RIEMANN_HLL, A.GEOM_MINKOWSKI),
     (3, (10, 8, 6), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_SPHERICAL_KS),
     (2, (12, 10), False, A.RECON_LINEAR, A.RIEMANN_LLF, A.GEOM_SPHERICAL_KS),
+    (3, (10, 8, 6), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_FMKS),
+    (2, (12, 10), False, A.RECON_MP5, A.RIEMANN_LLF, A.GEOM_FMKS),
 ]
 
 
@@ -190,12 +215,9 @@ def test_block_stage_parity(phx, case):
     say(f"[gpu] block stage {case_id(case)}")
     rng = np.random.default_rng(hash(case_id(case)) % 2 ** 32)
     gamma = 13.0 / 9.0
-    if geom == A.GEOM_SPHERICAL_KS:
-        b = block(ndim, nx, 4, xmin=(4.0, 0.6, 0.0), dx=(0.25, 0.05, 0.05))
-    else:
-        b = block(ndim, nx, 4, xmin=(0.0, 0.0, 0.0), dx=(0.1, 0.2, 0.3))
-    p = fluid_params(gamma=gamma, mhd=mhd, recon=recon, riemann=riemann, geom=geom, a=0.9375,
-                     c2p_tol=1e-10, c2p_max_iter=100)
+    b = geom_block(ndim, nx, geom)
+    p = geom_params(geom, gamma=gamma, mhd=mhd, recon=recon, riemann=riemann, c2p_tol=1e-10,
+                    c2p_max_iter=100)
     op, ob = oparams(p), oblock(b)
     exact = geom == A.GEOM_MINKOWSKI
     prim = random_block(rng, b, mhd, gamma, geom)
@@ -242,8 +264,11 @@ def test_block_stage_parity(phx, case):
         synchronize()
         I = O.interior(ob)
         gsrc = st.src.get()
+        # FMKS derivatives are finite differences with step ~6e-6: ulp-level metric
+        # differences grow by 1/step
+        rtol_src = 1e-8 if geom == A.GEOM_FMKS else 1e-11
         for l in range(1, 5):
-            assert_match(gsrc[l][I], src_ref[l][I], False, f"src {l}", rtol=1e-11,
+            assert_match(gsrc[l][I], src_ref[l][I], False, f"src {l}", rtol=rtol_src,
                          atol_scale=np.max(np.abs(src_ref[1:5][(slice(None),) + I])))
         st.src.upload(src_ref)
         src = src_ref
@@ -346,6 +371,7 @@ STAGE_CASES = [
     (2, (24, 20), False, A.RECON_LINEAR, A.RIEMANN_HLL, A.GEOM_MINKOWSKI, 0.5),
     (1, (96,), False, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_MINKOWSKI, 1.0),
     (3, (10, 8, 6), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_SPHERICAL_KS, 0.5),
+    (3, (10, 8, 6), True, A.RECON_WENO5Z, A.RIEMANN_HLL, A.GEOM_FMKS, 0.5),
 ]
 
 
@@ -356,11 +382,8 @@ def test_fused_stage_matches_oracle_sequence(phx, case):
     say(f"[gpu] fused stage {case_id(case)} beta={beta}")
     rng = np.random.default_rng(7 + ndim + 10 * recon)
     gamma = 13.0 / 9.0
-    if geom == A.GEOM_SPHERICAL_KS:
-        b = block(ndim, nx, 4, xmin=(4.0, 0.6, 0.0), dx=(0.25, 0.05, 0.05))
-    else:
-        b = block(ndim, nx, 4, dx=(0.1, 0.2, 0.3))
-    p = fluid_params(gamma=gamma, mhd=mhd, recon=recon, riemann=riemann, geom=geom, a=0.9375)
+    b = geom_block(ndim, nx, geom)
+    p = geom_params(geom, gamma=gamma, mhd=mhd, recon=recon, riemann=riemann)
     op, ob = oparams(p), oblock(b)
     exact = geom == A.GEOM_MINKOWSKI
     prim = random_block(rng, b, mhd, gamma, geom)
@@ -404,3 +427,47 @@ def test_fused_stage_matches_oracle_sequence(phx, case):
                          atol_scale=np.max(np.abs(out_ref[m][I])))
         assert np.max(np.abs(st.c2p_mu.get()[I] - mu_ref[I])) <= 4 * p.c2p_tol
         assert abs(d_dt.get()[0] - dt_ref) <= 1e-8 * dt_ref
+
+
+def test_tile_shapes_and_autotune_do_not_change_fluxes(phx):
+    # every admissible y/z tile shape (and the tuned one) gives the same bits
+    say("[gpu] flux tile shapes + autotune vs oracle")
+    rng = np.random.default_rng(77)
+    b = block(3, (24, 20, 18), 4, dx=(0.1, 0.2, 0.3))
+    p = fluid_params(gamma=13.0 / 9.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL)
+    op, ob = oparams(p), oblock(b)
+    prim = random_block(rng, b, True, 13.0 / 9.0, A.GEOM_MINKOWSKI)
+    st = BlockState(p, b)
+    st.prim.upload(prim)
+    f = st.fields()
+    ref_flux, ref_fsig = O.calculate_fluxes(op, ob, prim)
+    first = None
+    shapes = [(1, 6), (2, 16), (3, 24), (8, 64), (16, 32), (32, 16), (5, 29), (0, 0)]
+    for ti, ts in shapes:
+        for d in (1, 2):
+            phx.set_tile_shape(p, b, d, ti, ts)
+        st.flux.zero()
+        st.face_sig.zero()
+        phx.calculate_fluxes(p, b, f)
+        synchronize()
+        gflux, gfsig = st.flux.get(), st.face_sig.get()
+        if first is None:
+            first = (gflux, gfsig)
+        np.testing.assert_array_equal(gflux, first[0], err_msg=f"shape {(ti, ts)}")
+        np.testing.assert_array_equal(gfsig, first[1], err_msg=f"shape {(ti, ts)}")
+    for d in range(3):
+        sl = consumed_face_slices(b, d)
+        np.testing.assert_array_equal(first[0][d][:, sl[0], sl[1], sl[2]],
+                                      ref_flux[d][:, sl[0], sl[1], sl[2]])
+        np.testing.assert_array_equal(first[1][d][sl], ref_fsig[d][sl])
+    phx.set_tile_shape(p, b, 1, 0, 0)
+    phx.set_tile_shape(p, b, 2, 0, 0)
+    phx.autotune(p, b, f, reps=1)
+    tuned = [phx.tile_shape(p, b, d) for d in (1, 2)]
+    say(f"    tuned shapes {tuned}")
+    st.flux.zero()
+    st.face_sig.zero()
+    phx.calculate_fluxes(p, b, f)
+    synchronize()
+    np.testing.assert_array_equal(st.flux.get(), first[0])
+    np.testing.assert_array_equal(st.face_sig.get(), first[1])

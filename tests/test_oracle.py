@@ -512,3 +512,110 @@ def test_sources_zero_in_minkowski_and_match_identity_in_ks():
         # T^{mu nu} from tmunu.hpp agrees with the u/b construction
         Tref = PH.tmunu(g, O.geom_gcon4(pk, X), prims[:, s], True)
         assert np.max(np.abs(T - Tref)) < 1e-12 * np.max(np.abs(Tref))
+
+
+# ---------------------------------------------------------------- FMKS geometry -------
+def _rel(a, b):
+    # GetDifference of tst/unit/geometry/test_geometry.cpp:59-61
+    return 2 * abs(b - a) / (abs(a) + abs(b)) if (a or b) else 0.0
+
+
+def _mgr_from(d):
+    return O.mgr(**d["params"])
+
+
+def test_mgr_transformation_reference_known_answers():
+    ka = json.load(open(os.path.join(GOLDEN, "reference_known_answers.json")))["mgr"]
+    m = _mgr_from(ka)
+    for X, Cr, Jc, Jn, tol in ((ka["X"], ka["C"], ka["Jcov"], ka["Jcon"], ka["eps"]),
+                               (ka["Xfar"], ka["C_far"], ka["Jcov_far"], ka["Jcon_far"], 1e-12)):
+        Cc, Jcov, Jcon = O.mgr_transform(m, X)
+        for i in range(3):
+            assert _rel(Cc[i], Cr[i]) < tol, (i, Cc[i], Cr[i])
+        for key, v in Jc.items():
+            assert _rel(Jcov[int(key[0]), int(key[1])], v) < max(tol, 1e-8), ("Jcov", key)
+        for key, v in Jn.items():
+            assert _rel(Jcon[int(key[0]), int(key[1])], v) < max(tol, 1e-8), ("Jcon", key)
+        for (i, j) in ((0, 1), (0, 2), (1, 2), (2, 0), (2, 1)):
+            assert Jcov[i, j] == 0 and Jcon[i, j] == 0
+        np.testing.assert_allclose(Jcov @ Jcon, np.eye(3), atol=1e-12)
+
+
+def test_mgr_jacobian_matches_finite_differences():
+    m = O.mgr(x0=-0.3)
+    X = (1.7, 0.31, 0.4)
+    Cc, Jcov, _ = O.mgr_transform(m, X)
+    h = 1e-6
+    for j in range(3):
+        Xp, Xm = list(X), list(X)
+        Xp[j] += h
+        Xm[j] -= h
+        fd = (O.mgr_transform(m, Xp)[0] - O.mgr_transform(m, Xm)[0]) / (2 * h)
+        np.testing.assert_allclose(fd, Jcov[:, j], rtol=1e-7, atol=1e-9)
+
+
+def _fmks_params(ka):
+    return O.default_params(geom=O.GEOM_FMKS, a=ka["a"], mgr_params=_mgr_from(ka))
+
+
+def test_fmks_metric_reference_known_answers():
+    ka = json.load(open(os.path.join(GOLDEN, "reference_known_answers.json")))["fmks"]
+    p = _fmks_params(ka)
+    X = ka["X"]
+    g = O.point_geom(p, X)
+    eps = ka["eps"]
+    assert _rel(g.alpha, ka["alpha"]) < eps
+    assert _rel(g.detg, ka["gdet"]) < eps
+    G = g.np("gcov")
+    for key, v in ka["g"].items():
+        i, j = int(key[0]), int(key[1])
+        if v == 0.0:
+            assert abs(G[i, j]) < eps and abs(G[j, i]) < eps
+        else:
+            assert _rel(G[i, j], v) < eps and _rel(G[j, i], v) < eps, (key, G[i, j], v)
+    gi = O.geom_gcon4(p, X)
+    for key, v in ka["ginv"].items():
+        i, j = int(key[0]), int(key[1])
+        if v == 0.0:
+            assert abs(gi[i, j]) < eps
+        else:
+            assert _rel(gi[i, j], v) < eps, (key, gi[i, j], v)
+    np.testing.assert_allclose(gi @ G, np.eye(4), atol=1e-10)
+    # the spatial parts: gamma_ij = g_ij, gamma^ij from g^{mu nu}: gamma^ij = g^ij + beta^i beta^j / alpha^2
+    np.testing.assert_allclose(g.np("gam"), G[1:, 1:], rtol=1e-12, atol=1e-12)
+    beta = g.np("beta")
+    np.testing.assert_allclose(g.np("gcon"), gi[1:, 1:] + np.outer(beta, beta) / g.alpha ** 2,
+                               rtol=1e-10, atol=1e-12)
+    assert _rel(g.detgamma * g.alpha, g.detg) < 1e-12
+
+
+def test_fmks_connection_reference_known_answers():
+    ka = json.load(open(os.path.join(GOLDEN, "reference_known_answers.json")))["fmks"]
+    p = _fmks_params(ka)
+    X = ka["X"]
+    dg = O.geom_dg(p, X)
+    # geometry_utils.hpp:72-78: Gamma_{abc} = (dg[b][a][c] + dg[c][a][b] - dg[b][c][a]) / 2
+    lo = 0.5 * (np.einsum("bac->abc", dg) + np.einsum("cab->abc", dg) - np.einsum("bca->abc", dg))
+    up = np.einsum("sr,rmn->smn", O.geom_gcon4(p, X), lo)
+    ref = np.array(ka["Gamma_up"])
+    for idx in np.ndindex(4, 4, 4):
+        if abs(ref[idx]) > 0.1:
+            assert _rel(up[idx], ref[idx]) < ka["gamma_tol"], (idx, up[idx], ref[idx])
+        else:
+            assert abs(up[idx]) < ka["gamma_tol"], (idx, up[idx], ref[idx])
+
+
+def test_fmks_gradlnalpha_matches_metric():
+    # d ln alpha by the reference's finite differences against ln(alpha) differenced here
+    ka = json.load(open(os.path.join(GOLDEN, "reference_known_answers.json")))["fmks"]
+    p = _fmks_params(ka)
+    X = np.array(ka["X"])
+    da = O.geom_gradlnalpha(p, X)
+    for d in range(3):
+        h = 1e-5
+        Xp, Xm = X.copy(), X.copy()
+        Xp[d] += h
+        Xm[d] -= h
+        fd = (np.log(O.point_geom(p, Xp).alpha) - np.log(O.point_geom(p, Xm).alpha)) / (2 * h)
+        assert abs(da[d + 1] - fd) < 1e-7 * max(1.0, abs(fd))
+    assert da[0] == 0.0
