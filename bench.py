@@ -332,6 +332,13 @@ def main():
     kb = kernel_bytes(nx)
     dom = max(launch_ms, key=lambda k: launch_ms[k])
     achieved = kb[dom] / (launch_ms[dom] * 1e-3) / 1e9
+    # measured HBM bytes of that launch: rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+    # workload, scaled by the same counters on a copy of known size
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_traffic.json")
+    if nx == 128 and os.path.exists(tfile):
+        tk = json.load(open(tfile))["kernels"].get(dom)
+        traffic = tk["traffic_bytes"] if tk else None
     warm_iters = iters.get()[I]
     cons_h, mu0_h = st.cons.get(), mu0.get()
     value = nx ** 3 / (ms_per_step * 1e-3)
@@ -355,7 +362,7 @@ def main():
                    "flux_tiles_yz": tiles},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": kb[dom],
+                     "traffic": traffic, "bytes_per_launch": kb[dom],
                      "avg_launch_ms": launch_ms[dom]},
         "stage_roofline": {"bytes_per_zone_update": B_ALG_STAGE,
                            "achieved_GBs": value * B_ALG_STAGE / 1e9,

@@ -52,21 +52,30 @@ __device__ __forceinline__ void face_out(const DevParams &p, int dir, const Bloc
 // and the cell's upper edge is published in LDS for the next lane.
 constexpr int XB = 256;
 
+// number of XB-lane groups the x sweep of block b takes
+__host__ __device__ inline long flux_x_groups(const BlockGeom &b, int H) {
+  const int dj = b.ndim > 1, dk = b.ndim > 2;
+  const long total = (long)b.ni * (b.ie[1] - b.is[1] + 1 + 2 * dj) * (b.ie[2] - b.is[2] + 1 + 2 * dk);
+  const long stride = XB - 2 * H - 1;
+  return (total - 2 * H - 1 + stride - 1) / stride;
+}
+
+// the x sweep for logical group bx of block b; stage = LDS [NP][XB]
 template <int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
-                                                   const double *__restrict__ prim,
-                                                   double *__restrict__ flux,
-                                                   double *__restrict__ fsig) {
+__device__ __forceinline__ void flux_x_body(const DevParams &p, const BlockGeom &b, long bx,
+                                            const double *__restrict__ prim,
+                                            double *__restrict__ flux,
+                                            double *__restrict__ fsig,
+                                            double (*stage)[XB]) {
   using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
-  __shared__ double stage[NP][XB];  // primitives, then upper-edge states
   const int dj = b.ndim > 1, dk = b.ndim > 2;
   const int fj = b.ie[1] - b.is[1] + 1 + 2 * dj;
   const int fk = b.ie[2] - b.is[2] + 1 + 2 * dk;
   const long total = (long)b.ni * fj * fk;  // full rows of the face range
   const int t = threadIdx.x;
-  long L = (long)xcd_remap(blockIdx.x, gridDim.x) * (XB - 2 * H - 1) + t;
+  long L = bx * (XB - 2 * H - 1) + t;
   const bool valid = L < total;
   if (!valid) L = total - 1;
   const int i = (int)(L % b.ni);
@@ -104,6 +113,16 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
   face_out<MHD, RIEMANN, M>(p, 0, b, X, ql, lo, c, flux, fsig);
 }
 
+template <int RECON, int RIEMANN, bool MHD, int GEOM>
+__global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
+                                                   const double *__restrict__ prim,
+                                                   double *__restrict__ flux,
+                                                   double *__restrict__ fsig) {
+  __shared__ double stage[Layout<MHD>::nprim][XB];  // primitives, then upper-edge states
+  flux_x_body<RECON, RIEMANN, MHD, GEOM>(p, b, xcd_remap(blockIdx.x, gridDim.x), prim, flux,
+                                         fsig, stage);
+}
+
 // ---- y, z: LDS tiles of ti_n (i) x ts_n (own direction) cells ----------------------------
 // Rows of ti_n consecutive i are read into LDS; each lane reconstructs its cell along the
 // own direction from LDS and publishes the upper edge for the lane one row up.  Lanes in
@@ -116,17 +135,28 @@ constexpr int kTileMaxThreads = 1024;
 constexpr int kTileMaxRows = 256;
 constexpr int kTileMaxLdsBytes = 64 * 1024;  // two blocks per CU
 
+// number of ti_n x ts_n tiles the sweep along DIR of block b takes
+__host__ __device__ inline long flux_tile_groups(const BlockGeom &b, int dir, int ti_n,
+                                                 int ts_n, int H) {
+  const int T = dir == 1 ? 2 : 1;
+  const int S = ts_n - 2 * H - 1;
+  const long nic = (b.ie[0] - b.is[0] + 3 + ti_n - 1) / ti_n;
+  const long nft = (b.ie[dir] - b.is[dir] + 3 + S - 1) / S;
+  const long ntr = b.ie[T] - b.is[T] + 1 + 2 * (b.ndim > T);
+  return nic * nft * ntr;
+}
+
+// the DIR sweep for logical tile bx of block b; lds = [NP][ts_n][ti_n] doubles
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
-                                                            int ts_n,
-                                                            const double *__restrict__ prim,
-                                                            double *__restrict__ flux,
-                                                            double *__restrict__ fsig) {
+__device__ __forceinline__ void flux_tile_body(const DevParams &p, const BlockGeom &b, int bx,
+                                               int ti_n, int ts_n,
+                                               const double *__restrict__ prim,
+                                               double *__restrict__ flux,
+                                               double *__restrict__ fsig, double *lds) {
   using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
   constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
-  extern __shared__ double lds[];            // [NP][ts_n][ti_n]: primitives, then upper edges
   const int S = ts_n - 2 * H - 1;            // faces per tile
   const int tid = threadIdx.x;
   const int ti = tid % ti_n, ts = tid / ti_n;
@@ -134,7 +164,6 @@ __global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, in
   const int dt_ = b.ndim > T;
   const int nic = (b.ie[0] - b.is[0] + 3 + ti_n - 1) / ti_n;  // i chunks over [is-1, ie+1]
   const int nft = (b.ie[DIR] - b.is[DIR] + 3 + S - 1) / S;    // tiles over faces [s-1, e+1]
-  const int bx = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int ichunk = bx % nic;
   const int tile = (bx / nic) % nft;
   const int tr = bx / (nic * nft);
@@ -185,6 +214,16 @@ __global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, in
   X[T] = xc(b, T, t);
   X[DIR] = xf(b, DIR, o);
   face_out<MHD, RIEMANN, M>(p, DIR, b, X, ql, lo, c, flux, fsig);
+}
+
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
+__global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
+                                                    int ts_n, const double *__restrict__ prim,
+                                                    double *__restrict__ flux,
+                                                    double *__restrict__ fsig) {
+  extern __shared__ double lds[];  // [NP][ts_n][ti_n]: primitives, then upper edges
+  flux_tile_body<DIR, RECON, RIEMANN, MHD, GEOM>(p, b, (int)xcd_remap(blockIdx.x, gridDim.x),
+                                                 ti_n, ts_n, prim, flux, fsig, lds);
 }
 
 // ---- dispatch --------------------------------------------------------------------------
