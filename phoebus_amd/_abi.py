@@ -1,4 +1,5 @@
-"""ctypes binding of libphx.so (include/phx.h, phx_runtime.h, phx_mlp.h, phx_driver.h).
+"""ctypes binding of libphx.so (include/phx.h, phx_runtime.h, phx_mlp.h, phx_driver.h,
+phx_pack.h).
 
 The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading fails loudly
 when it has not been built; there is no CPU fallback.
@@ -63,6 +64,16 @@ class StageArgs(C.Structure):
                 ("cfl", C.c_double), ("dt_out", C.c_void_p)]
 
 
+class PackBlock(C.Structure):
+    _fields_ = [("block", Block), ("f", BlockFields), ("cons1", C.c_void_p),
+                ("iters", C.c_void_p)]
+
+
+class PackStageArgs(C.Structure):
+    _fields_ = [("stage", C.c_int), ("dt", C.c_double), ("dt_dev", C.c_void_p),
+                ("cfl", C.c_double), ("dt_out", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -112,6 +123,15 @@ def lib():
         "phx_autotune": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), i32, vp]),
         "phx_tile_shape": (i32, [vp, P(FluidParams), P(Block), i32, P(i32), P(i32)]),
         "phx_set_tile_shape": (i32, [vp, P(FluidParams), P(Block), i32, i32, i32]),
+        "phx_pack_create": (i32, [vp, P(FluidParams), i32, vp, P(vp)]),
+        "phx_pack_destroy": (i32, [vp]),
+        "phx_pack_nblocks": (i32, [vp, P(i32), P(C.c_int64)]),
+        "phx_pack_stage": (i32, [vp, vp, P(PackStageArgs), vp]),
+        "phx_pack_autotune": (i32, [vp, vp, i32, vp]),
+        "phx_pack_tile_shape": (i32, [vp, i32, P(i32), P(i32)]),
+        "phx_pack_graph_create": (i32, [vp, vp, i32, P(PackStageArgs), vp, P(vp)]),
+        "phx_graph_launch": (i32, [vp, vp]),
+        "phx_graph_destroy": (i32, [vp]),
         "phx_driver_create": (i32, [vp, P(FluidParams), P(Block), P(i32), dbl, dbl, P(vp)]),
         "phx_driver_destroy": (i32, [vp]),
         "phx_driver_set_prim": (i32, [vp, vp]),

@@ -311,3 +311,89 @@ class Driver:
         out = np.zeros((nv.get(what, 1),) + self.shape)
         self.phx._chk(self.L.phx_driver_get(self.h, what, out.ctypes.data), "phx_driver_get")
         return out if what in nv else out[0]
+
+
+class Graph:
+    """A captured, replayable sequence of pack stages (include/phx_pack.h)."""
+
+    def __init__(self, L, h):
+        self.L, self.h = L, h
+
+    def launch(self, stream):
+        rc = self.L.phx_graph_launch(self.h, stream)
+        if rc != A.PHX_OK:
+            raise A.PhxError(f"phx_graph_launch: status {rc}")
+
+    def close(self):
+        if self.h:
+            self.L.phx_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pack:
+    """The fluid stage over a pack of 3-D MHD blocks (include/phx_pack.h).  entries:
+    (block, BlockState, cons1 DeviceArray, iters DeviceArray or None) per block."""
+
+    def __init__(self, phx, p, entries):
+        self.phx, self.L, self.p = phx, phx.L, p
+        self.entries = entries  # keeps the device arrays alive
+        arr = (A.PackBlock * len(entries))()
+        curved = p.geom.kind != A.GEOM_MINKOWSKI
+        for i, (b, st, cons1, iters) in enumerate(entries):
+            arr[i].block = b
+            arr[i].f = st.fields(with_src=curved)
+            arr[i].cons1 = cons1.ptr
+            arr[i].iters = _p(iters)
+        h = C.c_void_p()
+        phx._chk(self.L.phx_pack_create(phx.h, C.byref(p), len(entries), arr, C.byref(h)),
+                 "phx_pack_create")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            self.L.phx_pack_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def args(stage, dt=0.0, dt_dev=None, cfl=0.4, dt_out=None):
+        return A.PackStageArgs(stage, dt, _p(dt_dev), cfl, _p(dt_out))
+
+    def stage(self, stage, dt=0.0, dt_dev=None, cfl=0.4, dt_out=None, stream=None):
+        a = self.args(stage, dt, dt_dev, cfl, dt_out)
+        self.phx._chk(self.L.phx_pack_stage(self.phx.h, self.h, C.byref(a), stream),
+                      "phx_pack_stage")
+
+    def autotune(self, reps=3, stream=None):
+        self.phx._chk(self.L.phx_pack_autotune(self.phx.h, self.h, reps, stream),
+                      "phx_pack_autotune")
+
+    def tile_shape(self, d):
+        ti, ts = C.c_int(), C.c_int()
+        self.phx._chk(self.L.phx_pack_tile_shape(self.h, d, C.byref(ti), C.byref(ts)),
+                      "phx_pack_tile_shape")
+        return ti.value, ts.value
+
+    def cells(self):
+        n, c = C.c_int(), C.c_int64()
+        self.L.phx_pack_nblocks(self.h, C.byref(n), C.byref(c))
+        return n.value, c.value
+
+    def graph(self, stages, stream):
+        """stages: list of PackStageArgs (Pack.args(...)); stream: a non-default stream."""
+        arr = (A.PackStageArgs * len(stages))(*stages)
+        h = C.c_void_p()
+        self.phx._chk(self.L.phx_pack_graph_create(self.phx.h, self.h, len(stages), arr, stream,
+                                                   C.byref(h)), "phx_pack_graph_create")
+        return Graph(self.L, h.value)

@@ -226,6 +226,28 @@ __global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, in
                                                  ti_n, ts_n, prim, flux, fsig, lds);
 }
 
+// ---- packs: the same sweeps over a work list of blocks --------------------------------
+template <int RECON, int RIEMANN, int GEOM>
+__global__ __launch_bounds__(XB) void k_pack_flux_x(DevParams p, PackView v) {
+  __shared__ double stage[Layout<true>::nprim][XB];
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  flux_x_body<RECON, RIEMANN, true, GEOM>(p, E.bg, g - v.start[b], E.prim, E.flux, E.fsig,
+                                          stage);
+}
+
+template <int DIR, int RECON, int RIEMANN, int GEOM>
+__global__ __launch_bounds__(1024) void k_pack_flux_tile(DevParams p, PackView v, int ti_n,
+                                                         int ts_n) {
+  extern __shared__ double lds[];
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  flux_tile_body<DIR, RECON, RIEMANN, true, GEOM>(p, E.bg, (int)(g - v.start[b]), ti_n, ts_n,
+                                                  E.prim, E.flux, E.fsig, lds);
+}
+
 // ---- dispatch --------------------------------------------------------------------------
 TileShape resolve_tile_shape(int d, int recon, int mhd, const BlockGeom &bg, TileShape want) {
   const int H = recon == PHX_RECON_CONSTANT ? 0 : recon == PHX_RECON_LINEAR ? 1 : 2;
@@ -327,6 +349,56 @@ void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevPa
   default:
     launch_riemann<PHX_RECON_MP5>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
+  }
+}
+
+}  // namespace phx
+
+namespace phx {
+
+long pack_groups_flux(int d, int recon, const BlockGeom &bg, TileShape shape) {
+  const int H = recon == PHX_RECON_CONSTANT ? 0 : recon == PHX_RECON_LINEAR ? 1 : 2;
+  if (d == 0) return flux_x_groups(bg, H);
+  return flux_tile_groups(bg, d, shape.ti, shape.ts, H);
+}
+
+template <int RECON, int RIEMANN, int GEOM>
+static void pack_dir(int d, const DevParams &dp, const PackView &v, long groups, TileShape sh,
+                     hipStream_t s) {
+  if (groups <= 0) return;
+  if (d == 0) {
+    hipLaunchKernelGGL((k_pack_flux_x<RECON, RIEMANN, GEOM>), dim3((unsigned)groups), dim3(XB), 0,
+                       s, dp, v);
+    return;
+  }
+  const int threads = ((sh.ti * sh.ts + 63) / 64) * 64;
+  const size_t lds = sizeof(double) * (size_t)Layout<true>::nprim * sh.ti * sh.ts;
+  if (d == 1)
+    hipLaunchKernelGGL((k_pack_flux_tile<1, RECON, RIEMANN, GEOM>), dim3((unsigned)groups),
+                       dim3(threads), lds, s, dp, v, sh.ti, sh.ts);
+  else
+    hipLaunchKernelGGL((k_pack_flux_tile<2, RECON, RIEMANN, GEOM>), dim3((unsigned)groups),
+                       dim3(threads), lds, s, dp, v, sh.ti, sh.ts);
+}
+
+template <int RECON>
+static void pack_recon(int d, int riemann, int geom, const DevParams &dp, const PackView &v,
+                       long groups, TileShape sh, hipStream_t s) {
+  with_geom(geom, [&](auto G) {
+    if (riemann == PHX_RIEMANN_HLL)
+      pack_dir<RECON, PHX_RIEMANN_HLL, decltype(G)::value>(d, dp, v, groups, sh, s);
+    else
+      pack_dir<RECON, PHX_RIEMANN_LLF, decltype(G)::value>(d, dp, v, groups, sh, s);
+  });
+}
+
+void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
+                      const PackView &v, long groups, TileShape sh, hipStream_t s) {
+  switch (recon) {
+  case PHX_RECON_CONSTANT: pack_recon<PHX_RECON_CONSTANT>(d, riemann, geom, dp, v, groups, sh, s); break;
+  case PHX_RECON_LINEAR: pack_recon<PHX_RECON_LINEAR>(d, riemann, geom, dp, v, groups, sh, s); break;
+  case PHX_RECON_WENO5Z: pack_recon<PHX_RECON_WENO5Z>(d, riemann, geom, dp, v, groups, sh, s); break;
+  default: pack_recon<PHX_RECON_MP5>(d, riemann, geom, dp, v, groups, sh, s); break;
   }
 }
 

@@ -80,6 +80,50 @@ void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
                     const double *prim, double *src, hipStream_t s);
 void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s);
 
+// ---- block packs (a MeshBlockPack-style work list over blocks of different sizes) -------
+// One entry per block: its geometry and field pointers for the stage being run.
+struct PackEntry {
+  BlockGeom bg;
+  const double *prim;       // reconstructed primitives (read by the flux sweeps)
+  double *flux, *fsig, *emf, *src;
+  double *u_stage;          // U_s (averaged in place)
+  const double *u_base;     // U^n
+  double *u_out;            // U_{s+1}
+  double *prim_out, *temp, *mu, *csig, *fail;
+  int32_t *iters;
+};
+// A launch over a pack: workgroup g belongs to the block b with start[b] <= g < start[b+1].
+struct PackView {
+  const PackEntry *e;
+  const long *start;  // [n + 1] prefix sums of workgroups per block for this launch
+  int n;
+};
+
+__device__ __forceinline__ int pack_find(const long *start, int n, long g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// pack launches (phx_flux.hip, phx_kernels.hip, phx_stage.hip); 3-D MHD blocks
+void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
+                      const PackView &v, long groups, TileShape shape, hipStream_t s);
+void launch_pack_ct(const PackView &v, long groups, hipStream_t s);
+void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
+                         hipStream_t s);
+void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
+                        double beta, double dt, const double *dt_dev, double *partial,
+                        hipStream_t s);
+// groups per block for each launch kind
+long pack_groups_flux(int d, int recon, const BlockGeom &bg, TileShape shape);
+long pack_groups_cells(const BlockGeom &bg, bool corners);
+
 // partial minima of the CFL reduction (one per 256-cell block of the fused stage kernel,
 // so one call covers up to 16.7 M cells; the separate reduction strides beyond that)
 constexpr int kDtPartials = 1 << 16;
@@ -93,6 +137,7 @@ struct phx_ctx {
   };
   static constexpr int kMaxTimed = 64;
   bool profiling = false;
+  bool capturing = false;  // a graph capture is recording: no profiling events
   int n_timed = 0;
   TimedLaunch timed[kMaxTimed] = {};
 
@@ -121,12 +166,12 @@ struct phx_ctx {
   }
   // events bracketing one launch on stream s (no-op unless profiling)
   void mark_begin(const char *name, hipStream_t s) {
-    if (!profiling || n_timed >= kMaxTimed) return;
+    if (!profiling || capturing || n_timed >= kMaxTimed) return;
     timed[n_timed].name = name;
     (void)hipEventRecord(timed[n_timed].start, s);
   }
   void mark_end(hipStream_t s) {
-    if (!profiling || n_timed >= kMaxTimed) return;
+    if (!profiling || capturing || n_timed >= kMaxTimed) return;
     (void)hipEventRecord(timed[n_timed].stop, s);
     ++n_timed;
   }

@@ -21,11 +21,12 @@
 namespace phx {
 
 // ---- FluxCT (fluid.cpp:775-841) --------------------------------------------------------
+// corner EMFs of a 3-D block: thread t of (nx+1)(ny+1)(nz+1)
 template <int NC>
-__global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, const double *__restrict__ flux,
-                                                   double *__restrict__ emf) {
+__device__ __forceinline__ void ct_emf_3d_body(const BlockGeom &b, long t,
+                                               const double *__restrict__ flux,
+                                               double *__restrict__ emf) {
   const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 2;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nk) return;
   const int i = (int)(t % ni) + b.is[0];
   const int j = (int)((t / ni) % nj) + b.is[1];
@@ -39,10 +40,17 @@ __global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, const double *__
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_ct_flux_3d(BlockGeom b, double *__restrict__ flux,
-                                                    const double *__restrict__ emf) {
+__global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, const double *__restrict__ flux,
+                                                   double *__restrict__ emf) {
+  ct_emf_3d_body<NC>(b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, emf);
+}
+
+// B fluxes from the corner EMFs of a 3-D block: thread t of (nx+1)(ny+1)(nz+1)
+template <int NC>
+__device__ __forceinline__ void ct_flux_3d_body(const BlockGeom &b, long t,
+                                                double *__restrict__ flux,
+                                                const double *__restrict__ emf) {
   const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 2;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nk) return;
   const int i = (int)(t % ni) + b.is[0];
   const int j = (int)((t / ni) % nj) + b.is[1];
@@ -60,6 +68,12 @@ __global__ __launch_bounds__(256) void k_ct_flux_3d(BlockGeom b, double *__restr
   F(2, 0) = 0.5 * (e1[c] + e1[c + 1]);
   F(2, 1) = -0.5 * (e0[c] + e0[c + sj]);
   F(2, 2) = 0.0;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_ct_flux_3d(BlockGeom b, double *__restrict__ flux,
+                                                    const double *__restrict__ emf) {
+  ct_flux_3d_body<NC>(b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, emf);
 }
 
 template <int NC>
@@ -353,14 +367,14 @@ __device__ __forceinline__ void geom_derivs(const DevParams &p, double X1, doubl
     ks_derivs(p.a, X1, X2, o);
 }
 
+// geometric source terms of interior cell t of block b
 template <bool MHD, int GEOM>
-__global__ __launch_bounds__(128) void k_sources(DevParams p, BlockGeom b,
-                                                 const double *__restrict__ prim,
-                                                 double *__restrict__ src) {
+__device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom &b, long t,
+                                             const double *__restrict__ prim,
+                                             double *__restrict__ src) {
   using L = Layout<MHD>;
   using M = typename MetricOf<GEOM>::type;
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nk) return;
   const int i = (int)(t % ni) + b.is[0];
   const int j = (int)((t / ni) % nj) + b.is[1];
@@ -430,11 +444,45 @@ __global__ __launch_bounds__(128) void k_sources(DevParams p, BlockGeom b,
   src[4L * n + c] = gdet * alpha * (Ta - TGam);
 }
 
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(128) void k_sources(DevParams p, BlockGeom b,
+                                                 const double *__restrict__ prim,
+                                                 double *__restrict__ src) {
+  sources_body<MHD, GEOM>(p, b, (long)blockIdx.x * blockDim.x + threadIdx.x, prim, src);
+}
+
 template <int NC>
 __global__ void k_zero_sources(BlockGeom b, double *__restrict__ src) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= b.ncell) return;
   for (int l = 1; l <= 4; ++l) src[(long)l * b.ncell + t] = 0.0;
+}
+
+// ---- packs: CT and sources over a work list of 3-D blocks -------------------------------
+template <int NC>
+__global__ __launch_bounds__(256) void k_pack_ct_emf(PackView v) {
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  ct_emf_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.emf);
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_pack_ct_flux(PackView v) {
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  ct_flux_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.emf);
+}
+
+template <int GEOM>
+__global__ __launch_bounds__(128) void k_pack_sources(DevParams p, PackView v) {
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  // two 128-thread halves per 256-cell group
+  sources_body<true, GEOM>(p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.prim, E.src);
+  sources_body<true, GEOM>(p, E.bg, (g - v.start[b]) * 256 + 128 + threadIdx.x, E.prim, E.src);
 }
 
 // ---- batched con2prim / prim2con over independent states --------------------------------
@@ -575,6 +623,29 @@ int check_block(phx_ctx *ctx, const phx_block *b, int recon) {
   for (int d = 0; d < 3; ++d) cells *= b->nx[d] + (d < b->ndim ? 2 * b->ng : 0);
   if (cells > (1L << 31)) return ctx->fail(PHX_ERR_INVALID, "block too large");
   return PHX_OK;
+}
+
+long pack_groups_cells(const BlockGeom &bg, bool corners) {
+  const long c = corners ? 1 : 0;
+  const long n = (bg.ie[0] - bg.is[0] + 1 + c) * (bg.ie[1] - bg.is[1] + 1 + c) *
+                 (bg.ie[2] - bg.is[2] + 1 + c);
+  return (n + 255) / 256;
+}
+
+void launch_pack_ct(const PackView &v, long groups, hipStream_t s) {
+  if (groups <= 0) return;
+  hipLaunchKernelGGL(k_pack_ct_emf<8>, dim3((unsigned)groups), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_pack_ct_flux<8>, dim3((unsigned)groups), dim3(256), 0, s, v);
+}
+
+void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
+                         hipStream_t s) {
+  if (groups <= 0) return;
+  with_geom(geom, [&](auto G) {
+    constexpr int GEOM = decltype(G)::value;
+    if constexpr (GEOM != PHX_GEOM_MINKOWSKI)
+      hipLaunchKernelGGL((k_pack_sources<GEOM>), dim3((unsigned)groups), dim3(128), 0, s, dp, v);
+  });
 }
 
 void launch_flux_ct(const BlockGeom &bg, double *flux, double *emf, hipStream_t s) {

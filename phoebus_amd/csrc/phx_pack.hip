@@ -1,0 +1,321 @@
+// phx_pack.hip -- the fluid stage over a pack of meshblocks (include/phx_pack.h): block
+// table and work lists on the device, one launch per kernel of the stage, tile tuning and
+// HIP-graph capture.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <new>
+#include <vector>
+
+#include "phx.h"
+#include "phx_internal.hpp"
+#include "phx_pack.h"
+
+using namespace phx;
+
+struct phx_pack {
+  int device = 0;
+  phx_fluid_params p;
+  DevParams dp;
+  int n = 0;
+  long interior = 0;
+  std::vector<phx_pack_block> blocks;
+  std::vector<BlockGeom> geoms;
+  PackEntry *entries[2] = {nullptr, nullptr};  // device: stage 1 and stage 2 roles
+  // work lists: 0..2 flux sweeps, 3 CT corners, 4 interior cells
+  long *start[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  long groups[5] = {0, 0, 0, 0, 0};
+  TileShape tiles[3];
+  double *partial = nullptr;  // [groups[4]] CFL partial minima
+};
+
+struct phx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// tile shapes must fit every block of the pack: resolve against the smallest extent
+TileShape pack_shape(const phx_pack *k, int d, TileShape want) {
+  TileShape best = resolve_tile_shape(d, k->p.recon, 1, k->geoms[0], want);
+  for (const auto &g : k->geoms) {
+    const TileShape t = resolve_tile_shape(d, k->p.recon, 1, g, want);
+    if (t.ts < best.ts) best.ts = t.ts;
+    if (t.ti < best.ti) best.ti = t.ti;
+  }
+  return best;
+}
+
+int build_start(phx_pack *k, int kind) {
+  std::vector<long> st(k->n + 1, 0);
+  for (int b = 0; b < k->n; ++b) {
+    const BlockGeom &g = k->geoms[b];
+    const long c = kind < 3 ? pack_groups_flux(kind, k->p.recon, g, k->tiles[kind])
+                            : pack_groups_cells(g, kind == 3);
+    st[b + 1] = st[b] + c;
+  }
+  if (st[k->n] > (long)UINT_MAX) return PHX_ERR_INVALID;
+  k->groups[kind] = st[k->n];
+  if (!k->start[kind] && hipMalloc(&k->start[kind], sizeof(long) * (k->n + 1)) != hipSuccess)
+    return PHX_ERR_NOMEM;
+  return hipMemcpy(k->start[kind], st.data(), sizeof(long) * (k->n + 1),
+                   hipMemcpyHostToDevice) == hipSuccess
+             ? PHX_OK
+             : PHX_ERR_DEVICE;
+}
+
+PackView view(const phx_pack *k, int stage, int kind) {
+  PackView v;
+  v.e = k->entries[stage - 1];
+  v.start = k->start[kind];
+  v.n = k->n;
+  return v;
+}
+
+void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipStream_t s) {
+  static const char *kName[3] = {"flux_x", "flux_y", "flux_z"};
+  const int st = a->stage;
+  for (int d = 0; d < 3; ++d) {
+    ctx->mark_begin(kName[d], s);
+    launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, view(k, st, d),
+                     k->groups[d], k->tiles[d], s);
+    ctx->mark_end(s);
+  }
+  ctx->mark_begin("flux_ct", s);
+  launch_pack_ct(view(k, st, 3), k->groups[3], s);
+  ctx->mark_end(s);
+  if (k->p.geom.kind != PHX_GEOM_MINKOWSKI) {
+    ctx->mark_begin("sources", s);
+    launch_pack_sources(k->p.geom.kind, k->dp, view(k, st, 4), k->groups[4], s);
+    ctx->mark_end(s);
+  }
+  ctx->mark_begin("update_con2prim", s);
+  launch_pack_update(k->p.geom.kind, k->dp, view(k, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
+                     a->dt, a->dt_dev, a->dt_out ? k->partial : nullptr, s);
+  ctx->mark_end(s);
+  if (a->dt_out) {
+    ctx->mark_begin("dt_finish", s);
+    launch_dt_finish(k->partial, (int)k->groups[4], a->cfl, a->dt_out, s);
+    ctx->mark_end(s);
+  }
+}
+
+int check_stage_args(phx_ctx *ctx, const phx_pack_stage_args *a) {
+  if (!a || (a->stage != 1 && a->stage != 2))
+    return ctx->fail(PHX_ERR_INVALID, "stage must be 1 or 2");
+  if (!a->dt_dev && !(a->dt >= 0.0)) return ctx->fail(PHX_ERR_INVALID, "invalid dt");
+  return PHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
+                    const phx_pack_block *blocks, phx_pack **out) {
+  if (!ctx || !out || !blocks || nblocks <= 0) return PHX_ERR_INVALID;
+  *out = nullptr;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (!p->mhd) return ctx->fail(PHX_ERR_UNSUPPORTED, "packs hold 3-D MHD blocks");
+  phx_pack *k = new (std::nothrow) phx_pack();
+  if (!k) return PHX_ERR_NOMEM;
+  k->device = ctx->device;
+  k->p = *p;
+  k->dp = make_dev_params(p);
+  k->n = nblocks;
+  k->blocks.assign(blocks, blocks + nblocks);
+  const bool curved = p->geom.kind != PHX_GEOM_MINKOWSKI;
+  std::vector<PackEntry> e[2];
+  for (int b = 0; b < nblocks; ++b) {
+    const phx_pack_block &B = blocks[b];
+    if ((rc = check_block(ctx, &B.block, p->recon)) || B.block.ndim != 3) {
+      phx_pack_destroy(k);
+      return rc ? rc : ctx->fail(PHX_ERR_UNSUPPORTED, "packs hold 3-D MHD blocks");
+    }
+    const phx_block_fields &f = B.f;
+    if (!f.prim || !f.cons || !B.cons1 || !f.flux || !f.face_sig || !f.cell_sig || !f.fail ||
+        !f.c2p_mu || !f.emf || (curved && !f.src)) {
+      phx_pack_destroy(k);
+      return ctx->fail(PHX_ERR_INVALID, "null field in a pack block");
+    }
+    const BlockGeom g = make_block_geom(&B.block);
+    k->geoms.push_back(g);
+    k->interior += (long)B.block.nx[0] * B.block.nx[1] * B.block.nx[2];
+    for (int st = 1; st <= 2; ++st) {
+      PackEntry x;
+      x.bg = g;
+      x.prim = f.prim;
+      x.flux = f.flux;
+      x.fsig = f.face_sig;
+      x.emf = f.emf;
+      x.src = curved ? f.src : nullptr;
+      x.u_stage = st == 1 ? f.cons : B.cons1;
+      x.u_base = f.cons;
+      x.u_out = st == 1 ? B.cons1 : f.cons;
+      x.prim_out = f.prim;
+      x.temp = f.temp;
+      x.mu = f.c2p_mu;
+      x.csig = f.cell_sig;
+      x.fail = f.fail;
+      x.iters = B.iters;
+      e[st - 1].push_back(x);
+    }
+  }
+  for (int d = 0; d < 3; ++d) k->tiles[d] = pack_shape(k, d, TileShape());
+  bool ok = true;
+  for (int st = 0; st < 2 && ok; ++st)
+    ok = hipMalloc(&k->entries[st], sizeof(PackEntry) * nblocks) == hipSuccess &&
+         hipMemcpy(k->entries[st], e[st].data(), sizeof(PackEntry) * nblocks,
+                   hipMemcpyHostToDevice) == hipSuccess;
+  for (int kind = 0; kind < 5 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
+  ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max(k->groups[4], 1L)) == hipSuccess;
+  if (!ok) {
+    phx_pack_destroy(k);
+    return ctx->fail(PHX_ERR_DEVICE, "pack set-up failed");
+  }
+  *out = k;
+  return PHX_OK;
+}
+
+int phx_pack_destroy(phx_pack *k) {
+  if (!k) return PHX_OK;
+  (void)hipSetDevice(k->device);
+  for (auto &e : k->entries)
+    if (e) (void)hipFree(e);
+  for (auto &s : k->start)
+    if (s) (void)hipFree(s);
+  if (k->partial) (void)hipFree(k->partial);
+  delete k;
+  return PHX_OK;
+}
+
+int phx_pack_nblocks(const phx_pack *k, int *n, int64_t *interior_cells) {
+  if (!k) return PHX_ERR_INVALID;
+  if (n) *n = k->n;
+  if (interior_cells) *interior_cells = k->interior;
+  return PHX_OK;
+}
+
+int phx_pack_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a,
+                   phx_stream stream) {
+  if (!ctx || !k) return PHX_ERR_INVALID;
+  int rc = check_stage_args(ctx, a);
+  if (rc) return rc;
+  launch_stage(ctx, k, a, ctx->stream_of(stream));
+  return ctx->check_launch();
+}
+
+int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
+  if (!ctx || !k) return PHX_ERR_INVALID;
+  if (reps < 1) reps = 3;
+  hipStream_t s = ctx->stream_of(stream);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "event create failed");
+  static const int kTi[] = {1, 2, 4, 8, 16};
+  static const int kTs[] = {16, 24, 32, 48, 64, 96, 128};
+  int rc = PHX_OK;
+  for (int d = 1; d < 3 && rc == PHX_OK; ++d) {
+    float best = 1e30f;
+    TileShape best_shape = k->tiles[d];
+    std::vector<TileShape> seen;
+    for (int ti : kTi)
+      for (int ts : kTs) {
+        const TileShape c = pack_shape(k, d, TileShape{ti, ts});
+        bool dup = false;
+        for (const auto &x : seen) dup = dup || (x.ti == c.ti && x.ts == c.ts);
+        if (dup) continue;
+        seen.push_back(c);
+        k->tiles[d] = c;
+        if ((rc = build_start(k, d)) != PHX_OK) break;
+        const PackView v = view(k, 1, d);
+        launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d], c, s);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r)
+          launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d],
+                           c, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          best_shape = c;
+        }
+      }
+    k->tiles[d] = best_shape;
+    if (rc == PHX_OK) rc = build_start(k, d);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PHX_OK) return rc;
+  return ctx->check_launch();
+}
+
+int phx_pack_tile_shape(const phx_pack *k, int dir, int *ti, int *ts) {
+  if (!k || !ti || !ts || dir < 0 || dir > 2) return PHX_ERR_INVALID;
+  *ti = k->tiles[dir].ti;
+  *ts = k->tiles[dir].ts;
+  return PHX_OK;
+}
+
+int phx_pack_graph_create(phx_ctx *ctx, phx_pack *k, int nstages,
+                          const phx_pack_stage_args *stages, phx_stream stream,
+                          phx_graph **out) {
+  if (!ctx || !k || !out || nstages < 1 || !stages) return PHX_ERR_INVALID;
+  *out = nullptr;
+  if (!stream) return ctx->fail(PHX_ERR_INVALID, "graph capture needs a non-default stream");
+  for (int i = 0; i < nstages; ++i) {
+    const int rc = check_stage_args(ctx, &stages[i]);
+    if (rc) return rc;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  (void)hipGetLastError();
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return ctx->fail(PHX_ERR_DEVICE, "stream capture failed to start");
+  ctx->capturing = true;
+  for (int i = 0; i < nstages; ++i) launch_stage(ctx, k, &stages[i], s);
+  ctx->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  const hipError_t el = hipGetLastError();
+  if (ec != hipSuccess || el != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return ctx->fail(PHX_ERR_DEVICE, "stage capture failed");
+  }
+  phx_graph *G = new (std::nothrow) phx_graph();
+  if (!G) {
+    (void)hipGraphDestroy(g);
+    return PHX_ERR_NOMEM;
+  }
+  G->graph = g;
+  if (hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    phx_graph_destroy(G);
+    return ctx->fail(PHX_ERR_DEVICE, "graph instantiation failed");
+  }
+  *out = G;
+  return PHX_OK;
+}
+
+int phx_graph_launch(phx_graph *G, phx_stream stream) {
+  if (!G || !G->exec) return PHX_ERR_INVALID;
+  return hipGraphLaunch(G->exec, reinterpret_cast<hipStream_t>(stream)) == hipSuccess
+             ? PHX_OK
+             : PHX_ERR_DEVICE;
+}
+
+int phx_graph_destroy(phx_graph *G) {
+  if (!G) return PHX_OK;
+  if (G->exec) (void)hipGraphExecDestroy(G->exec);
+  if (G->graph) (void)hipGraphDestroy(G->graph);
+  delete G;
+  return PHX_OK;
+}
+
+}  // extern "C"

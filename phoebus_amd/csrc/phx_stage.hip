@@ -12,18 +12,19 @@
 
 namespace phx {
 
+// interior cell t of block b: returns the cell's CFL time (DBL_MAX for t beyond the block
+// or when with_dt is false)
 template <bool MHD, int GEOM>
-__global__ __launch_bounds__(256) void k_update_con2prim(
-    DevParams p, BlockGeom b, const double *__restrict__ flux, const double *__restrict__ src,
-    double *u_stage, const double *u_base, double beta, double bdt,
-    double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
+__device__ __forceinline__ double update_con2prim_body(
+    const DevParams &p, const BlockGeom &b, long t, const double *__restrict__ flux,
+    const double *__restrict__ src, double *u_stage, const double *u_base, double beta,
+    double bdt, double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
-    double *__restrict__ fail, int32_t *__restrict__ iters, double *__restrict__ partial) {
+    double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
   constexpr int NC = L::ncons;
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = t < (long)ni * nj * nk;
   double ldt_min = DBL_MAX;
   if (active) {
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void k_update_con2prim(
     fail[c] = st;
     if (iters) iters[c] = it;
     // EstimateTimestepBlock with face speeds (fluid.cpp:913-928)
-    if (partial) {
+    if (with_dt) {
       double ldt = 0.0;
       for (int d = 0; d < b.ndim; ++d) {
         const long off = d == 0 ? 1 : d == 1 ? sj : sk;
@@ -91,10 +92,53 @@ __global__ __launch_bounds__(256) void k_update_con2prim(
       ldt_min = 1.0 / (ldt + 1.e-50);
     }
   }
+  return ldt_min;
+}
+
+template <bool MHD, int GEOM>
+__global__ __launch_bounds__(256) void k_update_con2prim(
+    DevParams p, BlockGeom b, const double *__restrict__ flux, const double *__restrict__ src,
+    double *u_stage, const double *u_base, double beta, double bdt,
+    double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
+    double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
+    double *__restrict__ fail, int32_t *__restrict__ iters, double *__restrict__ partial) {
+  const double v = update_con2prim_body<MHD, GEOM>(
+      p, b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, src, u_stage, u_base, beta, bdt,
+      u_out, prim_out, temp, mu, csig, fsig, fail, iters, partial != nullptr);
   if (partial) {
-    const double v = block_min(ldt_min);
-    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+    const double m = block_min(v);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
   }
+}
+
+// packs: the fused update over a work list of 3-D MHD blocks; dt from dt_dev when given
+// (a device-resident time step, so a captured graph needs no host value)
+template <int GEOM>
+__global__ __launch_bounds__(256) void k_pack_update(DevParams p, PackView v, double beta,
+                                                     double dt, const double *dt_dev,
+                                                     double *__restrict__ partial) {
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  const double bdt = beta * (dt_dev ? dt_dev[0] : dt);
+  const double val = update_con2prim_body<true, GEOM>(
+      p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.src, E.u_stage, E.u_base, beta,
+      bdt, E.u_out, E.prim_out, E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
+      partial != nullptr);
+  if (partial) {
+    const double m = block_min(val);
+    if (threadIdx.x == 0) partial[g] = m;
+  }
+}
+
+void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
+                        double beta, double dt, const double *dt_dev, double *partial,
+                        hipStream_t s) {
+  if (groups <= 0) return;
+  with_geom(geom, [&](auto G) {
+    hipLaunchKernelGGL((k_pack_update<decltype(G)::value>), dim3((unsigned)groups), dim3(256), 0,
+                       s, dp, v, beta, dt, dt_dev, partial);
+  });
 }
 
 }  // namespace phx
