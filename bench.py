@@ -223,6 +223,182 @@ def c3_learned_vs_plain(phx, args, log):
     return out
 
 
+class _ArenaBlock:
+    """Device fields of one pack block as views into shared arenas."""
+
+    def __init__(self, arenas, off, sh):
+        n = int(np.prod(sh))
+        self.shape = sh
+        self.views = {}
+        for name, (arr, nv) in arenas.items():
+            self.views[name] = arr.view(off * nv, (nv,) + sh if nv > 1 else sh)
+
+    def __getattr__(self, k):
+        v = self.__dict__.get("views", {})
+        if k in v:
+            return v[k]
+        raise AttributeError(k)
+
+    def fields(self, with_src=False):
+        from phoebus_amd.api import Phx
+        v = self.views
+        return Phx.fields(prim=v["prim"], cons=v["cons"], temp=v["temp"], c2p_mu=v["mu"],
+                          fail=v["fail"], cell_sig=v["csig"], face_sig=v["fsig"],
+                          flux=v["flux"], emf=v["emf"], src=v["src"] if with_src else None)
+
+
+def c4_uneven_pack(phx, args, log):
+    """Uneven batch: 512 blocks of {16^3: 50 %, 32^3: 35 %, 64^3: 15 %} cut from a
+    magnetized Fishbone-Moncrief torus (r_in 6, r_max 12, beta 100, 4 % u jitter) on an
+    FMKS grid (a = 0.9375, 256 x 128 x 128 spacing); WENO5-Z + HLL + CT + sources +
+    update + con2prim.  Times full RK2 steps (two stages, device-resident CFL dt) as one
+    HIP graph, as eager pack launches, and as one phx_fluid_stage per block."""
+    from phoebus_amd import _abi as A
+    from phoebus_amd.api import Pack, block, fluid_params, mgr
+    from phoebus_amd.device import DeviceArray, Event, synchronize
+    from phoebus_amd.workloads import torus_grid, uneven_blocks
+
+    rng = np.random.default_rng(SEED + 4)
+    grid_prim, xmin, dx = torus_grid(rng)
+    blocks = uneven_blocks(rng, args.c4_blocks)
+    x1min = xmin[0]
+    # floors and ceilings of the torus deck (inputs/torus.pin: ExpX1RhoSie rho 1e-8 r^-2,
+    # sie 1e-3 r^-1, EOS minima 1e-8, gamma <= 10, sie <= 1)
+    floors = dict(floor_type=A.FLOOR_EXPX1_RHO_SIE, rho0=1e-8, sie0=1e-3, rho_exp=-2.0,
+                  sie_exp=-1.0, rho_min_eos=1e-8, sie_min_eos=1e-8)
+    p = fluid_params(gamma=5.0 / 3.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
+                     geom=A.GEOM_FMKS, a=0.9375, c2p_tol=1e-10, c2p_max_iter=100,
+                     floors=floors, ceilings=(10.0, 1.0),
+                     mgr_params=mgr(derefine_poles=False, x0=min(x1min, max(0.0, x1min - NG * dx[0]))))
+    shapes = [(n + 2 * NG,) * 3 for n, _ in blocks]
+    cells = [int(np.prod(s)) for s in shapes]
+    offs = np.concatenate([[0], np.cumsum(cells)])
+    total = int(offs[-1])
+    per = {"prim": 10, "cons": 8, "cons1": 8, "temp": 1, "mu": 1, "fail": 1, "csig": 3,
+           "fsig": 3, "flux": 24, "emf": 3, "src": 8}
+    arenas = {k: (DeviceArray((nv * total,)), nv) for k, nv in per.items()}
+    it_arena = DeviceArray((total,), np.int32)
+    for k in ("temp", "mu", "fail", "csig", "fsig", "flux", "emf", "src"):
+        arenas[k][0].zero()
+    mu0 = DeviceArray((total,))
+    entries, interior = [], 0
+    for bidx, (n, o) in enumerate(blocks):
+        sh = shapes[bidx]
+        ab = _ArenaBlock(arenas, int(offs[bidx]), sh)
+        b = block(3, (n, n, n), NG, xmin=tuple(xmin[d] + o[d] * dx[d] for d in range(3)), dx=dx)
+        win = np.ascontiguousarray(grid_prim[:, o[2]:o[2] + n + 2 * NG, o[1]:o[1] + n + 2 * NG,
+                                             o[0]:o[0] + n + 2 * NG])
+        ab.prim.upload(win)
+        phx.prim_to_con(p, b, ab.fields())
+        it = it_arena.view(int(offs[bidx]), sh)
+        phx.con2prim(p, b, ab.fields(), iters=it)  # the warm start: the input's own roots
+        entries.append((b, ab, ab.cons1, it))
+        interior += n ** 3
+    synchronize()
+    del grid_prim
+
+    def interior_fails():
+        fh = arenas["fail"][0].get()
+        out = 0
+        for i in range(len(blocks)):
+            f = fh[int(offs[i]):int(offs[i + 1])].reshape(shapes[i])[NG:-NG, NG:-NG, NG:-NG]
+            out += int(np.sum(f == 0.0))
+        return out
+
+    fails_setup = interior_fails()
+    # snapshots to restart every timed variant from the same state
+    snap = {k: DeviceArray(arenas[k][0].shape) for k in ("prim", "cons", "mu")}
+    for k in snap:
+        snap[k].fill_from(arenas[k][0])
+
+    def restore():
+        for k in snap:
+            arenas[k][0].fill_from(snap[k])
+        synchronize()
+
+    pk = Pack(phx, p, entries)
+    sp = C.c_void_p()
+    phx.L.phx_rt_stream_create(C.byref(sp))
+    s = sp.value
+    pk.autotune(reps=2, stream=s)
+    phx.L.phx_rt_stream_synchronize(s)
+    tiles = [pk.tile_shape(d) for d in (1, 2)]
+    dt_dev = DeviceArray((1,))
+    # the first dt: the CFL estimate of a stage-1 pass
+    restore()
+    dt_dev.upload(np.array([1e-4]))
+    pk.stage(1, dt_dev=dt_dev, cfl=0.4, dt_out=dt_dev, stream=s)
+    phx.L.phx_rt_stream_synchronize(s)
+    dt0 = dt_dev.get().copy()
+    steps = args.c4_steps
+    e0, e1 = Event(), Event()
+
+    def timed(run):
+        restore()
+        dt_dev.upload(dt0)
+        run()  # warm-up step
+        phx.L.phx_rt_stream_synchronize(s)
+        restore()
+        dt_dev.upload(dt0)
+        e0.record(s)
+        for _ in range(steps):
+            run()
+        e1.record(s)
+        phx.L.phx_rt_stream_synchronize(s)
+        return e0.elapsed_ms(e1) / steps
+
+    stages = [Pack.args(1, dt_dev=dt_dev, cfl=0.4, dt_out=None),
+              Pack.args(2, dt_dev=dt_dev, cfl=0.4, dt_out=dt_dev)]
+    g = pk.graph(stages, s)
+    ms_graph = timed(lambda: g.launch(s))
+    fail_graph = interior_fails()
+
+    def eager():
+        pk.stage(1, dt_dev=dt_dev, cfl=0.4, stream=s)
+        pk.stage(2, dt_dev=dt_dev, cfl=0.4, dt_out=dt_dev, stream=s)
+
+    ms_eager = timed(eager)
+    d_one = DeviceArray((1,))
+
+    def per_block():
+        # one phx_fluid_stage per block and stage (7 launches each), host dt
+        for stage in (1, 2):
+            for b, ab, c1, it in entries:
+                f = ab.fields(with_src=True)
+                if stage == 1:
+                    phx.fluid_stage(p, b, f, 1.0, float(dt0[0]), ab.cons, c1, ab.prim,
+                                    iters=it, cfl=0.4, dt_out=None, stream=s)
+                else:
+                    f.cons = c1.ptr
+                    phx.fluid_stage(p, b, f, 0.5, float(dt0[0]), ab.cons, ab.cons, ab.prim,
+                                    iters=it, cfl=0.4, dt_out=None, stream=s)
+
+    ms_blocks = timed(per_block) if args.c4_per_block else None
+    g.close()
+    it_h = it_arena.get()
+    its = np.concatenate([it_h[int(offs[i]):int(offs[i + 1])].reshape(shapes[i])[NG:-NG, NG:-NG, NG:-NG].ravel()
+                          for i in range(len(blocks))])
+    phx.L.phx_rt_stream_destroy(s)
+    pk.close()
+    zu = 2 * interior
+    counts = {str(n): int(sum(1 for m, _ in blocks if m == n)) for n in (16, 32, 64)}
+    out = {"workload": f"{len(blocks)} FMKS blocks {counts} (interior {interior} zones) of a "
+                       "magnetized FM torus, a=0.9375, Gamma=5/3, torus-deck floors/ceilings, "
+                       "WENO5-Z + HLL + CT + sources + update + con2prim (tol 1e-10), full RK2 "
+                       "steps, device-resident dt",
+           "zone_updates_per_s_graph": zu / (ms_graph * 1e-3),
+           "zone_updates_per_s_eager_pack": zu / (ms_eager * 1e-3),
+           "zone_updates_per_s_per_block_launches": zu / (ms_blocks * 1e-3) if ms_blocks else None,
+           "ms_per_rk2_step_graph": ms_graph, "ms_per_rk2_step_eager_pack": ms_eager,
+           "ms_per_rk2_step_per_block_launches": ms_blocks,
+           "hbm_frac_stage_bytes": zu / (ms_graph * 1e-3) * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS,
+           "flux_tiles_yz": tiles, "con2prim_mean_iters": float(its.mean()),
+           "con2prim_fail_zones_interior": {"setup": fails_setup, f"after_{steps + 1}_steps": fail_graph},
+           "steps": steps}
+    log(f"c4: graph {ms_graph:.3f} ms/step, eager {ms_eager:.3f}, per-block {ms_blocks}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +413,10 @@ def main():
     ap.add_argument("--c3-states", type=int, default=1 << 26)
     ap.add_argument("--c3-pairs", type=int, default=1 << 22)
     ap.add_argument("--c3-train-steps", type=int, default=30000)
+    ap.add_argument("--no-c4", action="store_true", help="skip the uneven block pack leg")
+    ap.add_argument("--c4-blocks", type=int, default=512)
+    ap.add_argument("--c4-steps", type=int, default=5)
+    ap.add_argument("--c4-per-block", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
@@ -375,8 +555,14 @@ def main():
                      "fail_count": int(np.sum(st.fail.get()[I] == 0.0)),
                      "mlp_guess": None},
     }
+    if not args.no_c4:
+        if "st" in locals():
+            del st, u_out, prim_out, iters
+        out["uneven_pack"] = c4_uneven_pack(
+            phx, args, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_c3:
-        del st, u_out, prim_out, iters
+        if "st" in locals():
+            del st, u_out, prim_out, iters
         out["con2prim"]["mlp_guess"] = c3_learned_vs_plain(
             phx, args, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:

@@ -72,3 +72,151 @@ def shock_tube_prims(nx, ng=4, gamma=5.0 / 3.0, left=(1.0, 1.0, 0.0), right=(0.1
     prim[npr - 2, 0, 0] = P
     prim[npr - 1, 0, 0] = gamma
     return prim
+
+
+# ---- Fishbone-Moncrief torus on an FMKS grid (the torus workloads) ----------------------
+# Fishbone & Moncrief, ApJ 207 (1976) 962, in the form HARM-family codes use
+# (torus.cpp:744-790 restates the same: lfish_calc, log_enthalpy; u^phi in Boyer-Lindquist).
+TORUS = dict(a=0.9375, gamma=5.0 / 3.0, rin=6.0, rmax=12.0, kappa=0.01, beta=100.0,
+             jitter=0.04, hslope=0.3, r_outer=250.0, ninside=5)
+
+
+def fm_angular_momentum(a, r):
+    sr = np.sqrt(r)
+    return (((a * a - 2.0 * a * sr + r * r) *
+             ((-2.0 * a * r * (a * a - 2.0 * a * sr + r * r)) /
+              np.sqrt(2.0 * a * sr + (-3.0 + r) * r) +
+              ((a + (-2.0 + r) * sr) * (r ** 3 + a * a * (2.0 + r))) /
+              np.sqrt(1 + (2.0 * a) / r ** 1.5 - 3.0 / r))) /
+            (r ** 3 * np.sqrt(2.0 * a * sr + (-3.0 + r) * r) * (a * a + (-2.0 + r) * r)))
+
+
+def _fm_terms(r, th, a, l):
+    sth, cth = np.sin(th), np.cos(th)
+    DD = r * r - 2.0 * r + a * a
+    AA = (r * r + a * a) ** 2 - DD * a * a * sth * sth
+    SS = r * r + a * a * cth * cth
+    q = np.sqrt(1.0 + 4.0 * (l * l * SS * SS) * DD / (AA * AA * sth * sth))
+    return 0.5 * np.log((1.0 + q) / (SS * DD / AA)) - 0.5 * q - 2.0 * a * r * l / AA, (DD, AA, SS, sth)
+
+
+def fm_log_enthalpy(r, th, a, rin, l):
+    lnh, _ = _fm_terms(r, th, a, l)
+    lin, _ = _fm_terms(np.asarray(rin, float), np.asarray(0.5 * np.pi), a, l)
+    return lnh - lin
+
+
+def fm_uphi(r, th, a, l):
+    _, (DD, AA, SS, sth) = _fm_terms(r, th, a, l)
+    expm2chi = SS * SS * DD / (AA * AA * sth * sth)
+    up1 = np.sqrt((-1.0 + np.sqrt(1.0 + 4.0 * l * l * expm2chi)) / 2.0)
+    return 2.0 * a * r * np.sqrt(1.0 + up1 * up1) / np.sqrt(AA * SS * DD) + np.sqrt(SS / AA) * up1 / sth
+
+
+def torus_grid_geometry(nx=(256, 128, 128), a=TORUS["a"], r_outer=TORUS["r_outer"],
+                        ninside=TORUS["ninside"]):
+    """x1min, dx of the torus grid: x1max = log r_outer and ninside zones inside the
+    horizon (torus.cpp:477-497)."""
+    x1max = np.log(r_outer)
+    xh = np.log(1.0 + np.sqrt(1.0 - a * a))
+    dx1 = (x1max - xh) / (nx[0] - ninside)
+    x1min = xh - (ninside + 0.5) * dx1
+    return (x1min, 0.0, 0.0), ((x1max - x1min) / nx[0], 1.0 / nx[1], 2.0 * np.pi / nx[2])
+
+
+def torus_grid(rng, nx=(256, 128, 128), ng=4, **kw):
+    """Primitives [10][nz+2ng][ny+2ng][nx+2ng] of a magnetized Fishbone-Moncrief torus on an
+    FMKS grid (derefine_poles off, hslope 0.3): rho normalized to 1 at r_max, atmosphere
+    rho = 1e-5 r^-1.5, u = 1e-7 r^-2.5 at rest, u jittered by 1 + jitter (U - 1/2), a single
+    field loop A_phi = max(rho - 0.2, 0) scaled so that max P_gas / max P_mag = beta
+    (P_mag = gamma_ij B^i B^j / 2).  Returns (prim, xmin, dx) with xmin of the first
+    interior cell face."""
+    P = dict(TORUS, **kw)
+    a, gam, rin, rmax, kappa = P["a"], P["gamma"], P["rin"], P["rmax"], P["kappa"]
+    xmin, dx = torus_grid_geometry(nx, a, P["r_outer"], P["ninside"])
+    # cell centres including ghosts; the flow is axisymmetric, so compute one phi slice
+    X1 = xmin[0] + (np.arange(nx[0] + 2 * ng) - ng + 0.5) * dx[0]
+    X2 = xmin[1] + (np.arange(nx[1] + 2 * ng) - ng + 0.5) * dx[1]
+    x1, x2 = np.meshgrid(X1, X2, indexing="xy")  # [j][i]
+    r = np.exp(x1)
+    th = np.pi * x2 + ((1.0 - P["hslope"]) / 2.0) * np.sin(2.0 * np.pi * x2)
+    th = np.clip(th, 1e-10, np.pi - 1e-10)
+    dthdx2 = np.pi + np.pi * (1.0 - P["hslope"]) * np.cos(2.0 * np.pi * x2)
+    l = fm_angular_momentum(a, rmax)
+    with np.errstate(all="ignore"):
+        lnh = np.where(r > rin, fm_log_enthalpy(r, th, a, rin, l), -1.0)
+    inside = lnh > 0.0
+    hm1 = np.where(inside, np.exp(np.where(inside, lnh, 0.0)) - 1.0, 0.0)
+    rho = (hm1 * (gam - 1.0) / (kappa * gam)) ** (1.0 / (gam - 1.0))
+    hm1_max = np.exp(fm_log_enthalpy(np.asarray(rmax), np.asarray(0.5 * np.pi), a, rin, l)) - 1.0
+    rho_max = (hm1_max * (gam - 1.0) / (kappa * gam)) ** (1.0 / (gam - 1.0))
+    rho = rho / rho_max
+    u = kappa * (rho * rho_max) ** gam / (gam - 1.0) / rho_max
+    # Kerr-Schild metric pieces at (r, theta)
+    cth, sth = np.cos(th), np.sin(th)
+    rho2 = r * r + a * a * cth * cth
+    g00 = -1.0 + 2.0 * r / rho2
+    g03 = -2.0 * a * r * sth * sth / rho2
+    g33 = (rho2 + a * a * (1.0 + 2.0 * r / rho2) * sth * sth) * sth * sth
+    g11 = 1.0 + 2.0 * r / rho2
+    alpha = np.sqrt(rho2 / (rho2 + 2.0 * r))
+    betar = 2.0 * r / (rho2 + 2.0 * r)
+    # torus 4-velocity: u^r = u^th = 0 (Boyer-Lindquist and Kerr-Schild agree), u^t from
+    # the normalization, W v^i = u^i + u^t beta^i in FMKS (beta^1 = beta^r / r)
+    with np.errstate(all="ignore"):
+        up = np.where(inside, fm_uphi(r, th, a, l), 0.0)
+        A_ = g00
+        B_ = 2.0 * g03 * up
+        C_ = g33 * up * up + 1.0
+        ut = np.where(inside, (-B_ - np.sqrt(B_ * B_ - 4.0 * A_ * C_)) / (2.0 * A_), 1.0 / alpha)
+    v1 = np.where(inside, ut * betar / r, 0.0)
+    v3 = np.where(inside, up, 0.0)
+    rho_atm, u_atm = 1e-5 * r ** -1.5, 1e-7 * r ** -2.5
+    rho = np.where(inside, np.maximum(rho, rho_atm), rho_atm)
+    u = np.where(inside, np.maximum(u, u_atm), u_atm)
+    # single loop: A_phi at cell centres, B^1 = -dA/dX2 / sqrt(gamma), B^2 = dA/dX1 / sqrt(gamma)
+    Aphi = np.maximum(np.where(inside, rho, 0.0) - 0.2, 0.0)
+    sg = np.sqrt(rho2 * (rho2 + 2.0 * r)) * sth * (r * dthdx2)  # sqrt(gamma_KS) * det J
+    dA2 = np.gradient(Aphi, dx[1], axis=0)
+    dA1 = np.gradient(Aphi, dx[0], axis=1)
+    B1, B2 = -dA2 / sg, dA1 / sg
+    gam11 = g11 * r * r  # gamma_{X1 X1} = g_rr (dr/dX1)^2
+    gam22 = rho2 * dthdx2 * dthdx2
+    pmag = 0.5 * (gam11 * B1 * B1 + gam22 * B2 * B2)
+    pgas = (gam - 1.0) * u
+    if pmag.max() > 0:
+        s = np.sqrt(pgas.max() / pmag.max() / P["beta"])
+        B1, B2 = B1 * s, B2 * s
+    nz = nx[2] + 2 * ng
+    prim = np.empty((10, nz) + rho.shape)
+    prim[0] = rho
+    prim[1] = v1
+    prim[2] = 0.0
+    prim[3] = v3
+    prim[5] = B1
+    prim[6] = B2
+    prim[7] = 0.0
+    prim[4] = u * (1.0 + P["jitter"] * (rng.uniform(size=prim[4].shape) - 0.5))
+    prim[8] = (gam - 1.0) * prim[4]
+    prim[9] = gam
+    return prim, xmin, dx
+
+
+def uneven_blocks(rng, nblocks=512, sizes=(16, 32, 64), probs=(0.5, 0.35, 0.15),
+                  grid=(256, 128, 128), centre=None):
+    """Block sizes drawn from sizes with probs, and origins (interior index of the first
+    cell) in the grid, the small blocks drawn near the torus centre (an AMR-like mix:
+    finest blocks where rho is largest).  Blocks may overlap: each has its own fields."""
+    s = rng.choice(np.asarray(sizes), size=nblocks, p=np.asarray(probs))
+    ci = centre if centre is not None else (grid[0] // 2, grid[1] // 2, grid[2] // 2)
+    out = []
+    for n in s:
+        spread = {16: 0.15, 32: 0.3}.get(int(n), 0.5)
+        o = []
+        for d in range(3):
+            lo, hi = 0, grid[d] - n
+            mid = ci[d] - n // 2
+            v = int(np.clip(round(mid + rng.normal() * spread * grid[d]), lo, hi))
+            o.append(v)
+        out.append((int(n), tuple(o)))
+    return out
