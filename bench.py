@@ -84,7 +84,8 @@ def cpu_baseline(prim, cons, mu0, args):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
-    op = O.default_params(gamma=GAMMA, mhd=True, recon=O.RECON_WENO5, riemann=O.RIEMANN_HLL,
+    orecon = {"weno5z": O.RECON_WENO5, "plm": O.RECON_LINEAR, "mp5": O.RECON_MP5}[args.recon]
+    op = O.default_params(gamma=GAMMA, mhd=True, recon=orecon, riemann=O.RIEMANN_HLL,
                           tol=1e-10, max_iter=100)
     n = args.cpu_nx
     sl = slice(0, n + 2 * NG)
@@ -405,6 +406,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=NX)
+    ap.add_argument("--recon", choices=("weno5z", "plm", "mp5"), default="weno5z")
     ap.add_argument("--cpu-nx", type=int, default=NX)
     ap.add_argument("--cpu-stages", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -431,7 +433,8 @@ def main():
     rng = np.random.default_rng(SEED)
     prim_h = make_input(rng, nx, NG)
     phx = Phx(0)
-    p = fluid_params(gamma=GAMMA, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
+    recon = {"weno5z": A.RECON_WENO5Z, "plm": A.RECON_LINEAR, "mp5": A.RECON_MP5}[args.recon]
+    p = fluid_params(gamma=GAMMA, mhd=True, recon=recon, riemann=A.RIEMANN_HLL,
                      c2p_tol=1e-10, c2p_max_iter=100)
     b = block(3, (nx, nx, nx), NG, dx=(1.0 / nx,) * 3)
     st = BlockState(p, b)
@@ -516,7 +519,7 @@ def main():
     # workload, scaled by the same counters on a copy of known size
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_traffic.json")
-    if nx == 128 and os.path.exists(tfile):
+    if nx == 128 and args.recon == "weno5z" and os.path.exists(tfile):
         tk = json.load(open(tfile))["kernels"].get(dom)
         traffic = tk["traffic_bytes"] if tk else None
     warm_iters = iters.get()[I]
@@ -536,9 +539,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded; see bench.py docstring)",
         "config": {"workload": f"single 3-D GRMHD meshblock {nx}^3 + {NG} ghosts, Minkowski, "
-                               "Gamma=13/9, WENO5-Z + HLL + FluxCT + divergence/RK update + "
-                               "robust con2prim (tol 1e-10) + CFL, one RK stage",
-                   "zones": nx ** 3, "recon": "weno5z", "riemann": "hll",
+                               f"Gamma=13/9, {args.recon.upper()} + HLL + FluxCT + divergence/RK "
+                               "update + robust con2prim (tol 1e-10) + CFL, one RK stage",
+                   "zones": nx ** 3, "recon": args.recon, "riemann": "hll",
                    "flux_tiles_yz": tiles},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -278,6 +278,21 @@ struct KSDerivs {
   double gi[4][4];
 };
 
+// inverse spacetime metric of spherical Kerr-Schild (spherical_kerr_schild.hpp:86-102)
+__device__ void ks_gcon4(double a, double X1, double X2, double gi[4][4]) {
+  const double a2 = a * a;
+  const double r = fabs(X1), cth = cos(X2), sth = sin(X2);
+  const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y) gi[x][y] = 0.0;
+  gi[0][1] = gi[1][0] = ratio(2 * r, rho2);
+  gi[0][0] = -1 - gi[0][1];
+  gi[1][1] = ratio(-2 * r + rho2 + a2 * sth2, rho2);
+  gi[1][3] = gi[3][1] = ratio(a, rho2);
+  gi[2][2] = ratio(1, rho2);
+  gi[3][3] = ratio(1, rho2 * sth2);
+}
+
 __device__ void ks_derivs(double a, double X1, double X2, KSDerivs &o) {
   const double a2 = a * a;
   const double r = fabs(X1), th = X2, cth = cos(th), sth = sin(th);
@@ -307,12 +322,7 @@ __device__ void ks_derivs(double a, double X1, double X2, KSDerivs &o) {
   o.da[0] = o.da[3] = 0.0;
   o.da[1] = ratio(r, rho2) - ratio(1 + r, 2 * r + rho2);
   o.da[2] = -ratio(2 * a2 * r * cth * sth, 2 * r * rho2 + rho4);
-  o.gi[0][1] = o.gi[1][0] = ratio(2 * r, rho2);
-  o.gi[0][0] = -1 - o.gi[0][1];
-  o.gi[1][1] = ratio(-2 * r + rho2 + a2 * sth2, rho2);
-  o.gi[1][3] = o.gi[3][1] = ratio(a, rho2);
-  o.gi[2][2] = ratio(1, rho2);
-  o.gi[3][3] = ratio(1, rho2 * sth2);
+  ks_gcon4(a, X1, X2, o.gi);
 }
 
 // FMKS derivatives by central differences (geometry_utils.hpp:94-187): step
@@ -326,9 +336,11 @@ __device__ void fmks_derivs(const DevParams &p, double X1, double X2, double X3,
       o.gi[x][y] = 0.0;
       for (int z = 0; z < 4; ++z) o.dg[x][y][z] = 0.0;
     }
-  o.da[0] = 0.0;
+  o.da[0] = o.da[3] = 0.0;
   const double X[3] = {X1, X2, X3};
-  for (int d = 1; d < 4; ++d) {
+  // the FMKS metric does not depend on X3 (FMKSMetric ignores it), so the X3 differences
+  // are exactly zero: d < 3 gives the same numbers without evaluating those two points
+  for (int d = 1; d < 3; ++d) {
     double L[3] = {X1, X2, X3}, R[3] = {X1, X2, X3};
     const double dx = smax(fabs(X[d - 1]), 1.0) * p.fd_eps;
     L[d - 1] -= dx;
@@ -342,7 +354,7 @@ __device__ void fmks_derivs(const DevParams &p, double X1, double X2, double X3,
   }
   const MgrMap m(p.mgr, X1, X2);
   KSDerivs k;
-  ks_derivs(p.a, m.r, m.th, k);  // only k.gi is used
+  ks_gcon4(p.a, m.r, m.th, k.gi);
   double SJ[4][4] = {};
   SJ[0][0] = 1.0;
   SJ[1][1] = ratio(1., m.r1);
