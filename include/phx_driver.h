@@ -6,8 +6,10 @@
  * fluid::CalculateFluidSourceTerms (curved geometry only, phoebus_driver.cpp:323),
  * fluid::CalculateFluxes + fluid::FluxCT (:344-345), FluxDivergence + SumData +
  * AverageIndependentData + UpdateIndependentData (:560-576), FillDerived con2prim over
- * the entire block (:589), physical boundaries (:662) and FillDerived again (:667), and
- * EstimateTimestep after the last stage (:719).  Stage containers follow parthenon's
+ * the entire block (:589), physical boundaries (:662) and FillDerived again (:667), each
+ * FillDerived followed by fixup::ConservedToPrimitiveFixup and fixup::ApplyFloors (:592-597,
+ * :671-676; no-ops unless phx_driver_set_fixup enabled them), and EstimateTimestep after
+ * the last stage (:719).  Stage containers follow parthenon's
  * LowStorageIntegrator for rk2: "base" and "1", beta = {1, 1/2}.  The timestep grows by at
  * most 2x per step and the last step is clipped to tlim.
  *
@@ -21,6 +23,7 @@
 #include <stdint.h>
 
 #include "phx.h"
+#include "phx_fixup.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -39,9 +42,11 @@ enum { PHX_DRIVER_PRIM = 0, PHX_DRIVER_CONS = 1, PHX_DRIVER_FAIL = 2, PHX_DRIVER
 int phx_driver_create(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
                       const int bc[6], double cfl, double dt_init_fact, phx_driver **out);
 int phx_driver_destroy(phx_driver *d);
+/* the fixup package's switches (default: all off, the fixup.cpp:63-74 defaults) */
+int phx_driver_set_fixup(phx_driver *d, const phx_fixup_params *fx);
 /* host primitives [nprim][ncell] over the entire block */
 int phx_driver_set_prim(phx_driver *d, const double *host_prim);
-/* p2c, boundaries, con2prim over the entire block, first timestep
+/* p2c, floors, boundaries, con2prim over the entire block, first timestep
  * (PostInitializationCommunication, phoebus_driver.cpp:147-202) */
 int phx_driver_initialize(phx_driver *d);
 /* one RK2 step of size dt (clipped to the tlim of the last execute call) */

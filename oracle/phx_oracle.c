@@ -1319,3 +1319,230 @@ void orc_p2c_batch(const orc_params *p, long n, const double *coords, const doub
     for (int v = 0; v < ncn; ++v) cons[v * n + s] = cs[v];
   }
 }
+
+/* ---- fixups: fixup_c2p.cpp:44-356 and fixup.cpp:285-530 ------------------------------- */
+static inline double eos_bmod_rho_sie(const orc_params *p, double rho, double sie) {
+  return (p->gm1 + 1.0) * p->gm1 * rho * sie;
+}
+
+/* effective switches after fixup::Initialize's forcing rules (fixup.cpp:102-135) */
+static void fixup_flags(const orc_params *p, const orc_fixup *fx, int *c2p_fixup, int *floors,
+                        int *mhd_floors) {
+  *c2p_fixup = fx->enable_c2p_fixup;
+  *mhd_floors = fx->enable_mhd_floors && p->mhd;
+  *floors = fx->enable_floors || *c2p_fixup || *mhd_floors;
+}
+
+/* ConservedToPrimitiveFixupImpl (fixup_c2p.cpp:44-356) for the fluid, interior cells.
+ * Failed cells take the average of their valid face neighbours' rho, v, u (or the
+ * near-vacuum state), the gamma ceiling with the reference's rescale
+ * sqrt(gmax^2 - 1) / (W^2 - 1) (:269-275), then T, P, Gamma_1 and cons (not B) and
+ * the cell signal speeds by p2c.  A neighbour that is itself a failed interior cell
+ * contributes exactly 0: its weight fail = 0 multiplies a value that the same pass
+ * rewrites, which the reference reads racily (only non-finite values make it differ). */
+void orc_c2p_fixup_block(const orc_params *p, const orc_fixup *fx, const orc_block *b,
+                         double *prim, double *cons, double *temp, double *cell_sig,
+                         const double *fail) {
+  int c2p_fixup, floors, mhd_floors;
+  fixup_flags(p, fx, &c2p_fixup, &floors, &mhd_floors);
+  if (!c2p_fixup) return; /* :120-122 */
+  int n[3];
+  orc_block_dims(b, n);
+  const long nc = (long)n[0] * n[1] * n[2];
+  const int mhd = p->mhd, prs = mhd ? 8 : 5, gm1i = mhd ? 9 : 6;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = d < b->ndim ? b->ng : 0;
+    hi[d] = lo[d] + b->nx[d] - 1;
+  }
+  const long st[3] = {1, n[0], (long)n[0] * n[1]};
+  const int pv[5] = {0, 1, 2, 3, 4}; /* rho, v1..3, u (:229-234) */
+#pragma omp parallel for collapse(2)
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        const long c = ((long)k * n[1] + j) * n[0] + i;
+        if (fail[c] != 0.0) continue; /* FailFlags::fail == 0 */
+        const int idx[3] = {i, j, k};
+        /* neighbour (d, side) at q = 2d + side: weight = its fail flag; a failed
+         * interior neighbour is skipped (it is rewritten by this same pass) */
+        double w[6] = {0, 0, 0, 0, 0, 0};
+        long nb[6] = {c, c, c, c, c, c};
+        int skip[6] = {1, 1, 1, 1, 1, 1};
+        for (int d = 0; d < b->ndim; ++d)
+          for (int s = 0; s < 2; ++s) {
+            const int q = 2 * d + s, ni = idx[d] + (s == 0 ? -1 : 1);
+            nb[q] = c + (s == 0 ? -st[d] : st[d]);
+            w[q] = fail[nb[q]];
+            skip[q] = w[q] == 0.0 && ni >= lo[d] && ni <= hi[d];
+          }
+        double num_valid = w[0] + w[1]; /* :218-221 */
+        if (b->ndim > 1) num_valid += w[2] + w[3];
+        if (b->ndim == 3) num_valid += w[4] + w[5];
+        double pr[10];
+        for (int v = 0; v < orc_nprim(mhd); ++v) pr[v] = prim[v * nc + c];
+        if (num_valid > 0.5 && fx->failure_strategy == ORC_FAILURE_INTERPOLATE) {
+          const double norm = 1.0 / num_valid;
+          for (int m = 0; m < 5; ++m) {
+            const int iv = pv[m];
+            double term[6];
+            for (int q = 0; q < 6; ++q) term[q] = skip[q] ? 0.0 : w[q] * prim[iv * nc + nb[q]];
+            double acc = term[0] + term[1];
+            if (b->ndim > 1) {
+              acc += term[2] + term[3];
+              if (b->ndim == 3) acc += term[4] + term[5];
+            }
+            pr[iv] = norm * acc;
+          }
+          pr[0] = smax(pr[0], 100. * ORC_SMALL);
+          pr[4] = smax(pr[4], 100. * ORC_SMALL);
+        } else {
+          pr[0] = 100. * ORC_SMALL;
+          pr[4] = 100. * ORC_SMALL;
+          for (int m = 0; m < 3; ++m) pr[1 + m] = 0.;
+        }
+        const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+        orc_point_geom g;
+        orc_geom_point(&p->geom, X[0], X[1], X[2], &g);
+        const double gamma_max = p->bounds.gam_max;
+        double vpcon[3] = {pr[1], pr[2], pr[3]};
+        double W = lorentz(vpcon, g.gcov);
+        if (W > gamma_max) { /* :269-275 */
+          const double rescale = sqrt(gamma_max * gamma_max - 1.) / (W * W - 1.);
+          for (int m = 0; m < 3; ++m) vpcon[m] *= rescale;
+          for (int m = 0; m < 3; ++m) pr[1 + m] = vpcon[m];
+          W = gamma_max;
+        }
+        const double t = eos_t_rho_sie(p, pr[0], orc_ratio(pr[4], pr[0])); /* :277-287 */
+        pr[prs] = eos_p_rho_t(p, pr[0], t);
+        pr[gm1i] = orc_ratio(eos_bmod_rho_t(p, pr[0], t), pr[prs]);
+        double bp[3] = {0.0, 0.0, 0.0};
+        if (mhd)
+          for (int m = 0; m < 3; ++m) bp[m] = pr[5 + m];
+        double D, S[3], bc[3], tau, sig[3];
+        p2c_core(pr[0], vpcon, bp, pr[4], pr[prs], pr[gm1i], g.gcov, g.gcon, g.beta, g.alpha,
+                 g.detgamma, &D, S, bc, &tau, sig); /* :290-312 */
+        for (int v = 0; v < orc_nprim(mhd); ++v) prim[v * nc + c] = pr[v];
+        if (temp) temp[c] = t;
+        cons[c] = D;
+        for (int m = 0; m < 3; ++m) cons[(1 + m) * nc + c] = S[m];
+        cons[4 * nc + c] = tau;
+        for (int d = 0; d < b->ndim; ++d) cell_sig[d * nc + c] = sig[d];
+      }
+}
+
+/* ApplyFloorsImpl (fixup.cpp:285-530) for the fluid over cells lo..hi (the reference's
+ * default domain is the entire block).  Density/energy floors (and the MHD b^2 ceilings)
+ * are injected as a fluid at rest in the normal-observer frame, added to the conserved
+ * state and re-inverted by the robust con2prim (falling back to the floor state when it
+ * fails); then the gamma / sie ceilings with the rescale quirk of :510; then T, P,
+ * Gamma_1.  mu and cell_sig are the con2prim warm start and outputs. */
+void orc_apply_floors_block(const orc_params *p, const orc_fixup *fx, const orc_block *b,
+                            const int lo[3], const int hi[3], double *prim, double *cons,
+                            double *temp, double *mu, double *cell_sig) {
+  int c2p_fixup, floors, mhd_floors;
+  fixup_flags(p, fx, &c2p_fixup, &floors, &mhd_floors);
+  if (!floors) return; /* :309 */
+  int n[3];
+  orc_block_dims(b, n);
+  const long nc = (long)n[0] * n[1] * n[2];
+  const int mhd = p->mhd, prs = mhd ? 8 : 5, gm1i = mhd ? 9 : 6;
+  const int np = orc_nprim(mhd), ncn = orc_ncons(mhd);
+#pragma omp parallel for collapse(2)
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        const long c = ((long)k * n[1] + j) * n[0] + i;
+        const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+        double rho_floor, sie_floor;
+        orc_get_floors(&p->bounds, X[0], X[1], X[2], &rho_floor, &sie_floor);
+        const double gamma_max = p->bounds.gam_max, e_max = p->bounds.sie_max;
+        const double bsqorho_max = fx->bsqorho_max, bsqou_max = fx->bsqou_max;
+        double rho_floor_max = rho_floor;
+        double u_floor_max = rho_floor * sie_floor;
+        int floor_applied = 0, ceiling_applied = 0;
+        orc_point_geom g;
+        orc_geom_point(&p->geom, X[0], X[1], X[2], &g);
+        double pr[10], cs[8];
+        for (int v = 0; v < np; ++v) pr[v] = prim[v * nc + c];
+        for (int v = 0; v < ncn; ++v) cs[v] = cons[v * nc + c];
+        double t = temp ? temp[c] : 0.0;
+        if (mhd_floors) { /* :411-428 */
+          double Bsq = 0.0, Bdotv = 0.0;
+          const double vp[3] = {pr[1], pr[2], pr[3]};
+          const double bp[3] = {pr[5], pr[6], pr[7]};
+          const double W = lorentz(vp, g.gcov);
+          const double iW = 1.0 / W;
+          for (int ii = 0; ii < 3; ++ii)
+            for (int jj = 0; jj < 3; ++jj) {
+              Bsq += g.gcov[ii + 1][jj + 1] * bp[ii] * bp[jj];
+              Bdotv += g.gcov[ii + 1][jj + 1] * bp[ii] * vp[jj];
+            }
+          const double bcon0 = W * Bdotv / g.alpha;
+          const double bsq = (Bsq + g.alpha * g.alpha * bcon0 * bcon0) * iW * iW;
+          rho_floor_max = smax(rho_floor_max, bsq / bsqorho_max);
+          u_floor_max = smax(u_floor_max, bsq / bsqou_max);
+          rho_floor_max = smax(rho_floor_max, smax(pr[4], u_floor_max) / e_max);
+        }
+        double drho = rho_floor_max - pr[0];
+        double du = u_floor_max - pr[4];
+        if (drho > 0. || du > 0.) { /* :434-440 */
+          floor_applied = 1;
+          drho = smax(drho, du / sie_floor);
+          du = smax(du, sie_floor * drho);
+        }
+        double bp[3] = {0.0, 0.0, 0.0};
+        if (mhd)
+          for (int m = 0; m < 3; ++m) bp[m] = pr[5 + m];
+        double dS[3], dB[3];
+        if (floor_applied) { /* :448-498 */
+          const double vzero[3] = {0.0, 0.0, 0.0};
+          const double dprs = eos_p_rho_sie(p, drho, orc_ratio(du, drho));
+          const double dgm1 = orc_ratio(eos_bmod_rho_sie(p, drho, orc_ratio(du, drho)), dprs);
+          double dcrho, dtau;
+          p2c_core(drho, vzero, bp, du, dprs, dgm1, g.gcov, g.gcon, g.beta, g.alpha,
+                   g.detgamma, &dcrho, dS, dB, &dtau, NULL);
+          cs[0] += dcrho;
+          for (int m = 0; m < 3; ++m) cs[1 + m] += dS[m];
+          cs[4] += dtau;
+          double sig[3], m_io = mu[c];
+          for (int d = 0; d < 3; ++d) sig[d] = d < b->ndim ? cell_sig[d * nc + c] : 0.0;
+          int it = 0;
+          const double status = orc_c2p(p, &g, X, cs, pr, &t, &m_io, sig, &it);
+          mu[c] = m_io;
+          for (int d = 0; d < b->ndim; ++d) cell_sig[d * nc + c] = sig[d];
+          if (status == 0.0) { /* :473-497 */
+            pr[0] = drho;
+            for (int m = 0; m < 3; ++m) pr[1 + m] = vzero[m];
+            pr[4] = du;
+            t = eos_t_rho_sie(p, pr[0], pr[4] / pr[0]);
+            double tau;
+            p2c_core(drho, vzero, bp, du, dprs, dgm1, g.gcov, g.gcon, g.beta, g.alpha,
+                     g.detgamma, &cs[0], dS, dB, &tau, NULL);
+            cs[4] = tau;
+            for (int m = 0; m < 3; ++m) cs[1 + m] = dS[m];
+          }
+        }
+        double vpcon[3] = {pr[1], pr[2], pr[3]}; /* :501-507 */
+        const double W = lorentz(vpcon, g.gcov);
+        if (W > gamma_max || pr[4] / pr[0] > e_max) ceiling_applied = 1;
+        if (ceiling_applied) { /* :509-526 */
+          const double rescale = sqrt(gamma_max * gamma_max - 1.) / (W * W - 1.);
+          for (int m = 0; m < 3; ++m) vpcon[m] *= rescale;
+          for (int m = 0; m < 3; ++m) pr[1 + m] = vpcon[m];
+          double tau;
+          p2c_core(pr[0], vpcon, bp, pr[4], pr[prs], pr[gm1i], g.gcov, g.gcon, g.beta,
+                   g.alpha, g.detgamma, &cs[0], dS, dB, &tau, NULL);
+          cs[4] = tau;
+          for (int m = 0; m < 3; ++m) cs[1 + m] = dS[m];
+        }
+        if (floor_applied || ceiling_applied) { /* :528-539 */
+          t = eos_t_rho_sie(p, pr[0], orc_ratio(pr[4], pr[0]));
+          pr[prs] = eos_p_rho_t(p, pr[0], t);
+          pr[gm1i] = orc_ratio(eos_bmod_rho_t(p, pr[0], t), pr[prs]);
+          for (int v = 0; v < np; ++v) prim[v * nc + c] = pr[v];
+          for (int v = 0; v < ncn; ++v) cons[v * nc + c] = cs[v];
+          if (temp) temp[c] = t;
+        }
+      }
+}

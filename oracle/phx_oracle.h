@@ -195,6 +195,25 @@ void orc_c2p_batch(const orc_params *p, long n, const double *coords /* [2][n] *
 void orc_p2c_batch(const orc_params *p, long n, const double *coords, const double *prim,
                    double *cons);
 
+/* ---- fixups (fixup_c2p.cpp:44-356, fixup.cpp:285-530) -------------------------------
+ * Switches as the fixup package reads them (fixup.cpp:63-135, 217-243); enable_floors is
+ * forced on by enable_c2p_fixup or enable_mhd_floors, and MHD floors need MHD. */
+enum { ORC_FAILURE_INTERPOLATE = 0, ORC_FAILURE_FLOORS = 1 };
+typedef struct {
+  int enable_c2p_fixup, failure_strategy;
+  int enable_floors, enable_mhd_floors;
+  double bsqorho_max, bsqou_max; /* MHDCeilings ConstantBsqRat */
+} orc_fixup;
+
+/* interior cells; fail is the con2prim flag field */
+void orc_c2p_fixup_block(const orc_params *p, const orc_fixup *fx, const orc_block *b,
+                         double *prim, double *cons, double *temp, double *cell_sig,
+                         const double *fail);
+/* cells lo..hi (inclusive, i j k) */
+void orc_apply_floors_block(const orc_params *p, const orc_fixup *fx, const orc_block *b,
+                            const int lo[3], const int hi[3], double *prim, double *cons,
+                            double *temp, double *mu, double *cell_sig);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -1,5 +1,5 @@
 """ctypes binding of libphx.so (include/phx.h, phx_runtime.h, phx_mlp.h, phx_driver.h,
-phx_pack.h).
+phx_pack.h, phx_fixup.h).
 
 The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading fails loudly
 when it has not been built; there is no CPU fallback.
@@ -17,6 +17,7 @@ RECON_CONSTANT, RECON_LINEAR, RECON_WENO5Z, RECON_MP5 = 0, 1, 2, 3
 RIEMANN_LLF, RIEMANN_HLL = 0, 1
 BC_OUTFLOW, BC_PERIODIC = 0, 1
 DRIVER_PRIM, DRIVER_CONS, DRIVER_FAIL, DRIVER_C2P_MU = 0, 1, 2, 3
+FAILURE_INTERPOLATE, FAILURE_FLOORS = 0, 1
 (FLOOR_CONSTANT_RHO_SIE, FLOOR_EXPX1_RHO_SIE, FLOOR_EXPX1_RHO_U, FLOOR_X1_RHO_SIE,
  FLOOR_R_RHO_SIE) = range(5)
 
@@ -62,6 +63,12 @@ class StageArgs(C.Structure):
     _fields_ = [("beta", C.c_double), ("dt", C.c_double), ("u_base", C.c_void_p),
                 ("u_out", C.c_void_p), ("prim_out", C.c_void_p), ("iters", C.c_void_p),
                 ("cfl", C.c_double), ("dt_out", C.c_void_p)]
+
+
+class FixupParams(C.Structure):
+    _fields_ = [("enable_c2p_fixup", C.c_int), ("failure_strategy", C.c_int),
+                ("enable_floors", C.c_int), ("enable_mhd_floors", C.c_int),
+                ("bsqorho_max", C.c_double), ("bsqou_max", C.c_double)]
 
 
 class PackBlock(C.Structure):
@@ -120,6 +127,9 @@ def lib():
         "phx_mlp_train_step": (i32, [vp, vp, C.c_int64, vp, vp, C.c_float, C.c_float, vp, vp]),
         "phx_con2prim_batch_learned": (i32, [vp, P(FluidParams), vp, C.c_int64, vp, vp, vp, vp, vp,
                                              vp, vp, vp, vp]),
+        "phx_c2p_fixup": (i32, [vp, P(FluidParams), P(FixupParams), P(Block), P(BlockFields), vp]),
+        "phx_apply_floors": (i32, [vp, P(FluidParams), P(FixupParams), P(Block), P(BlockFields),
+                                   P(i32), P(i32), vp]),
         "phx_autotune": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), i32, vp]),
         "phx_tile_shape": (i32, [vp, P(FluidParams), P(Block), i32, P(i32), P(i32)]),
         "phx_set_tile_shape": (i32, [vp, P(FluidParams), P(Block), i32, i32, i32]),
@@ -135,6 +145,7 @@ def lib():
         "phx_driver_create": (i32, [vp, P(FluidParams), P(Block), P(i32), dbl, dbl, P(vp)]),
         "phx_driver_destroy": (i32, [vp]),
         "phx_driver_set_prim": (i32, [vp, vp]),
+        "phx_driver_set_fixup": (i32, [vp, P(FixupParams)]),
         "phx_driver_initialize": (i32, [vp]),
         "phx_driver_step": (i32, [vp]),
         "phx_driver_execute": (i32, [vp, dbl, i32]),

@@ -37,6 +37,13 @@ def fluid_params(gamma=5.0 / 3.0, cv=1.0, mhd=True, recon=A.RECON_WENO5Z,
                          A.Geometry(geom, a, mgr_params if mgr_params is not None else mgr()))
 
 
+def fixup_params(c2p_fixup=True, strategy=A.FAILURE_INTERPOLATE, floors=True, mhd_floors=False,
+                 bsqorho_max=50.0, bsqou_max=2500.0):
+    """fixup package switches (fixup.cpp:63-135, 217-243; bsq ceilings default 50, 2500)."""
+    return A.FixupParams(int(c2p_fixup), strategy, int(floors), int(mhd_floors), bsqorho_max,
+                         bsqou_max)
+
+
 def block(ndim, nx, ng=4, xmin=(0.0, 0.0, 0.0), dx=(1.0, 1.0, 1.0)):
     nxx = list(nx) + [1] * (3 - len(nx))
     return A.Block(ndim, (C.c_int * 3)(*nxx), ng, (C.c_double * 3)(*xmin),
@@ -119,6 +126,17 @@ class Phx:
         hi_ = (C.c_int * 3)(*hi) if hi is not None else None
         return self._chk(self.L.phx_con2prim(self.h, C.byref(p), C.byref(b), C.byref(f), lo_,
                                              hi_, _p(iters), stream), "phx_con2prim")
+
+    def c2p_fixup(self, p, fx, b, f, stream=None):
+        return self._chk(self.L.phx_c2p_fixup(self.h, C.byref(p), C.byref(fx), C.byref(b),
+                                              C.byref(f), stream), "phx_c2p_fixup")
+
+    def apply_floors(self, p, fx, b, f, lo=None, hi=None, stream=None):
+        lo_ = (C.c_int * 3)(*lo) if lo is not None else None
+        hi_ = (C.c_int * 3)(*hi) if hi is not None else None
+        return self._chk(self.L.phx_apply_floors(self.h, C.byref(p), C.byref(fx), C.byref(b),
+                                                 C.byref(f), lo_, hi_, stream),
+                         "phx_apply_floors")
 
     def estimate_dt(self, p, b, f, use_face_speeds, cfl, dt_out, stream=None):
         return self._chk(self.L.phx_estimate_dt(self.h, C.byref(p), C.byref(b), C.byref(f),
@@ -285,6 +303,9 @@ class Driver:
             self.close()
         except Exception:
             pass
+
+    def set_fixup(self, fx):
+        self.phx._chk(self.L.phx_driver_set_fixup(self.h, C.byref(fx)), "phx_driver_set_fixup")
 
     def set_prim(self, prim):
         prim = np.ascontiguousarray(prim, np.float64)

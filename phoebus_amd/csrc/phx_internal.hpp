@@ -9,6 +9,7 @@
 #include <type_traits>
 
 #include "phx.h"
+#include "phx_fixup.h"
 
 #include "phx_device.hpp"
 
@@ -72,6 +73,21 @@ void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevPa
 TileShape resolve_tile_shape(int d, int recon, int mhd, const BlockGeom &bg, TileShape want);
 
 DevParams make_dev_params(const phx_fluid_params *p);
+
+// a cell range lo..lo+n-1 per dimension (i, j, k)
+struct Range3 {
+  int lo[3], n[3];
+};
+// validates lo..hi (inclusive; NULL = the entire block) against the block
+int make_range(phx_ctx *ctx, const phx_block *b, const int lo[3], const int hi[3], Range3 &rg);
+
+// the fixup package's switches after fixup::Initialize's normalisation (fixup.cpp:102-135)
+struct DevFixup {
+  int c2p_fixup, strategy, floors, mhd_floors;
+  double bsqorho_max, bsqou_max;
+};
+DevFixup make_dev_fixup(const phx_fluid_params *p, const phx_fixup_params *fx);
+int check_fixup(phx_ctx *ctx, const phx_fixup_params *fx);
 int check_params(phx_ctx *ctx, const phx_fluid_params *p);
 int check_block(phx_ctx *ctx, const phx_block *b, int recon);
 
@@ -120,6 +136,12 @@ void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long 
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
                         hipStream_t s);
+// the fixups over a pack's interior cells (phx_fixup.hip): con2prim failure repair, then
+// floors and ceilings folded with the CFL partial minima (partial may be null)
+void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
+                           long groups, hipStream_t s);
+void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
+                        long groups, double *partial, hipStream_t s);
 // groups per block for each launch kind
 long pack_groups_flux(int d, int recon, const BlockGeom &bg, TileShape shape);
 long pack_groups_cells(const BlockGeom &bg, bool corners);

@@ -146,10 +146,6 @@ __global__ __launch_bounds__(256) void k_update(BlockGeom b, const double *__res
 }
 
 // ---- con2prim over a cell range (fluid.cpp:478-517) ------------------------------------
-struct Range3 {
-  int lo[3], n[3];
-};
-
 template <bool MHD, int GEOM>
 __global__ __launch_bounds__(256) void k_con2prim(DevParams p, BlockGeom b, Range3 rg,
                                                   double *__restrict__ cons,
@@ -698,6 +694,18 @@ void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hip
   hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1024), 0, s, partial, np, cfl, dt);
 }
 
+int make_range(phx_ctx *ctx, const phx_block *b, const int lo[3], const int hi[3], Range3 &rg) {
+  const BlockGeom bg = make_block_geom(b);
+  const int n[3] = {bg.ni, bg.nj, bg.nk};
+  for (int d = 0; d < 3; ++d) {
+    const int l = lo ? lo[d] : 0, h = hi ? hi[d] : n[d] - 1;
+    if (l < 0 || h >= n[d] || h < l) return ctx->fail(PHX_ERR_INVALID, "cell range outside the block");
+    rg.lo[d] = l;
+    rg.n[d] = h - l + 1;
+  }
+  return PHX_OK;
+}
+
 }  // namespace phx
 
 using namespace phx;
@@ -765,18 +773,6 @@ int phx_flux_divergence_update(phx_ctx *ctx, const phx_fluid_params *p, const ph
   return ctx->check_launch();
 }
 
-static int make_range(phx_ctx *ctx, const phx_block *b, const int lo[3], const int hi[3],
-                      Range3 &rg) {
-  const BlockGeom bg = make_block_geom(b);
-  const int n[3] = {bg.ni, bg.nj, bg.nk};
-  for (int d = 0; d < 3; ++d) {
-    const int l = lo ? lo[d] : 0, h = hi ? hi[d] : n[d] - 1;
-    if (l < 0 || h >= n[d] || h < l) return ctx->fail(PHX_ERR_INVALID, "cell range outside the block");
-    rg.lo[d] = l;
-    rg.n[d] = h - l + 1;
-  }
-  return PHX_OK;
-}
 
 int phx_con2prim(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
                  const phx_block_fields *f, const int lo[3], const int hi[3], int32_t *iters,

@@ -117,6 +117,19 @@ double EstimateTimestepBlock(MeshBlockData *rc, bool &has_face_speeds, double cf
 }
 }  // namespace fluid
 
+namespace fixup {
+TaskStatus ConservedToPrimitiveFixup(MeshBlockData *rc, int container, const phx_fixup_params &fx) {
+  const phx_block_fields f = rc->fields(container);
+  return st(phx_c2p_fixup(rc->ctx(), &rc->params(), &fx, &rc->block(), &f, nullptr));
+}
+TaskStatus ApplyFloors(MeshBlockData *rc, int container, const phx_fixup_params &fx) {
+  int lo[3], hi[3];
+  rc->entire(lo, hi);
+  const phx_block_fields f = rc->fields(container);
+  return st(phx_apply_floors(rc->ctx(), &rc->params(), &fx, &rc->block(), &f, lo, hi, nullptr));
+}
+}  // namespace fixup
+
 namespace update {
 TaskStatus FluxDivergenceUpdate(MeshBlockData *rc, int stage_container, int base_container,
                                 double beta, double dt, int out_container) {
@@ -193,10 +206,18 @@ Driver::Driver(MeshBlockData *rc, const boundaries::Kind kinds[6], double cfl,
   for (int i = 0; i < 6; ++i) kinds_[i] = kinds[i];
 }
 
+TaskStatus Driver::FillDerived(int container) {
+  if (fluid::ConservedToPrimitive(rc_, container) != TaskStatus::complete) return TaskStatus::fail;
+  if (fixup::ConservedToPrimitiveFixup(rc_, container, fx_) != TaskStatus::complete)
+    return TaskStatus::fail;
+  return fixup::ApplyFloors(rc_, container, fx_);
+}
+
 TaskStatus Driver::Initialize() {
-  // shock_tube.cpp:121 (p2c), then PostInitializationCommunication: boundaries and
+  // shock_tube.cpp:121 (p2c), then PostInitializationCommunication: floors, boundaries and
   // FillDerived (phoebus_driver.cpp:147-202); the stage container starts as a copy of base
   if (fluid::PrimitiveToConserved(rc_, kBase) != TaskStatus::complete) return TaskStatus::fail;
+  if (fixup::ApplyFloors(rc_, kBase, fx_) != TaskStatus::complete) return TaskStatus::fail;
   if (boundaries::ApplyBoundaryConditions(rc_, kBase, kinds_) != TaskStatus::complete)
     return TaskStatus::fail;
   if (fluid::ConservedToPrimitive(rc_, kBase) != TaskStatus::complete) return TaskStatus::fail;
@@ -223,11 +244,10 @@ TaskStatus Driver::RungeKuttaStage(int stage) {
   if (update::FluxDivergenceUpdate(rc_, sc0, kBase, beta[stage - 1], dt_, sc1) !=
       TaskStatus::complete)
     return TaskStatus::fail;
-  if (fluid::ConservedToPrimitive(rc_, sc1) != TaskStatus::complete) return TaskStatus::fail;
+  if (FillDerived(sc1) != TaskStatus::complete) return TaskStatus::fail;
   if (boundaries::ApplyBoundaryConditions(rc_, sc1, kinds_) != TaskStatus::complete)
     return TaskStatus::fail;
-  if (fluid::ConservedToPrimitive(rc_, sc1) != TaskStatus::complete) return TaskStatus::fail;
-  return TaskStatus::complete;
+  return FillDerived(sc1);
 }
 
 TaskStatus Driver::Step() {
@@ -297,6 +317,14 @@ int phx_driver_destroy(phx_driver *d) {
   delete d->drv;
   delete d->rc;
   delete d;
+  return PHX_OK;
+}
+
+int phx_driver_set_fixup(phx_driver *d, const phx_fixup_params *fx) {
+  if (!d || !fx) return PHX_ERR_INVALID;
+  if (fx->failure_strategy != PHX_FAILURE_INTERPOLATE && fx->failure_strategy != PHX_FAILURE_FLOORS)
+    return PHX_ERR_INVALID;
+  d->drv->set_fixup(*fx);
   return PHX_OK;
 }
 

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "phx.h"
+#include "phx_fixup.h"
 
 namespace phx {
 namespace host {
@@ -77,6 +78,13 @@ TaskStatus ConservedToPrimitive(MeshBlockData *rc, int container);
 double EstimateTimestepBlock(MeshBlockData *rc, bool &has_face_speeds, double cfl);
 }  // namespace fluid
 
+namespace fixup {
+// fixup_c2p.cpp:44-356 (fluid part, interior cells of one container)
+TaskStatus ConservedToPrimitiveFixup(MeshBlockData *rc, int container, const phx_fixup_params &fx);
+// fixup.cpp:285-539 (fluid part, IndexDomain::entire of one container)
+TaskStatus ApplyFloors(MeshBlockData *rc, int container, const phx_fixup_params &fx);
+}  // namespace fixup
+
 namespace update {
 // parthenon FluxDivergence + SumData + AverageIndependentData + UpdateIndependentData,
 // phoebus_driver.cpp:560-576: containers by index
@@ -97,7 +105,9 @@ TaskStatus ApplyBoundaryConditions(MeshBlockData *rc, int container, const Kind 
 // block with the LowStorageIntegrator rk2 weights (beta = 1, 1/2): per stage fluxes, CT,
 // sources, update, con2prim (entire), boundaries, con2prim (entire); the timestep after
 // the last stage; the first dt scaled by dt_init_fact, growth capped at 2x per step and
-// the last step clipped to tlim (parthenon's EvolutionDriver conventions).
+// the last step clipped to tlim (parthenon's EvolutionDriver conventions).  Every
+// con2prim is followed by the c2p fixup and the floors (phoebus_driver.cpp:586-597,
+// 664-676), which do nothing unless enabled in the fixup parameters.
 class Driver {
  public:
   Driver(MeshBlockData *rc, const boundaries::Kind kinds[6], double cfl, double dt_init_fact);
@@ -110,6 +120,7 @@ class Driver {
   double dt() const { return dt_; }
   long ncycle() const { return ncycle_; }
   void set_tlim(double t) { tlim_ = t; }
+  void set_fixup(const phx_fixup_params &fx) { fx_ = fx; }
 
  private:
   MeshBlockData *rc_;
@@ -117,6 +128,9 @@ class Driver {
   double cfl_, dt_init_fact_, time_ = 0.0, dt_ = 0.0, tlim_ = 1e300;
   long ncycle_ = 0;
   bool has_face_speeds_ = false;
+  phx_fixup_params fx_ = {0, PHX_FAILURE_INTERPOLATE, 0, 0, 50.0, 2500.0};
+  // FillDerived on one container followed by the fixup tasks
+  TaskStatus FillDerived(int container);
 };
 
 }  // namespace host

@@ -53,6 +53,20 @@ class Params(C.Structure):
                 ("bounds", Bounds), ("geom", Geom)]
 
 
+FAILURE_INTERPOLATE, FAILURE_FLOORS = 0, 1
+
+
+class Fixup(C.Structure):
+    _fields_ = [("enable_c2p_fixup", C.c_int), ("failure_strategy", C.c_int),
+                ("enable_floors", C.c_int), ("enable_mhd_floors", C.c_int),
+                ("bsqorho_max", C.c_double), ("bsqou_max", C.c_double)]
+
+
+def fixup(c2p_fixup=True, strategy=FAILURE_INTERPOLATE, floors=True, mhd_floors=False,
+          bsqorho_max=50.0, bsqou_max=2500.0):
+    return Fixup(int(c2p_fixup), strategy, int(floors), int(mhd_floors), bsqorho_max, bsqou_max)
+
+
 class PointGeom(C.Structure):
     _fields_ = [("alpha", C.c_double), ("beta", C.c_double * 3),
                 ("gcov", (C.c_double * 4) * 4), ("gcon", (C.c_double * 3) * 3),
@@ -252,6 +266,23 @@ def prim_to_con_block(params, b, prim, lo=None, hi=None):
     return cons, sig
 
 
+def c2p_fixup_block(params, fx, b, prim, cons, temp, cell_sig, fail):
+    """orc_c2p_fixup_block in place on prim, cons, temp, cell_sig."""
+    lib().orc_c2p_fixup_block(C.byref(params), C.byref(fx), C.byref(b), ptr(prim), ptr(cons),
+                              ptr(temp), ptr(cell_sig), ptr(fail))
+
+
+def apply_floors_block(params, fx, b, prim, cons, temp, mu, cell_sig, lo=None, hi=None):
+    """orc_apply_floors_block in place over lo..hi (default: the entire block)."""
+    sh = block_shape(b)
+    if lo is None:
+        lo = (0, 0, 0)
+        hi = (sh[2] - 1, sh[1] - 1, sh[0] - 1)
+    lib().orc_apply_floors_block(C.byref(params), C.byref(fx), C.byref(b), (C.c_int * 3)(*lo),
+                                 (C.c_int * 3)(*hi), ptr(prim), ptr(cons), ptr(temp), ptr(mu),
+                                 ptr(cell_sig))
+
+
 def source_terms_block(params, b, prim):
     src = np.zeros((ncons(params.mhd),) + block_shape(b))
     lib().orc_source_terms_block(C.byref(params), C.byref(b), ptr(np.ascontiguousarray(prim)),
@@ -338,12 +369,15 @@ class Driver:
     """The one-block RK2 task sequence of a phoebus cycle on the oracle's block functions
     (phoebus_driver.cpp:147-202 for the start, :204-738 per stage; rk2 containers "base"
     and "1" with beta = 1, 1/2): the CPU counterpart of include/phx_driver.h, step for
-    step and container for container, so that results compare bit for bit in Minkowski."""
+    step and container for container, so that results compare bit for bit in Minkowski.
+    Each con2prim is followed by the c2p fixup and the floors (phoebus_driver.cpp:586-597,
+    664-676) when fixup switches are given."""
 
     BETA = (1.0, 0.5)
 
-    def __init__(self, params, b, kinds, cfl, dt_init_fact=1.0):
+    def __init__(self, params, b, kinds, cfl, dt_init_fact=1.0, fx=None):
         self.p, self.b, self.kinds, self.cfl = params, b, tuple(kinds), cfl
+        self.fx = fx if fx is not None else fixup(c2p_fixup=False, floors=False)
         self.dt_init_fact = dt_init_fact
         sh = block_shape(b)
         nc, npr = ncons(params.mhd), nprim(params.mhd)
@@ -369,6 +403,21 @@ class Driver:
                                  ptr(self.temp), ptr(self.mu), ptr(self.cell_sig),
                                  ptr(self.fail), ptr(self.iters))
 
+    def _fill_derived(self, container):
+        self._c2p(container)
+        lib().orc_c2p_fixup_block(C.byref(self.p), C.byref(self.fx), C.byref(self.b),
+                                  ptr(self.prim), ptr(self.cons[container]), ptr(self.temp),
+                                  ptr(self.cell_sig), ptr(self.fail))
+        self._floors(container)
+
+    def _floors(self, container):
+        lo = (0, 0, 0)
+        hi = (self.n[0] - 1, self.n[1] - 1, self.n[2] - 1)
+        lib().orc_apply_floors_block(C.byref(self.p), C.byref(self.fx), C.byref(self.b),
+                                     (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), ptr(self.prim),
+                                     ptr(self.cons[container]), ptr(self.temp), ptr(self.mu),
+                                     ptr(self.cell_sig))
+
     def _dt(self):
         fs = self.face_sig if self.has_face_speeds else None
         d = lib().orc_estimate_dt(C.byref(self.p), C.byref(self.b), ptr(self.cell_sig),
@@ -384,6 +433,7 @@ class Driver:
         lib().orc_prim_to_con_block(C.byref(self.p), C.byref(self.b), (C.c_int * 3)(*lo),
                                     (C.c_int * 3)(*hi), ptr(self.prim), ptr(self.cons[0]),
                                     ptr(sig))
+        self._floors(0)
         apply_bcs(self.p, self.b, self.cons[0], self.kinds)
         self._c2p(0)
         self.cons[1][...] = self.cons[0]
@@ -409,9 +459,9 @@ class Driver:
             C.byref(self.p), C.byref(self.b), ptr(flux), ptr(src) if src is not None else None,
             ptr(self.cons[sc0]), ptr(self.cons[0]), C.c_double(self.BETA[stage - 1]),
             C.c_double(self.dt), ptr(self.cons[sc1]))
-        self._c2p(sc1)
+        self._fill_derived(sc1)
         apply_bcs(self.p, self.b, self.cons[sc1], self.kinds)
-        self._c2p(sc1)
+        self._fill_derived(sc1)
 
     def step(self):
         if self.time + self.dt > self.tlim:

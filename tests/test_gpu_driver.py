@@ -9,7 +9,7 @@ import pytest
 
 import _oracle as O
 from phoebus_amd import _abi as A
-from phoebus_amd.api import Driver, Phx, block, fluid_params
+from phoebus_amd.api import Driver, Phx, block, fixup_params, fluid_params
 from phoebus_amd.workloads import shock_tube_prims
 
 pytestmark = pytest.mark.gpu
@@ -157,3 +157,51 @@ def test_driver_rejects_bad_arguments(phx):
         Driver(phx, p, b, cfl=0.0)
     with pytest.raises(A.PhxError):
         Driver(phx, p, block(1, (32,), ng=2, dx=(1 / 32, 1.0, 1.0)))  # WENO5 needs 4 ghosts
+
+
+def test_driver_with_fixups_bit_exact(phx):
+    """A relativistic blast into a near vacuum with the c2p fixup and floors on
+    (fixup.cpp:285-539, fixup_c2p.cpp:44-356): the density floor holds the vacuum and the
+    gamma ceiling (2) caps the blast's Lorentz factor; the device and oracle sequences
+    agree bit for bit."""
+    say("[gpu] driver: 1-D blast into vacuum with floors, ceilings and c2p fixup vs oracle")
+    nx = 256
+    gamma = 5 / 3
+    p, op = _pair(gamma, False, A.RECON_WENO5Z, A.RIEMANN_HLL, tol=1e-10)
+    floors = dict(floor_type=A.FLOOR_CONSTANT_RHO_SIE, rho0=1e-5, sie0=1e-2)
+    p = fluid_params(gamma=gamma, mhd=False, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
+                     c2p_tol=1e-10, floors=floors, ceilings=(2.0, 1e4))
+    op.bounds.floor_type = O.FLOOR_CONSTANT
+    op.bounds.r0, op.bounds.s0 = 1e-5, 1e-2
+    op.bounds.gam_max, op.bounds.sie_max = 2.0, 1e4
+    b = block(1, (nx,), ng=4, dx=(1.0 / nx, 1.0, 1.0))
+    ob = O.make_block(1, (nx,), ng=4, dx=(1.0 / nx, 1.0, 1.0))
+    sh = O.block_shape(ob)
+    x = (np.arange(sh[2]) - 4 + 0.5) / nx
+    prim0 = np.zeros((7,) + sh)
+    prim0[0] = np.where(x < 0.4, 1.0, 1e-6)
+    P = np.where(x < 0.4, 1000.0, 1e-8)
+    prim0[1] = np.where(x < 0.4, 0.0, -0.5)
+    prim0[4] = P / (gamma - 1)
+    prim0[5] = P
+    prim0[6] = gamma
+    fx = fixup_params(c2p_fixup=True, floors=True)
+    d = Driver(phx, p, b, cfl=0.4)
+    d.set_fixup(fx)
+    d.set_prim(prim0)
+    d.initialize()
+    od = O.Driver(op, ob, (O.BC_OUTFLOW,) * 6, cfl=0.4, fx=O.fixup(c2p_fixup=True, floors=True))
+    od.initialize(prim0)
+    np.testing.assert_array_equal(d.get(A.DRIVER_PRIM), od.prim)
+    nfail = 0
+    for _ in range(60):
+        d.step()
+        od.step()
+        nfail += int(np.sum(od.fail == 0.0))
+    say(f"    cycles {od.ncycle}, con2prim failures repaired along the way {nfail}")
+    assert d.time()[:3] == (od.time, od.dt, od.ncycle)
+    np.testing.assert_array_equal(d.get(A.DRIVER_PRIM), od.prim)
+    np.testing.assert_array_equal(d.get(A.DRIVER_CONS), od.cons[0])
+    np.testing.assert_array_equal(d.get(A.DRIVER_FAIL), od.fail)
+    assert np.all(od.prim[0] >= 1e-5 * (1 - 1e-12))  # the floor holds
+    d.close()
