@@ -29,6 +29,9 @@ NG = 4
 GAMMA = 13.0 / 9.0
 SEED = 20261202
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 vector
+VALU_SIMDS = 256 * 4         # CUs x SIMDs; one wave64 VALU instruction per 4 cycles each
+CLOCK_HZ = 2.4e9             # peak engine clock
 # compulsory FP64 state traffic of a fused stage per zone-update (reads U_s, U^n, 10 prims,
 # mu, T guess = 28 doubles; writes U_{s+1}, 11 prims incl. T, mu, fail, 3 cell speeds = 24)
 B_ALG_STAGE = 52 * 8
@@ -522,6 +525,26 @@ def main():
     if nx == 128 and args.recon == "weno5z" and os.path.exists(tfile):
         tk = json.load(open(tfile))["kernels"].get(dom)
         traffic = tk["traffic_bytes"] if tk else None
+    # the flux sweeps are bound by VALU issue: FP64 FLOP/s and issue occupancy from the
+    # instruction counts of rocprofv3 SQ_INSTS_VALU* runs of this workload (per launch,
+    # at the tile shape in use) over the live launch times
+    valu = None
+    vfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_valu.json")
+    if nx == 128 and args.recon == "weno5z" and os.path.exists(vfile):
+        vk = json.load(open(vfile))["kernels"]
+        valu = {}
+        for name in ("flux_x", "flux_y", "flux_z", "update_con2prim"):
+            key = name
+            if name in ("flux_y", "flux_z"):
+                t = tiles[1 if name == "flux_y" else 2]
+                key = f"{name}@{t[0]}x{t[1]}"
+            if key not in vk or name not in launch_ms:
+                continue
+            sec = launch_ms[name] * 1e-3
+            valu[name] = {"fp64_tflops": vk[key]["fp64_flop"] / sec / 1e12,
+                          "fp64_frac": vk[key]["fp64_flop"] / sec / 1e12 / FP64_PEAK_TFLOPS,
+                          "valu_issue_frac": vk[key]["valu_wave_instr"] * 4 /
+                          (sec * VALU_SIMDS * CLOCK_HZ)}
     warm_iters = iters.get()[I]
     cons_h, mu0_h = st.cons.get(), mu0.get()
     value = nx ** 3 / (ms_per_step * 1e-3)
@@ -551,6 +574,7 @@ def main():
                            "achieved_GBs": value * B_ALG_STAGE / 1e9,
                            "frac": value * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS},
         "launch_ms": launch_ms,
+        "valu_roofline": valu,
         "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
                      "mean_iters_cold": float(np.mean(cold_iters)),
                      "hist_warm": np.bincount(np.maximum(warm_iters, 0).ravel()).tolist(),

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "phx.h"
+#include "phx_fixup.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -51,7 +52,12 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
                     const phx_pack_block *blocks, phx_pack **out);
 int phx_pack_destroy(phx_pack *pack);
 int phx_pack_nblocks(const phx_pack *pack, int *n, int64_t *interior_cells);
-/* One RK stage over the pack (4-8 launches). */
+/* Enables (fx non-NULL) or disables (NULL) fixup::ConservedToPrimitiveFixup and
+ * fixup::ApplyFloors after each stage's con2prim (phoebus_driver.cpp:586-597), on the
+ * interior cells of every block (include/phx_fixup.h for the semantics); the stage's
+ * dt_out then sees the signal speeds after the fixups.  Off by default. */
+int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *pack, const phx_fixup_params *fx);
+/* One RK stage over the pack (4-10 launches). */
 int phx_pack_stage(phx_ctx *ctx, phx_pack *pack, const phx_pack_stage_args *a,
                    phx_stream stream);
 /* Times every admissible y/z tile shape on the pack's own fields (overwriting fluxes) and

@@ -1546,3 +1546,32 @@ void orc_apply_floors_block(const orc_params *p, const orc_fixup *fx, const orc_
         }
       }
 }
+
+/* ConvertBoundaryConditions' outer-x1 inflow check (phoebus_boundaries.cpp:356-380) on
+ * cells lo..hi: where u^1 = W v^1 - W beta^1 / alpha < 0 the normal-observer velocity's
+ * x1 component is replaced by beta^1 / alpha and W recomputed. */
+void orc_inflow_check_block(const orc_params *p, const orc_block *b, const int lo[3],
+                            const int hi[3], double *prim) {
+  int n[3];
+  orc_block_dims(b, n);
+  const long nc = (long)n[0] * n[1] * n[2];
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        const long c = ((long)k * n[1] + j) * n[0] + i;
+        orc_point_geom g;
+        orc_geom_point(&p->geom, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k), &g);
+        double vcon[3] = {prim[nc + c], prim[2 * nc + c], prim[3 * nc + c]};
+        double W = lorentz(vcon, g.gcov);
+        const double ucon1 = vcon[0] - W * g.beta[0] / g.alpha;
+        if (ucon1 < 0) {
+          for (int ii = 0; ii < 3; ++ii) vcon[ii] /= W;
+          vcon[0] = g.beta[0] / g.alpha;
+          double vsq = 0.;
+          for (int ii = 0; ii < 3; ++ii)
+            for (int jj = 0; jj < 3; ++jj) vsq += g.gcov[ii + 1][jj + 1] * vcon[ii] * vcon[jj];
+          W = 1. / sqrt(1. - vsq);
+          for (int ii = 0; ii < 3; ++ii) prim[(1 + ii) * nc + c] = W * vcon[ii];
+        }
+      }
+}

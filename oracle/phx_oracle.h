@@ -214,6 +214,11 @@ void orc_apply_floors_block(const orc_params *p, const orc_fixup *fx, const orc_
                             const int lo[3], const int hi[3], double *prim, double *cons,
                             double *temp, double *mu, double *cell_sig);
 
+/* the outer-x1 inflow check of ConvertBoundaryConditions (phoebus_boundaries.cpp:356-380)
+ * on the primitives of cells lo..hi */
+void orc_inflow_check_block(const orc_params *p, const orc_block *b, const int lo[3],
+                            const int hi[3], double *prim);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

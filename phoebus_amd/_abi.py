@@ -1,5 +1,5 @@
 """ctypes binding of libphx.so (include/phx.h, phx_runtime.h, phx_mlp.h, phx_driver.h,
-phx_pack.h, phx_fixup.h).
+phx_pack.h, phx_fixup.h, phx_mesh.h).
 
 The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading fails loudly
 when it has not been built; there is no CPU fallback.
@@ -71,6 +71,13 @@ class FixupParams(C.Structure):
                 ("bsqorho_max", C.c_double), ("bsqou_max", C.c_double)]
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [("nx", C.c_int * 3), ("block_nx", C.c_int * 3), ("ng", C.c_int),
+                ("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("bc", C.c_int * 6),
+                ("flux_fixup", C.c_int), ("ox1_inflow_check", C.c_int), ("cfl", C.c_double),
+                ("dt_init_fact", C.c_double)]
+
+
 class PackBlock(C.Structure):
     _fields_ = [("block", Block), ("f", BlockFields), ("cons1", C.c_void_p),
                 ("iters", C.c_void_p)]
@@ -137,6 +144,7 @@ def lib():
         "phx_pack_destroy": (i32, [vp]),
         "phx_pack_nblocks": (i32, [vp, P(i32), P(C.c_int64)]),
         "phx_pack_stage": (i32, [vp, vp, P(PackStageArgs), vp]),
+        "phx_pack_set_fixup": (i32, [vp, vp, P(FixupParams)]),
         "phx_pack_autotune": (i32, [vp, vp, i32, vp]),
         "phx_pack_tile_shape": (i32, [vp, i32, P(i32), P(i32)]),
         "phx_pack_graph_create": (i32, [vp, vp, i32, P(PackStageArgs), vp, P(vp)]),
@@ -151,6 +159,18 @@ def lib():
         "phx_driver_execute": (i32, [vp, dbl, i32]),
         "phx_driver_time": (i32, [vp, P(dbl), P(dbl), P(C.c_int64)]),
         "phx_driver_get": (i32, [vp, i32, vp]),
+        "phx_mesh_create": (i32, [vp, P(FluidParams), P(FixupParams), P(MeshDesc), P(vp)]),
+        "phx_mesh_destroy": (i32, [vp]),
+        "phx_mesh_nblocks": (i32, [vp, P(i32), P(C.c_int64)]),
+        "phx_mesh_block": (i32, [vp, i32, P(Block)]),
+        "phx_mesh_set_prim": (i32, [vp, i32, vp]),
+        "phx_mesh_initialize": (i32, [vp]),
+        "phx_mesh_step": (i32, [vp]),
+        "phx_mesh_execute": (i32, [vp, dbl, i32]),
+        "phx_mesh_time": (i32, [vp, P(dbl), P(dbl), P(C.c_int64)]),
+        "phx_mesh_get": (i32, [vp, i32, i32, vp]),
+        "phx_mesh_get_iters": (i32, [vp, i32, vp]),
+        "phx_mesh_autotune": (i32, [vp, i32]),
         "phx_rt_device_count": (i32, [P(i32)]),
         "phx_rt_set_device": (i32, [i32]),
         "phx_rt_malloc": (i32, [P(vp), C.c_size_t]),

@@ -334,6 +334,78 @@ class Driver:
         return out if what in nv else out[0]
 
 
+class Mesh:
+    """A uniform 3-D mesh of equal MHD meshblocks on one device, advanced by the RK2 cycle
+    with ghost exchange, boundaries and fixups (include/phx_mesh.h).  bc: six of
+    BC_OUTFLOW / BC_PERIODIC (ix1, ox1, ix2, ox2, ix3, ox3)."""
+
+    def __init__(self, phx, p, nx, block_nx, xmin, xmax, bc, fx=None, ng=4, flux_fixup=True,
+                 inflow_check=False, cfl=0.4, dt_init_fact=1.0):
+        self.phx, self.L, self.p = phx, phx.L, p
+        d = A.MeshDesc((C.c_int * 3)(*nx), (C.c_int * 3)(*block_nx), ng,
+                       (C.c_double * 3)(*xmin), (C.c_double * 3)(*xmax), (C.c_int * 6)(*bc),
+                       int(flux_fixup), int(inflow_check), cfl, dt_init_fact)
+        h = C.c_void_p()
+        phx._chk(self.L.phx_mesh_create(phx.h, C.byref(p), C.byref(fx) if fx is not None else None,
+                                        C.byref(d), C.byref(h)), "phx_mesh_create")
+        self.h = h.value
+        n, cells = C.c_int(), C.c_int64()
+        self.L.phx_mesh_nblocks(self.h, C.byref(n), C.byref(cells))
+        self.nblocks, self.interior_cells = n.value, cells.value
+        self.blocks = []
+        for g in range(self.nblocks):
+            b = A.Block()
+            self.L.phx_mesh_block(self.h, g, C.byref(b))
+            self.blocks.append(b)
+        self.shape = block_shape(self.blocks[0])
+
+    def close(self):
+        if self.h:
+            self.L.phx_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_prim(self, gid, prim):
+        prim = np.ascontiguousarray(prim, np.float64)
+        assert prim.shape == (NPRIM[True],) + self.shape
+        self.phx._chk(self.L.phx_mesh_set_prim(self.h, gid, prim.ctypes.data), "phx_mesh_set_prim")
+
+    def initialize(self):
+        self.phx._chk(self.L.phx_mesh_initialize(self.h), "phx_mesh_initialize")
+
+    def step(self):
+        self.phx._chk(self.L.phx_mesh_step(self.h), "phx_mesh_step")
+
+    def execute(self, tlim, max_steps=1 << 30):
+        self.phx._chk(self.L.phx_mesh_execute(self.h, tlim, max_steps), "phx_mesh_execute")
+
+    def autotune(self, reps=2):
+        self.phx._chk(self.L.phx_mesh_autotune(self.h, reps), "phx_mesh_autotune")
+
+    def time(self):
+        t, dt, n = C.c_double(), C.c_double(), C.c_int64()
+        self.phx._chk(self.L.phx_mesh_time(self.h, C.byref(t), C.byref(dt), C.byref(n)),
+                      "phx_mesh_time")
+        return t.value, dt.value, n.value
+
+    def get(self, gid, what):
+        nv = {A.DRIVER_PRIM: NPRIM[True], A.DRIVER_CONS: NCONS[True]}
+        out = np.zeros((nv.get(what, 1),) + self.shape)
+        self.phx._chk(self.L.phx_mesh_get(self.h, gid, what, out.ctypes.data), "phx_mesh_get")
+        return out if what in nv else out[0]
+
+    def get_iters(self, gid):
+        out = np.zeros(self.shape, np.int32)
+        self.phx._chk(self.L.phx_mesh_get_iters(self.h, gid, out.ctypes.data),
+                      "phx_mesh_get_iters")
+        return out
+
+
 class Graph:
     """A captured, replayable sequence of pack stages (include/phx_pack.h)."""
 
@@ -395,6 +467,12 @@ class Pack:
         a = self.args(stage, dt, dt_dev, cfl, dt_out)
         self.phx._chk(self.L.phx_pack_stage(self.phx.h, self.h, C.byref(a), stream),
                       "phx_pack_stage")
+
+    def set_fixup(self, fx=None):
+        """fixup switches after each stage's con2prim (None disables)."""
+        self.phx._chk(self.L.phx_pack_set_fixup(self.phx.h, self.h,
+                                                C.byref(fx) if fx is not None else None),
+                      "phx_pack_set_fixup")
 
     def autotune(self, reps=3, stream=None):
         self.phx._chk(self.L.phx_pack_autotune(self.phx.h, self.h, reps, stream),

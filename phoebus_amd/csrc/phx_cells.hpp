@@ -1,0 +1,103 @@
+// phx_cells.hpp -- per-cell bodies that several kernels share: the Lorentz factor, the
+// robust con2prim of one cell (fluid.cpp:478-517) and p2c of one cell (fluid.cpp:359-449).
+#pragma once
+
+#include "phx_device.hpp"
+
+namespace phx {
+
+// relativity_utils.hpp:54-59 (W from the primitive W v^i)
+template <class M>
+__device__ __forceinline__ double lorentz_factor(const M &g, const double vp[3]) {
+  double vsq = 0.0;
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+      if (M::nz_cov(ii + 1, jj + 1)) vsq += g.gcov(ii + 1, jj + 1) * vp[ii] * vp[jj];
+  return sqrt(1. + vsq);
+}
+
+// singularity-eos IdealGas BulkModulusFromDensityInternalEnergy (unit scales 1)
+__device__ __forceinline__ double eos_bmod_rho_sie(const DevParams &p, double rho, double sie) {
+  return (p.gm1 + 1.0) * p.gm1 * rho * sie;
+}
+
+// con2prim of cell c = (i, j, k) of block b, in place on cons / prim / temp / mu / csig;
+// writes the fail flag and the iteration count
+template <bool MHD, int GEOM>
+__device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeom &b, int i,
+                                              int j, int k, double *__restrict__ cons,
+                                              double *__restrict__ prim,
+                                              double *__restrict__ temp,
+                                              double *__restrict__ mu,
+                                              double *__restrict__ csig,
+                                              double *__restrict__ fail,
+                                              int32_t *__restrict__ iters) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell;
+  double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
+  // zones rejected up front by the NaN test keep every field (con2prim_robust.hpp:303-304)
+  if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
+    fail[c] = 0.0;
+    if (iters) iters[c] = 0;
+    return;
+  }
+  const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+  const M g(p, X[0], X[1], X[2]);
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sig[d] = d < b.ndim ? csig[d * n + c] : 0.0;
+  double tt = temp ? temp[c] : 0.0, m = mu[c];
+  int it = 0;
+  const double st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) cons[v * n + c] = cs[v];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) prim[v * n + c] = pr[v];
+  for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
+  if (temp) temp[c] = tt;
+  mu[c] = m;
+  fail[c] = st;
+  if (iters) iters[c] = it;
+}
+
+// p2c of cell (i, j, k): cons (with B) and, when csig is given, the cell signal speeds
+template <bool MHD, int GEOM>
+__device__ __forceinline__ void prim_to_con_cell(const DevParams &p, const BlockGeom &b, int i,
+                                                 int j, int k,
+                                                 const double *__restrict__ prim,
+                                                 double *__restrict__ cons,
+                                                 double *__restrict__ csig) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const long n = b.ncell;
+  const M g(p, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
+  double pr[L::nprim];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  const double vp[3] = {pr[1], pr[2], pr[3]};
+  const double bp[3] = {MHD ? pr[5] : 0.0, MHD ? pr[6] : 0.0, MHD ? pr[7] : 0.0};
+  double D, S[3], bc[3], tau, sig[3];
+  p2c<MHD>(pr[0], vp, bp, pr[4], pr[L::prs], pr[L::gm1], g, D, S, bc, tau, sig);
+  cons[c] = D;
+  cons[n + c] = S[0];
+  cons[2 * n + c] = S[1];
+  cons[3 * n + c] = S[2];
+  cons[4 * n + c] = tau;
+  if constexpr (MHD) {
+    cons[5 * n + c] = bc[0];
+    cons[6 * n + c] = bc[1];
+    cons[7 * n + c] = bc[2];
+  }
+  if (csig)
+    for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
+}
+
+}  // namespace phx

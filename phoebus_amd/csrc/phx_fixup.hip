@@ -10,25 +10,9 @@
 #include "phx_device.hpp"
 #include "phx_fixup.h"
 #include "phx_internal.hpp"
+#include "phx_cells.hpp"
 
 namespace phx {
-
-// relativity_utils.hpp:54-59 (W from the primitive W v^i)
-template <class M>
-__device__ __forceinline__ double lorentz_factor(const M &g, const double vp[3]) {
-  double vsq = 0.0;
-#pragma unroll
-  for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-      if (M::nz_cov(ii + 1, jj + 1)) vsq += g.gcov(ii + 1, jj + 1) * vp[ii] * vp[jj];
-  return sqrt(1. + vsq);
-}
-
-// singularity-eos IdealGas BulkModulusFromDensityInternalEnergy (unit scales 1)
-__device__ __forceinline__ double eos_bmod_rho_sie(const DevParams &p, double rho, double sie) {
-  return (p.gm1 + 1.0) * p.gm1 * rho * sie;
-}
 
 // ConservedToPrimitiveFixupImpl (fixup_c2p.cpp:175-312), fluid part, for interior cell
 // (i, j, k).  A failed neighbour inside the interior is being rewritten by this same pass:
@@ -285,31 +269,42 @@ __global__ __launch_bounds__(256) void k_pack_c2p_fixup(DevParams p, DevFixup fx
   c2p_fixup_cell<true, GEOM>(p, fx, E.bg, i, j, k, E.prim_out, E.u_out, E.temp, E.csig, E.fail);
 }
 
-// floors, then the cell's CFL time with face speeds (fluid.cpp:913-928) into the
+// floors (on the interior cells, or on the entire blocks for a work list over them),
+// then the interior cell's CFL time with face speeds (fluid.cpp:913-928) into the
 // workgroup's partial minimum: the time step sees the signal speeds after the fixups
 template <int GEOM>
 __global__ __launch_bounds__(256) void k_pack_floors(DevParams p, DevFixup fx, PackView v,
-                                                     double *__restrict__ partial) {
+                                                     int entire, double *__restrict__ partial) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
+  const long t = (g - v.start[b]) * 256 + threadIdx.x;
   int i, j, k;
+  bool cell, inner;
+  if (entire) {
+    cell = t < bg.ncell;
+    i = (int)(t % bg.ni);
+    j = (int)((t / bg.ni) % bg.nj);
+    k = (int)(t / ((long)bg.ni * bg.nj));
+    inner = cell && i >= bg.is[0] && i <= bg.ie[0] && j >= bg.is[1] && j <= bg.ie[1] &&
+            k >= bg.is[2] && k <= bg.ie[2];
+  } else {
+    cell = inner = interior_cell(bg, t, i, j, k);
+  }
   double ldt_min = DBL_MAX;
-  if (interior_cell(bg, (g - v.start[b]) * 256 + threadIdx.x, i, j, k)) {
-    if (fx.floors)
-      floors_cell<true, GEOM>(p, fx, bg, i, j, k, E.prim_out, E.u_out, E.temp, E.mu, E.csig);
-    if (partial) {
-      const long n = bg.ncell, c = ((long)k * bg.nj + j) * bg.ni + i;
-      const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
-      double ldt = 0.0;
-      for (int d = 0; d < bg.ndim; ++d) {
-        const double s =
-            smax(E.csig[d * n + c], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
-        ldt += s / bg.dx[d];
-      }
-      ldt_min = 1.0 / (ldt + 1.e-50);
+  if (cell && fx.floors)
+    floors_cell<true, GEOM>(p, fx, bg, i, j, k, E.prim_out, E.u_out, E.temp, E.mu, E.csig);
+  if (inner && partial) {
+    const long n = bg.ncell, c = ((long)k * bg.nj + j) * bg.ni + i;
+    const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
+    double ldt = 0.0;
+    for (int d = 0; d < bg.ndim; ++d) {
+      const double s =
+          smax(E.csig[d * n + c], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
+      ldt += s / bg.dx[d];
     }
+    ldt_min = 1.0 / (ldt + 1.e-50);
   }
   if (partial) {
     const double m = block_min(ldt_min);
@@ -327,11 +322,11 @@ void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, co
 }
 
 void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
-                        long groups, double *partial, hipStream_t s) {
+                        long groups, double *partial, hipStream_t s, bool entire) {
   if (groups <= 0 || (!fx.floors && !partial)) return;
   with_geom(geom, [&](auto G) {
     hipLaunchKernelGGL((k_pack_floors<decltype(G)::value>), dim3((unsigned)groups), dim3(256),
-                       0, s, dp, fx, v, partial);
+                       0, s, dp, fx, v, entire ? 1 : 0, partial);
   });
 }
 

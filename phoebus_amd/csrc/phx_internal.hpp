@@ -136,12 +136,13 @@ void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long 
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
                         hipStream_t s);
-// the fixups over a pack's interior cells (phx_fixup.hip): con2prim failure repair, then
-// floors and ceilings folded with the CFL partial minima (partial may be null)
+// the fixups over a pack (phx_fixup.hip): con2prim failure repair on the interior cells,
+// then floors and ceilings (interior cells, or the entire blocks over an entire-cell work
+// list) folded with the interior CFL partial minima (partial may be null)
 void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
                            long groups, hipStream_t s);
 void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
-                        long groups, double *partial, hipStream_t s);
+                        long groups, double *partial, hipStream_t s, bool entire = false);
 // groups per block for each launch kind
 long pack_groups_flux(int d, int recon, const BlockGeom &bg, TileShape shape);
 long pack_groups_cells(const BlockGeom &bg, bool corners);

@@ -17,6 +17,7 @@
 #include "phx.h"
 #include "phx_device.hpp"
 #include "phx_internal.hpp"
+#include "phx_cells.hpp"
 
 namespace phx {
 
@@ -155,42 +156,12 @@ __global__ __launch_bounds__(256) void k_con2prim(DevParams p, BlockGeom b, Rang
                                                   double *__restrict__ csig,
                                                   double *__restrict__ fail,
                                                   int32_t *__restrict__ iters) {
-  using M = typename MetricOf<GEOM>::type;
-  using L = Layout<MHD>;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rg.n[0] * rg.n[1] * rg.n[2]) return;
   const int i = (int)(t % rg.n[0]) + rg.lo[0];
   const int j = (int)((t / rg.n[0]) % rg.n[1]) + rg.lo[1];
   const int k = (int)(t / ((long)rg.n[0] * rg.n[1])) + rg.lo[2];
-  const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long n = b.ncell;
-  const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  const M g(p, X[0], X[1], X[2]);
-  double cs[L::ncons], pr[L::nprim], sig[3];
-#pragma unroll
-  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) sig[d] = d < b.ndim ? csig[d * n + c] : 0.0;
-  // zones rejected up front by the NaN test keep every field (con2prim_robust.hpp:303-304)
-  if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
-    fail[c] = 0.0;
-    if (iters) iters[c] = 0;
-    return;
-  }
-  double tt = temp ? temp[c] : 0.0, m = mu[c];
-  int it = 0;
-  const double st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) cons[v * n + c] = cs[v];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) prim[v * n + c] = pr[v];
-  for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
-  if (temp) temp[c] = tt;
-  mu[c] = m;
-  fail[c] = st;
-  if (iters) iters[c] = it;
+  con2prim_cell<MHD, GEOM>(p, b, i, j, k, cons, prim, temp, mu, csig, fail, iters);
 }
 
 // ---- prim -> cons over a cell range (fluid.cpp:359-449) --------------------------------
@@ -199,35 +170,12 @@ __global__ __launch_bounds__(256) void k_prim_to_con(DevParams p, BlockGeom b, R
                                                      const double *__restrict__ prim,
                                                      double *__restrict__ cons,
                                                      double *__restrict__ csig) {
-  using M = typename MetricOf<GEOM>::type;
-  using L = Layout<MHD>;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rg.n[0] * rg.n[1] * rg.n[2]) return;
   const int i = (int)(t % rg.n[0]) + rg.lo[0];
   const int j = (int)((t / rg.n[0]) % rg.n[1]) + rg.lo[1];
   const int k = (int)(t / ((long)rg.n[0] * rg.n[1])) + rg.lo[2];
-  const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long n = b.ncell;
-  const M g(p, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
-  double pr[L::nprim];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
-  const double vp[3] = {pr[1], pr[2], pr[3]};
-  const double bp[3] = {MHD ? pr[5] : 0.0, MHD ? pr[6] : 0.0, MHD ? pr[7] : 0.0};
-  double D, S[3], bc[3], tau, sig[3];
-  p2c<MHD>(pr[0], vp, bp, pr[4], pr[L::prs], pr[L::gm1], g, D, S, bc, tau, sig);
-  cons[c] = D;
-  cons[n + c] = S[0];
-  cons[2 * n + c] = S[1];
-  cons[3 * n + c] = S[2];
-  cons[4 * n + c] = tau;
-  if constexpr (MHD) {
-    cons[5 * n + c] = bc[0];
-    cons[6 * n + c] = bc[1];
-    cons[7 * n + c] = bc[2];
-  }
-  if (csig)
-    for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
+  prim_to_con_cell<MHD, GEOM>(p, b, i, j, k, prim, cons, csig);
 }
 
 // ---- CFL timestep (fluid.cpp:899-944): per-block minimum, then one finishing block ----

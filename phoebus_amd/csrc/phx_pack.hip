@@ -11,31 +11,11 @@
 #include "phx.h"
 #include "phx_internal.hpp"
 #include "phx_pack.h"
+#include "phx_pack_impl.hpp"
 
 using namespace phx;
 
-struct phx_pack {
-  int device = 0;
-  phx_fluid_params p;
-  DevParams dp;
-  int n = 0;
-  long interior = 0;
-  std::vector<phx_pack_block> blocks;
-  std::vector<BlockGeom> geoms;
-  PackEntry *entries[2] = {nullptr, nullptr};  // device: stage 1 and stage 2 roles
-  // work lists: 0..2 flux sweeps, 3 CT corners, 4 interior cells
-  long *start[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  long groups[5] = {0, 0, 0, 0, 0};
-  TileShape tiles[3];
-  double *partial = nullptr;  // [groups[4]] CFL partial minima
-};
-
-struct phx_graph {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-};
-
-namespace {
+namespace phx {
 
 // tile shapes must fit every block of the pack: resolve against the smallest extent
 TileShape pack_shape(const phx_pack *k, int d, TileShape want) {
@@ -52,8 +32,9 @@ int build_start(phx_pack *k, int kind) {
   std::vector<long> st(k->n + 1, 0);
   for (int b = 0; b < k->n; ++b) {
     const BlockGeom &g = k->geoms[b];
-    const long c = kind < 3 ? pack_groups_flux(kind, k->p.recon, g, k->tiles[kind])
-                            : pack_groups_cells(g, kind == 3);
+    const long c = kind < 3    ? pack_groups_flux(kind, k->p.recon, g, k->tiles[kind])
+                   : kind < 5 ? pack_groups_cells(g, kind == 3)
+                              : (g.ncell + 255) / 256;
     st[b + 1] = st[b] + c;
   }
   if (st[k->n] > (long)UINT_MAX) return PHX_ERR_INVALID;
@@ -74,9 +55,8 @@ PackView view(const phx_pack *k, int stage, int kind) {
   return v;
 }
 
-void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipStream_t s) {
+void launch_stage_fluxes(phx_ctx *ctx, phx_pack *k, int st, hipStream_t s) {
   static const char *kName[3] = {"flux_x", "flux_y", "flux_z"};
-  const int st = a->stage;
   for (int d = 0; d < 3; ++d) {
     ctx->mark_begin(kName[d], s);
     launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, view(k, st, d),
@@ -91,16 +71,39 @@ void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipSt
     launch_pack_sources(k->p.geom.kind, k->dp, view(k, st, 4), k->groups[4], s);
     ctx->mark_end(s);
   }
+}
+
+void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipStream_t s) {
+  const int st = a->stage;
+  launch_stage_fluxes(ctx, k, st, s);
+  // with fixups on, the CFL partials are taken after them (the repaired and floored cells'
+  // signal speeds), in the floors launch
+  const bool fixups = k->fx.c2p_fixup || k->fx.floors;
+  double *partial = a->dt_out ? k->partial : nullptr;
   ctx->mark_begin("update_con2prim", s);
   launch_pack_update(k->p.geom.kind, k->dp, view(k, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
-                     a->dt, a->dt_dev, a->dt_out ? k->partial : nullptr, s);
+                     a->dt, a->dt_dev, fixups ? nullptr : partial, s);
   ctx->mark_end(s);
+  if (fixups) {
+    if (k->fx.c2p_fixup) {
+      ctx->mark_begin("c2p_fixup", s);
+      launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, view(k, st, 4), k->groups[4], s);
+      ctx->mark_end(s);
+    }
+    ctx->mark_begin("floors", s);
+    launch_pack_floors(k->p.geom.kind, k->dp, k->fx, view(k, st, 4), k->groups[4], partial, s);
+    ctx->mark_end(s);
+  }
   if (a->dt_out) {
     ctx->mark_begin("dt_finish", s);
     launch_dt_finish(k->partial, (int)k->groups[4], a->cfl, a->dt_out, s);
     ctx->mark_end(s);
   }
 }
+
+}  // namespace phx
+
+namespace {
 
 int check_stage_args(phx_ctx *ctx, const phx_pack_stage_args *a) {
   if (!a || (a->stage != 1 && a->stage != 2))
@@ -170,8 +173,9 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
     ok = hipMalloc(&k->entries[st], sizeof(PackEntry) * nblocks) == hipSuccess &&
          hipMemcpy(k->entries[st], e[st].data(), sizeof(PackEntry) * nblocks,
                    hipMemcpyHostToDevice) == hipSuccess;
-  for (int kind = 0; kind < 5 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
-  ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max(k->groups[4], 1L)) == hipSuccess;
+  for (int kind = 0; kind < 6 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
+  ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max({k->groups[4], k->groups[5], 1L})) ==
+                 hipSuccess;
   if (!ok) {
     phx_pack_destroy(k);
     return ctx->fail(PHX_ERR_DEVICE, "pack set-up failed");
@@ -196,6 +200,18 @@ int phx_pack_nblocks(const phx_pack *k, int *n, int64_t *interior_cells) {
   if (!k) return PHX_ERR_INVALID;
   if (n) *n = k->n;
   if (interior_cells) *interior_cells = k->interior;
+  return PHX_OK;
+}
+
+int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *k, const phx_fixup_params *fx) {
+  if (!ctx || !k) return PHX_ERR_INVALID;
+  if (!fx) {
+    k->fx = DevFixup{};
+    return PHX_OK;
+  }
+  const int rc = check_fixup(ctx, fx);
+  if (rc) return rc;
+  k->fx = make_dev_fixup(&k->p, fx);
   return PHX_OK;
 }
 

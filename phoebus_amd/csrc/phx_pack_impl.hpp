@@ -1,0 +1,51 @@
+// phx_pack_impl.hpp -- the pack object behind include/phx_pack.h, shared with the mesh
+// driver (phx_mesh.hip), which runs its own sequence of the same pack launches.
+#pragma once
+
+#include <vector>
+
+#include "phx.h"
+#include "phx_internal.hpp"
+#include "phx_pack.h"
+
+using phx::BlockGeom;
+using phx::DevFixup;
+using phx::DevParams;
+using phx::PackEntry;
+using phx::TileShape;
+
+struct phx_pack {
+  int device = 0;
+  phx_fluid_params p;
+  DevParams dp;
+  int n = 0;
+  long interior = 0;
+  std::vector<phx_pack_block> blocks;
+  std::vector<BlockGeom> geoms;
+  PackEntry *entries[2] = {nullptr, nullptr};  // device: stage 1 and stage 2 roles
+  // work lists: 0..2 flux sweeps, 3 CT corners, 4 interior cells, 5 entire blocks
+  long *start[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  long groups[6] = {0, 0, 0, 0, 0, 0};
+  TileShape tiles[3];
+  double *partial = nullptr;  // [groups[4]] CFL partial minima
+  DevFixup fx{};              // c2p fixup and floors after the update (off by default)
+};
+
+struct phx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+
+namespace phx {
+// tile shapes must fit every block of the pack: resolved against the smallest extent
+TileShape pack_shape(const phx_pack *k, int d, TileShape want);
+// (re)builds work list kind (0..5) on the device
+int build_start(phx_pack *k, int kind);
+// the launch view of work list kind for stage st (1, 2: which containers are U_s, U_{s+1})
+PackView view(const phx_pack *k, int st, int kind);
+// the flux part of a stage: the three flux sweeps, CT, geometric sources (curved metrics)
+void launch_stage_fluxes(phx_ctx *ctx, phx_pack *k, int st, hipStream_t s);
+// one full pack stage: fluxes, CT, sources, fused update + con2prim, fixups, CFL partials
+void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipStream_t s);
+}  // namespace phx

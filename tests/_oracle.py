@@ -487,3 +487,246 @@ def mgr_transform(m, X):
     lib().orc_mgr_transform(C.byref(m), C.c_double(X[0]), C.c_double(X[1]), C.c_double(X[2]),
                             Cc, Jcov, Jcon)
     return np.array(Cc[:]), np.array([r[:] for r in Jcov]), np.array([r[:] for r in Jcon])
+
+
+class MeshDriver:
+    """The RK2 cycle of phoebus on a uniform mesh of equal 3-D blocks, on the oracle's block
+    functions, in the reference's task order (phoebus_driver.cpp:147-202 for the start,
+    :204-738 per stage): sources, fluxes, CT, FixFluxes (gr_outflow faces), update,
+    FillDerived (con2prim over the entire blocks) + c2p fixup + floors (entire), the
+    same-level ghost exchange of the FillGhost fields (bc_vars = primitive: primitives,
+    temperature, con2prim root, fail flag, conserved), gr_outflow boundaries (copy of the
+    first interior layer, faces in the order x1, x2, x3), ConvertBoundaryConditions (outer-x1
+    inflow check for FMKS, p2c on the ghost slabs of every face that is not a block
+    boundary), FillDerived + fixups again; EstimateTimestep after the last stage.  Block
+    gid = lx1 + nb1 (lx2 + nb2 lx3); block edges xmin + (xmax - xmin) lx / nb."""
+
+    BETA = (1.0, 0.5)
+    NEX = 21  # exchanged fields: 10 prims, 8 cons, temp, mu, fail
+
+    def __init__(self, params, nx, block_nx, xmin, xmax, bc, fx=None, ng=4, flux_fixup=True,
+                 inflow_check=False, cfl=0.4, dt_init_fact=1.0):
+        self.p, self.ng, self.cfl, self.dt_init_fact = params, ng, cfl, dt_init_fact
+        self.fx = fx if fx is not None else fixup(c2p_fixup=False, floors=False)
+        self.N, self.bnx = list(nx), list(block_nx)
+        self.nb = [nx[d] // block_nx[d] for d in range(3)]
+        self.bc = list(bc)
+        self.periodic = [bc[2 * d] == BC_PERIODIC for d in range(3)]
+        self.flux_fixup, self.inflow = flux_fixup, inflow_check and params.geom.kind == GEOM_FMKS
+        self.blocks, self.loc = [], []
+        for g in range(self.nb[0] * self.nb[1] * self.nb[2]):
+            lx = (g % self.nb[0], (g // self.nb[0]) % self.nb[1], g // (self.nb[0] * self.nb[1]))
+            bmin = [xmin[d] + (xmax[d] - xmin[d]) * (lx[d] / self.nb[d]) for d in range(3)]
+            dx = [(xmax[d] - xmin[d]) / nx[d] for d in range(3)]
+            self.blocks.append(make_block(3, block_nx, ng, tuple(bmin), tuple(dx)))
+            self.loc.append(lx)
+        sh = block_shape(self.blocks[0])
+        self.sh = sh
+        nb = len(self.blocks)
+        self.prim = [np.zeros((10,) + sh) for _ in range(nb)]
+        self.cons = [[np.zeros((8,) + sh), np.zeros((8,) + sh)] for _ in range(nb)]
+        self.temp = [np.zeros(sh) for _ in range(nb)]
+        self.mu = [np.zeros(sh) for _ in range(nb)]
+        self.fail = [np.zeros(sh) for _ in range(nb)]
+        self.sig = [np.zeros((3,) + sh) for _ in range(nb)]
+        self.fsig = [np.zeros((3,) + sh) for _ in range(nb)]
+        self.time, self.dt, self.ncycle = 0.0, 0.0, 0
+        self.has_face_speeds = False
+        self.tlim = 1e300
+
+    # ---- block tasks -------------------------------------------------------------------
+    def _entire(self):
+        return (0, 0, 0), (self.sh[2] - 1, self.sh[1] - 1, self.sh[0] - 1)
+
+    def _c2p(self, g, c):
+        lo, hi = self._entire()
+        it = np.zeros(self.sh, np.int32)
+        lib().orc_con2prim_block(C.byref(self.p), C.byref(self.blocks[g]), (C.c_int * 3)(*lo),
+                                 (C.c_int * 3)(*hi), ptr(self.cons[g][c]), ptr(self.prim[g]),
+                                 ptr(self.temp[g]), ptr(self.mu[g]), ptr(self.sig[g]),
+                                 ptr(self.fail[g]), ptr(it))
+
+    def _fixups(self, g, c):
+        b = self.blocks[g]
+        lib().orc_c2p_fixup_block(C.byref(self.p), C.byref(self.fx), C.byref(b), ptr(self.prim[g]),
+                                  ptr(self.cons[g][c]), ptr(self.temp[g]), ptr(self.sig[g]),
+                                  ptr(self.fail[g]))
+        lo, hi = self._entire()
+        lib().orc_apply_floors_block(C.byref(self.p), C.byref(self.fx), C.byref(b),
+                                     (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), ptr(self.prim[g]),
+                                     ptr(self.cons[g][c]), ptr(self.temp[g]), ptr(self.mu[g]),
+                                     ptr(self.sig[g]))
+
+    def _fill_derived(self, c):
+        for g in range(len(self.blocks)):
+            self._c2p(g, c)
+            self._fixups(g, c)
+
+    def _fields(self, g, c):
+        return [self.prim[g], self.cons[g][c], self.temp[g][None], self.mu[g][None],
+                self.fail[g][None]]
+
+    def _exchange(self, c):
+        ng = self.ng
+        G = np.full((self.NEX, self.N[2] + 2 * ng, self.N[1] + 2 * ng, self.N[0] + 2 * ng), np.nan)
+        for g, lx in enumerate(self.loc):
+            F = np.concatenate(self._fields(g, c))
+            sl = tuple(slice(ng + lx[d] * self.bnx[d], ng + (lx[d] + 1) * self.bnx[d])
+                       for d in (2, 1, 0))
+            G[(slice(None),) + sl] = F[:, ng:-ng, ng:-ng, ng:-ng]
+        for d in range(3):  # periodic wrap of the padding (axis 3 - d)
+            if not self.periodic[d]:
+                continue
+            ax, n = 3 - d, self.N[d]
+            lo_src = [slice(None)] * 4
+            hi_src = [slice(None)] * 4
+            lo_dst = [slice(None)] * 4
+            hi_dst = [slice(None)] * 4
+            lo_dst[ax], lo_src[ax] = slice(0, ng), slice(n, n + ng)
+            hi_dst[ax], hi_src[ax] = slice(n + ng, n + 2 * ng), slice(ng, 2 * ng)
+            G[tuple(lo_dst)] = G[tuple(lo_src)]
+            G[tuple(hi_dst)] = G[tuple(hi_src)]
+        for g, lx in enumerate(self.loc):
+            sl = tuple(slice(lx[d] * self.bnx[d], lx[d] * self.bnx[d] + self.bnx[d] + 2 * ng)
+                       for d in (2, 1, 0))
+            P = G[(slice(None),) + sl]
+            valid = np.ones(self.sh, bool)
+            ghost = np.zeros(self.sh, bool)
+            for d in range(3):
+                ax = 2 - d
+                idx = np.arange(self.sh[ax])
+                gi = lx[d] * self.bnx[d] + idx - ng
+                ok = (gi >= 0) & (gi < self.N[d]) | self.periodic[d]
+                shp = [1, 1, 1]
+                shp[ax] = -1
+                valid &= ok.reshape(shp)
+                ghost |= ((idx < ng) | (idx >= ng + self.bnx[d])).reshape(shp)
+            m = valid & ghost
+            for F, rows in zip(self._fields(g, c), (slice(0, 10), slice(10, 18), slice(18, 19),
+                                                   slice(19, 20), slice(20, 21))):
+                F[:, m] = P[rows][:, m]
+
+    def _outflow(self, c):
+        ng = self.ng
+        for d in range(3):
+            ax = 3 - d  # axis of dimension d in a [var][k][j][i] array
+            for side in (0, 1):
+                if self.bc[2 * d + side] != BC_OUTFLOW:
+                    continue
+                for g, lx in enumerate(self.loc):
+                    if lx[d] != (0 if side == 0 else self.nb[d] - 1):
+                        continue
+                    ref = ng if side == 0 else ng + self.bnx[d] - 1
+                    gh = slice(0, ng) if side == 0 else slice(ng + self.bnx[d], self.sh[2 - d])
+                    for F in self._fields(g, c):
+                        dst = [slice(None)] * 4
+                        src = [slice(None)] * 4
+                        dst[ax], src[ax] = gh, slice(ref, ref + 1)
+                        F[tuple(dst)] = F[tuple(src)]
+
+    def _convert(self, c):
+        ng = self.ng
+        for g, lx in enumerate(self.loc):
+            b = self.blocks[g]
+            slabs = []
+            for d in range(3):
+                for side in (0, 1):
+                    if lx[d] != (0 if side == 0 else self.nb[d] - 1):
+                        continue
+                    lo = [0, 0, 0]
+                    hi = [self.sh[2] - 1, self.sh[1] - 1, self.sh[0] - 1]
+                    if side == 0:
+                        hi[d] = ng - 1
+                    else:
+                        lo[d] = ng + self.bnx[d]
+                    slabs.append((d, side, lo, hi))
+            for d, side, lo, hi in slabs:
+                if d == 0 and side == 1 and self.inflow:
+                    lib().orc_inflow_check_block(C.byref(self.p), C.byref(b), (C.c_int * 3)(*lo),
+                                                 (C.c_int * 3)(*hi), ptr(self.prim[g]))
+            for d, side, lo, hi in slabs:
+                lib().orc_prim_to_con_block(C.byref(self.p), C.byref(b), (C.c_int * 3)(*lo),
+                                            (C.c_int * 3)(*hi), ptr(self.prim[g]),
+                                            ptr(self.cons[g][c]), ptr(self.sig[g]))
+
+    def _fix_fluxes(self, g, flux):
+        if not self.flux_fixup:
+            return
+        ng, lx = self.ng, self.loc[g]
+        inn = [slice(ng, ng + self.bnx[d]) for d in range(3)]
+        for d in range(3):
+            for side in (0, 1):
+                if self.bc[2 * d + side] != BC_OUTFLOW or lx[d] != (0 if side == 0 else self.nb[d] - 1):
+                    continue
+                face = ng if side == 0 else ng + self.bnx[d]
+                sl = [inn[2], inn[1], inn[0]]
+                sl[2 - d] = face
+                F = flux[d, 0]
+                F[tuple(sl)] = np.minimum(F[tuple(sl)], 0.0) if side == 0 else np.maximum(F[tuple(sl)], 0.0)
+
+    def _dt(self):
+        lo = np.inf
+        for g, b in enumerate(self.blocks):
+            fs = self.fsig[g] if self.has_face_speeds else None
+            lo = min(lo, lib().orc_estimate_dt(C.byref(self.p), C.byref(b), ptr(self.sig[g]),
+                                               ptr(fs) if fs is not None else None,
+                                               C.c_double(self.cfl)))
+        self.has_face_speeds = True
+        return lo
+
+    # ---- driver ------------------------------------------------------------------------
+    def set_prim(self, g, prim):
+        self.prim[g][...] = prim
+
+    def initialize(self):
+        lo, hi = self._entire()
+        for g, b in enumerate(self.blocks):
+            lib().orc_prim_to_con_block(C.byref(self.p), C.byref(b), (C.c_int * 3)(*lo),
+                                        (C.c_int * 3)(*hi), ptr(self.prim[g]),
+                                        ptr(self.cons[g][0]), ptr(self.sig[g]))
+            lib().orc_apply_floors_block(C.byref(self.p), C.byref(self.fx), C.byref(b),
+                                         (C.c_int * 3)(*lo), (C.c_int * 3)(*hi),
+                                         ptr(self.prim[g]), ptr(self.cons[g][0]),
+                                         ptr(self.temp[g]), ptr(self.mu[g]), ptr(self.sig[g]))
+        self._exchange(0)
+        self._outflow(0)
+        self._convert(0)
+        for g in range(len(self.blocks)):
+            self._c2p(g, 0)
+            self.cons[g][1][...] = self.cons[g][0]
+        self.time, self.ncycle = 0.0, 0
+        self.has_face_speeds = False
+        self.dt = self.dt_init_fact * self._dt()
+
+    def stage(self, st):
+        sc0, sc1 = (0, 1) if st == 1 else (1, 0)
+        for g, b in enumerate(self.blocks):
+            src = None
+            if self.p.geom.kind != GEOM_MINKOWSKI:
+                src = source_terms_block(self.p, b, self.prim[g])
+            flux = np.zeros((3, 8) + self.sh)
+            fsig = np.zeros((3,) + self.sh)
+            lib().orc_calculate_fluxes(C.byref(self.p), C.byref(b), ptr(self.prim[g]),
+                                       ptr(flux), ptr(fsig))
+            emf = np.zeros((3,) + self.sh)
+            lib().orc_flux_ct(C.byref(self.p), C.byref(b), ptr(flux), ptr(emf))
+            self._fix_fluxes(g, flux)
+            self.fsig[g] = fsig
+            lib().orc_flux_divergence_update(
+                C.byref(self.p), C.byref(b), ptr(flux), ptr(src) if src is not None else None,
+                ptr(self.cons[g][sc0]), ptr(self.cons[g][0]), C.c_double(self.BETA[st - 1]),
+                C.c_double(self.dt), ptr(self.cons[g][sc1]))
+        self._fill_derived(sc1)
+        self._exchange(sc1)
+        self._outflow(sc1)
+        self._convert(sc1)
+        self._fill_derived(sc1)
+
+    def step(self):
+        if self.time + self.dt > self.tlim:
+            self.dt = self.tlim - self.time
+        for st in (1, 2):
+            self.stage(st)
+        self.time += self.dt
+        self.ncycle += 1
+        self.dt = min(self._dt(), 2.0 * self.dt)

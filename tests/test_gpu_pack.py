@@ -9,7 +9,7 @@ import pytest
 
 import _physics as PH
 from phoebus_amd import _abi as A
-from phoebus_amd.api import BlockState, Pack, Phx, block, fluid_params, mgr
+from phoebus_amd.api import BlockState, Pack, Phx, block, fixup_params, fluid_params, mgr
 from phoebus_amd.device import DeviceArray, synchronize
 
 pytestmark = pytest.mark.gpu
@@ -210,3 +210,45 @@ def test_pack_rejects_bad_input(phx):
     st3 = BlockState(ph, b3)
     with pytest.raises(A.PhxError, match="MHD"):
         Pack(phx, ph, [(b3, st3, DeviceArray(st3.cons.shape), None)])
+
+
+def test_pack_fixups_match_block_sequence(phx):
+    """Pack stage with the c2p fixup and floors on == per block: the single-block stage,
+    phx_c2p_fixup, phx_apply_floors on the interior; dt after the fixups."""
+    say("[gpu] pack stage with fixups vs per-block stage + fixup + floors")
+    geom = A.GEOM_MINKOWSKI
+    rng = np.random.default_rng(77)
+    floors = dict(floor_type=A.FLOOR_CONSTANT_RHO_SIE, rho0=0.3, sie0=0.05)
+    p = fluid_params(gamma=13.0 / 9.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
+                     geom=geom, c2p_tol=1e-10, floors=floors, ceilings=(2.5, 1e300))
+    blocks = _blocks(geom)
+    S = Setup(phx, p, blocks, rng, geom)
+    pk = Pack(phx, p, [(b, st, c1, it) for b, st, c1, it in zip(blocks, S.states, S.cons1, S.iters)])
+    fx = fixup_params(c2p_fixup=True, mhd_floors=True, bsqorho_max=20.0, bsqou_max=400.0)
+    pk.set_fixup(fx)
+    dt = 1e-3
+    d_dt = DeviceArray((1,))
+    S.reset()
+    pk.stage(1, dt=dt, cfl=0.4, dt_out=d_dt)
+    synchronize()
+    after1, dt1 = S.snapshot(), d_dt.get()[0]
+    S.reset()
+    dts = []
+    d_one = DeviceArray((1,))
+    for b, st, c1, it in zip(blocks, S.states, S.cons1, S.iters):
+        f = st.fields()
+        phx.fluid_stage(p, b, f, 1.0, dt, st.cons, c1, st.prim, iters=it)
+        f1 = st.fields()
+        f1.cons = c1.ptr  # the fixups act on U_1
+        phx.c2p_fixup(p, fx, b, f1)
+        lo = tuple(b.ng for _ in range(3))
+        hi = tuple(b.ng + b.nx[d] - 1 for d in range(3))
+        phx.apply_floors(p, fx, b, f1, lo=lo, hi=hi)
+        phx.estimate_dt(p, b, f1, True, 0.4, d_one)
+        synchronize()
+        dts.append(d_one.get()[0])
+    _equal(S.snapshot(), after1, "stage 1 with fixups")
+    assert dt1 == min(dts)
+    nfail = sum(int(np.sum(s[4][4:-4, 4:-4, 4:-4] == 0.0)) for s in after1)
+    say(f"    con2prim failures repaired {nfail}")
+    pk.close()

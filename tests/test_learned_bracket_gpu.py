@@ -108,12 +108,15 @@ def test_training_cuts_iterations_and_keeps_results(phx):
     assert fb[ok].mean() < 0.05
     # same root to the solver tolerance: both stop inside [root - tol, root + tol]
     assert np.max(np.abs(mu[ok] - ev["mu"][ok])) <= 4 * p.c2p_tol
-    # density and velocities follow mu smoothly (a mu difference of 4 tol on roots down to
-    # mu ~ 2.5e-4 is a relative 1.6e-6); B is an input
+    # density and velocities follow mu smoothly: their relative change is at most about
+    # (2 W^2 + 4) times the relative difference of the two roots (W enters through
+    # vhat^2 = mu^2 rbar^2; W <= 10 here), and each root is within the tolerance of the
+    # true one, so hot states with tiny mu see the largest differences; B is an input
     pr, pp = d_out.get()[:, ok], ev["prim"][:, ok]
     scale = np.maximum(np.abs(pp), np.max(np.abs(pp[1:4]), axis=0))
+    dmu = np.abs(mu[ok] - ev["mu"][ok]) / ev["mu"][ok]
     for row in (0, 1, 2, 3, 5, 6, 7):
-        assert np.all(np.abs(pr[row] - pp[row]) <= 2e-5 * scale[row]), row
+        assert np.all(np.abs(pr[row] - pp[row]) <= (204 * dmu + 1e-11) * scale[row]), row
     # u and P of cold states carry the cancellation h - 1 - P/rho, so a mu difference
     # inside the tolerance moves them far more in relative terms: judge both solves by
     # how well their primitives reproduce the conserved state (FP64 oracle p2c)
