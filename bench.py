@@ -403,6 +403,95 @@ def c4_uneven_pack(phx, args, log):
     return out
 
 
+def c5_torus_mesh(phx, args, log):
+    """Full 3-D GRMHD torus: the magnetized Fishbone-Moncrief torus of the uneven-batch leg
+    on one 256 x 128 x 128 FMKS grid cut into 32 meshblocks of 64 x 64 x 32, advanced
+    args.c5_steps RK2 steps by the mesh driver: WENO5-Z + HLL + CT + sources + FixFluxes +
+    update + con2prim + c2p fixup + floors (torus deck floors and ceilings), ghost exchange,
+    gr_outflow boundaries in x1 and x2 with the FMKS inflow check, periodic x3; cfl 0.8 and
+    dt_init_fact 1e-6 as inputs/torus.pin.  Reports zone-updates/s over the whole run, the
+    per-launch times of one step, the con2prim iteration histogram and failures of the
+    last step, and the drift of the conserved totals."""
+    from phoebus_amd import _abi as A
+    from phoebus_amd.api import Mesh, fixup_params, fluid_params, mgr
+    from phoebus_amd.workloads import torus_grid
+
+    N, bnx = (256, 128, 128), (64, 64, 32)
+    rng = np.random.default_rng(SEED + 4)
+    grid_prim, xmin, dx = torus_grid(rng, nx=N)
+    xmax = tuple(xmin[d] + N[d] * dx[d] for d in range(3))
+    x1min = xmin[0]
+    floors = dict(floor_type=A.FLOOR_EXPX1_RHO_SIE, rho0=1e-8, sie0=1e-3, rho_exp=-2.0,
+                  sie_exp=-1.0, rho_min_eos=1e-8, sie_min_eos=1e-8)
+    p = fluid_params(gamma=5.0 / 3.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
+                     geom=A.GEOM_FMKS, a=0.9375, c2p_tol=1e-10, c2p_max_iter=100,
+                     floors=floors, ceilings=(10.0, 1.0),
+                     mgr_params=mgr(derefine_poles=False, x0=min(x1min, max(0.0, x1min - NG * dx[0]))))
+    fx = fixup_params(c2p_fixup=True, mhd_floors=True)
+    bc = (A.BC_OUTFLOW,) * 4 + (A.BC_PERIODIC,) * 2
+    m = Mesh(phx, p, N, bnx, xmin, xmax, bc, fx=fx, inflow_check=True, cfl=0.8,
+             dt_init_fact=1e-6)
+    for g, b in enumerate(m.blocks):
+        lx = [int(round((b.xmin[d] - xmin[d]) / dx[d])) for d in range(3)]
+        win = grid_prim[:, lx[2]:lx[2] + bnx[2] + 2 * NG, lx[1]:lx[1] + bnx[1] + 2 * NG,
+                        lx[0]:lx[0] + bnx[0] + 2 * NG]
+        m.set_prim(g, np.ascontiguousarray(win))
+    del grid_prim
+    m.initialize()
+    if not args.no_tune:
+        m.autotune(2)
+
+    def totals():
+        t = np.zeros(5)
+        for g in range(m.nblocks):
+            c = m.get(g, A.DRIVER_CONS)[:5, NG:-NG, NG:-NG, NG:-NG]
+            t += c.reshape(5, -1).sum(axis=1)
+        return t
+
+    tot0 = totals()
+    log(f"[c5] mesh {N} in {m.nblocks} blocks of {bnx}, {args.c5_steps} RK2 steps")
+    # one profiled step: per-launch times
+    phx.L.phx_rt_profile_start(phx.h)
+    phx.L.phx_rt_profile_reset(phx.h)
+    m.step()
+    n = C.c_int()
+    phx.L.phx_rt_profile_count(phx.h, C.byref(n))
+    launch_ms = {}
+    for r in range(n.value):
+        name = C.create_string_buffer(64)
+        ms = C.c_float()
+        phx.L.phx_rt_profile_get(phx.h, r, name, 64, C.byref(ms))
+        k = name.value.decode()
+        launch_ms[k] = launch_ms.get(k, 0.0) + ms.value / 2.0  # two stages per step
+    phx.L.phx_rt_profile_stop(phx.h)
+    t0 = time.perf_counter()
+    m.execute(1e300, args.c5_steps - 1)
+    wall = time.perf_counter() - t0
+    t, dt, ncycle = m.time()
+    tot1 = totals()
+    iters = np.concatenate([m.get_iters(g)[NG:-NG, NG:-NG, NG:-NG].ravel()
+                            for g in range(m.nblocks)])
+    fails = sum(int(np.sum(m.get(g, A.DRIVER_FAIL)[NG:-NG, NG:-NG, NG:-NG] == 0.0))
+                for g in range(m.nblocks))
+    steps = args.c5_steps - 1
+    zu = m.interior_cells * 2 * steps / wall
+    log(f"[c5] {steps} timed steps in {wall:.2f} s: {1e3 * wall / steps:.2f} ms/step, "
+        f"{zu / 1e9:.3f} G zone-updates/s, t = {t:.4g}, dt = {dt:.4g}")
+    m.close()
+    return {"workload": "FMKS torus 256x128x128, 32 blocks of 64x64x32, WENO5Z + HLL, "
+                        "fixups + floors (torus deck), gr_outflow x1/x2 + periodic x3",
+            "steps_timed": steps, "ms_per_step": 1e3 * wall / steps,
+            "zone_updates_per_s": zu, "time": t, "dt": dt, "ncycle": ncycle,
+            "launch_ms_per_stage": launch_ms,
+            "con2prim_last_stage": {"mean_iters": float(np.mean(np.maximum(iters, 0))),
+                                    "hist": np.bincount(np.maximum(iters, 0)).tolist(),
+                                    "unbracketed": int(np.sum(iters < 0)),
+                                    "fail_count_after_fixup_pass": fails},
+            "conserved_totals_initial": tot0.tolist(),
+            "conserved_totals_final": tot1.tolist(),
+            "conserved_drift_rel": ((tot1 - tot0) / np.maximum(np.abs(tot0), 1e-300)).tolist()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +511,8 @@ def main():
     ap.add_argument("--c4-blocks", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=5)
     ap.add_argument("--c4-per-block", type=int, default=1)
+    ap.add_argument("--no-c5", action="store_true", help="skip the full torus mesh leg")
+    ap.add_argument("--c5-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
@@ -586,6 +677,11 @@ def main():
         if "st" in locals():
             del st, u_out, prim_out, iters
         out["uneven_pack"] = c4_uneven_pack(
+            phx, args, lambda m: print(m, file=sys.stderr, flush=True))
+    if not args.no_c5:
+        if "st" in locals():
+            del st, u_out, prim_out, iters
+        out["torus_mesh"] = c5_torus_mesh(
             phx, args, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_c3:
         if "st" in locals():

@@ -75,7 +75,8 @@ int phx_mesh_execute(phx_mesh *m, double tlim, int max_steps);
 int phx_mesh_time(const phx_mesh *m, double *time, double *dt, int64_t *ncycle);
 /* a field (PHX_DRIVER_PRIM, _CONS (base container), _FAIL, _C2P_MU) of block gid to host */
 int phx_mesh_get(const phx_mesh *m, int gid, int what, double *host);
-/* con2prim iteration counts [cells] of block gid from the last FillDerived (int32) */
+/* con2prim iteration counts [cells] of block gid (int32): on the interior those of the
+ * last stage's inversion of U_{s+1}, on the ghosts those of the last FillDerived */
 int phx_mesh_get_iters(const phx_mesh *m, int gid, int32_t *host);
 /* tunes the y/z flux tile shapes on the mesh's own fields (overwriting fluxes) */
 int phx_mesh_autotune(phx_mesh *m, int reps);

@@ -217,8 +217,11 @@ __global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v, 
     }
     if (!(out == 1 && layer == 1)) return;
   }
+  // the interior's iteration counts stay those of the stage's inversion of U_{s+1} (the
+  // update kernel's); the re-inversion after the exchange does not overwrite them
+  int32_t *it = (mode == 0 && is_interior(bg, idx)) ? nullptr : E.iters;
   con2prim_cell<true, GEOM>(p, bg, idx[0], idx[1], idx[2], E.u_out, E.prim_out, E.temp, E.mu,
-                            E.csig, E.fail, E.iters);
+                            E.csig, E.fail, it);
 }
 
 }  // namespace phx
