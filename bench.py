@@ -409,11 +409,17 @@ def c5_torus_mesh(phx, args, log):
     args.c5_steps RK2 steps by the mesh driver: WENO5-Z + HLL + CT + sources + FixFluxes +
     update + con2prim + c2p fixup + floors (torus deck floors and ceilings), ghost exchange,
     gr_outflow boundaries in x1 and x2 with the FMKS inflow check, periodic x3; cfl 0.8 and
-    dt_init_fact 1e-6 as inputs/torus.pin.  Reports zone-updates/s over the whole run, the
-    per-launch times of one step, the con2prim iteration histogram and failures of the
-    last step, and the drift of the conserved totals."""
+    dt_init_fact 1e-6 as inputs/torus.pin.  Two runs from the same initial data: plain
+    (con2prim warm-started from the previous root, as the reference), and learned: plain
+    for the first args.c5_log_steps steps while logging solver pairs from the blocks, then
+    the bracket network trained on them on the device and every later con2prim with the
+    learned bracket.  Reports zone-updates/s (all steps, and the steps after the switch for
+    both runs), per-launch times of one step of each, iteration histograms of the last
+    step, the learned bracket's miss rate, and the drift of the conserved totals."""
     from phoebus_amd import _abi as A
     from phoebus_amd.api import Mesh, fixup_params, fluid_params, mgr
+    from phoebus_amd.learned import mesh_pairs, train_bracket_on
+    from phoebus_amd.device import DeviceArray
     from phoebus_amd.workloads import torus_grid
 
     N, bnx = (256, 128, 128), (64, 64, 32)
@@ -429,67 +435,125 @@ def c5_torus_mesh(phx, args, log):
                      mgr_params=mgr(derefine_poles=False, x0=min(x1min, max(0.0, x1min - NG * dx[0]))))
     fx = fixup_params(c2p_fixup=True, mhd_floors=True)
     bc = (A.BC_OUTFLOW,) * 4 + (A.BC_PERIODIC,) * 2
-    m = Mesh(phx, p, N, bnx, xmin, xmax, bc, fx=fx, inflow_check=True, cfl=0.8,
-             dt_init_fact=1e-6)
-    for g, b in enumerate(m.blocks):
-        lx = [int(round((b.xmin[d] - xmin[d]) / dx[d])) for d in range(3)]
-        win = grid_prim[:, lx[2]:lx[2] + bnx[2] + 2 * NG, lx[1]:lx[1] + bnx[1] + 2 * NG,
-                        lx[0]:lx[0] + bnx[0] + 2 * NG]
-        m.set_prim(g, np.ascontiguousarray(win))
-    del grid_prim
-    m.initialize()
-    if not args.no_tune:
-        m.autotune(2)
 
-    def totals():
+    def make():
+        m = Mesh(phx, p, N, bnx, xmin, xmax, bc, fx=fx, inflow_check=True, cfl=0.8,
+                 dt_init_fact=1e-6)
+        for g, b in enumerate(m.blocks):
+            lx = [int(round((b.xmin[d] - xmin[d]) / dx[d])) for d in range(3)]
+            win = grid_prim[:, lx[2]:lx[2] + bnx[2] + 2 * NG, lx[1]:lx[1] + bnx[1] + 2 * NG,
+                            lx[0]:lx[0] + bnx[0] + 2 * NG]
+            m.set_prim(g, np.ascontiguousarray(win))
+        m.initialize()
+        if not args.no_tune:
+            m.autotune(2)
+        return m
+
+    def totals(m):
         t = np.zeros(5)
         for g in range(m.nblocks):
             c = m.get(g, A.DRIVER_CONS)[:5, NG:-NG, NG:-NG, NG:-NG]
             t += c.reshape(5, -1).sum(axis=1)
         return t
 
-    tot0 = totals()
-    log(f"[c5] mesh {N} in {m.nblocks} blocks of {bnx}, {args.c5_steps} RK2 steps")
-    # one profiled step: per-launch times
-    phx.L.phx_rt_profile_start(phx.h)
-    phx.L.phx_rt_profile_reset(phx.h)
-    m.step()
-    n = C.c_int()
-    phx.L.phx_rt_profile_count(phx.h, C.byref(n))
-    launch_ms = {}
-    for r in range(n.value):
-        name = C.create_string_buffer(64)
-        ms = C.c_float()
-        phx.L.phx_rt_profile_get(phx.h, r, name, 64, C.byref(ms))
-        k = name.value.decode()
-        launch_ms[k] = launch_ms.get(k, 0.0) + ms.value / 2.0  # two stages per step
-    phx.L.phx_rt_profile_stop(phx.h)
-    t0 = time.perf_counter()
-    m.execute(1e300, args.c5_steps - 1)
-    wall = time.perf_counter() - t0
-    t, dt, ncycle = m.time()
-    tot1 = totals()
-    iters = np.concatenate([m.get_iters(g)[NG:-NG, NG:-NG, NG:-NG].ravel()
-                            for g in range(m.nblocks)])
-    fails = sum(int(np.sum(m.get(g, A.DRIVER_FAIL)[NG:-NG, NG:-NG, NG:-NG] == 0.0))
-                for g in range(m.nblocks))
-    steps = args.c5_steps - 1
-    zu = m.interior_cells * 2 * steps / wall
-    log(f"[c5] {steps} timed steps in {wall:.2f} s: {1e3 * wall / steps:.2f} ms/step, "
-        f"{zu / 1e9:.3f} G zone-updates/s, t = {t:.4g}, dt = {dt:.4g}")
+    def profiled_step(m):
+        phx.L.phx_rt_profile_start(phx.h)
+        phx.L.phx_rt_profile_reset(phx.h)
+        m.step()
+        n = C.c_int()
+        phx.L.phx_rt_profile_count(phx.h, C.byref(n))
+        launch = {}
+        for r in range(n.value):
+            name = C.create_string_buffer(64)
+            ms = C.c_float()
+            phx.L.phx_rt_profile_get(phx.h, r, name, 64, C.byref(ms))
+            k = name.value.decode()
+            launch[k] = launch.get(k, 0.0) + ms.value / 2.0  # per stage
+        phx.L.phx_rt_profile_stop(phx.h)
+        return launch
+
+    def timed(m, nsteps):
+        t0 = time.perf_counter()
+        m.execute(1e300, nsteps)
+        return time.perf_counter() - t0
+
+    def last_iters(m):
+        it = np.concatenate([m.get_iters(g)[NG:-NG, NG:-NG, NG:-NG].ravel()
+                             for g in range(m.nblocks)])
+        fails = sum(int(np.sum(m.get(g, A.DRIVER_FAIL)[NG:-NG, NG:-NG, NG:-NG] == 0.0))
+                    for g in range(m.nblocks))
+        return {"mean_iters": float(np.mean(np.maximum(it, 0))),
+                "hist": np.bincount(np.maximum(it, 0)).tolist(),
+                "unbracketed": int(np.sum(it < 0)), "fail_count_after_fixup_pass": fails}
+
+    K, L = args.c5_steps, args.c5_log_steps
+    cells = None
+    # ---- plain ------------------------------------------------------------------------
+    log(f"[c5] mesh {N} in 32 blocks of {bnx}, {K} RK2 steps, plain")
+    m = make()
+    cells = m.interior_cells
+    tot0 = totals(m)
+    launch_plain = profiled_step(m)
+    wall_a = timed(m, L - 1)
+    wall_b = timed(m, K - L)
+    t_plain, dt_plain, n_plain = m.time()
+    tot_plain = totals(m)
+    it_plain = last_iters(m)
     m.close()
+    wall_all = wall_a + wall_b
+    log(f"[c5] plain: {1e3 * wall_all / (K - 1):.2f} ms/step, steps {L + 1}-{K} "
+        f"{1e3 * wall_b / (K - L):.2f} ms/step, t = {t_plain:.4g}")
+    # ---- learned: log pairs over the first L steps, train, switch ------------------------
+    m = make()
+    logged = []
+    per_block = max(1, (args.c5_pairs // L) // 2)
+    nb_int = bnx[0] * bnx[1] * bnx[2]
+    stride = max(1, nb_int // per_block)
+    for s_ in range(L):
+        m.step()
+        logged.append(mesh_pairs(m, [(2 * s_) % m.nblocks, (2 * s_ + 1) % m.nblocks], stride))
+    data = {k: np.ascontiguousarray(np.concatenate([d[k] for d in logged], axis=-1))
+            for k in logged[0]}
+    data["d_coords"] = DeviceArray.from_host(data["coords"])
+    t0 = time.perf_counter()
+    net, losses = train_bracket_on(phx, p, np.random.default_rng(SEED + 5), data,
+                                   steps=args.c5_train_steps)
+    train_s = time.perf_counter() - t0
+    log(f"[c5] trained on {data['cons'].shape[1]} pairs from steps 1-{L} in {train_s:.1f} s, "
+        f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    m.set_learned(net)
+    m.learned_stats(reset=True)
+    launch_learned = profiled_step(m)
+    wall_l = timed(m, K - L - 1)
+    misses, solves = m.learned_stats()
+    t_l, dt_l, n_l = m.time()
+    tot_l = totals(m)
+    it_l = last_iters(m)
+    m.close()
+    log(f"[c5] learned: steps {L + 2}-{K} {1e3 * wall_l / (K - L - 1):.2f} ms/step, "
+        f"bracket misses {misses}/{solves}")
+    rel = lambda a, b: ((a - b) / np.maximum(np.abs(b), 1e-300)).tolist()
     return {"workload": "FMKS torus 256x128x128, 32 blocks of 64x64x32, WENO5Z + HLL, "
                         "fixups + floors (torus deck), gr_outflow x1/x2 + periodic x3",
-            "steps_timed": steps, "ms_per_step": 1e3 * wall / steps,
-            "zone_updates_per_s": zu, "time": t, "dt": dt, "ncycle": ncycle,
-            "launch_ms_per_stage": launch_ms,
-            "con2prim_last_stage": {"mean_iters": float(np.mean(np.maximum(iters, 0))),
-                                    "hist": np.bincount(np.maximum(iters, 0)).tolist(),
-                                    "unbracketed": int(np.sum(iters < 0)),
-                                    "fail_count_after_fixup_pass": fails},
+            "steps": K, "interior_cells": cells,
+            "plain": {"ms_per_step": 1e3 * wall_all / (K - 1),
+                      "zone_updates_per_s": cells * 2 * (K - 1) / wall_all,
+                      "ms_per_step_after_switch": 1e3 * wall_b / (K - L),
+                      "time": t_plain, "dt": dt_plain, "ncycle": n_plain,
+                      "launch_ms_per_stage": launch_plain, "con2prim_last_stage": it_plain},
+            "learned": {"pairs": int(data["cons"].shape[1]), "log_steps": L,
+                        "train_steps": args.c5_train_steps, "train_s": train_s,
+                        "loss_first_last": [losses[0], losses[-1]],
+                        "ms_per_step_after_switch": 1e3 * wall_l / (K - L - 1),
+                        "zone_updates_per_s_after_switch": cells * 2 * (K - L - 1) / wall_l,
+                        "bracket_misses": misses, "learned_solves": solves,
+                        "miss_rate": misses / max(solves, 1),
+                        "time": t_l, "ncycle": n_l, "launch_ms_per_stage": launch_learned,
+                        "con2prim_last_stage": it_l},
             "conserved_totals_initial": tot0.tolist(),
-            "conserved_totals_final": tot1.tolist(),
-            "conserved_drift_rel": ((tot1 - tot0) / np.maximum(np.abs(tot0), 1e-300)).tolist()}
+            "conserved_drift_plain": (tot_plain - tot0).tolist(),
+            "conserved_drift_learned": (tot_l - tot0).tolist(),
+            "learned_vs_plain_totals_rel": rel(tot_l, tot_plain)}
 
 
 def main():
@@ -513,6 +577,9 @@ def main():
     ap.add_argument("--c4-per-block", type=int, default=1)
     ap.add_argument("--no-c5", action="store_true", help="skip the full torus mesh leg")
     ap.add_argument("--c5-steps", type=int, default=200)
+    ap.add_argument("--c5-log-steps", type=int, default=50)
+    ap.add_argument("--c5-pairs", type=int, default=1 << 22)
+    ap.add_argument("--c5-train-steps", type=int, default=6000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "

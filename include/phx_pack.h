@@ -23,6 +23,7 @@
 
 #include "phx.h"
 #include "phx_fixup.h"
+#include "phx_mlp.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -57,6 +58,13 @@ int phx_pack_nblocks(const phx_pack *pack, int *n, int64_t *interior_cells);
  * interior cells of every block (include/phx_fixup.h for the semantics); the stage's
  * dt_out then sees the signal speeds after the fixups.  Off by default. */
 int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *pack, const phx_fixup_params *fx);
+/* Selects the learned con2prim bracket (include/phx_mlp.h) for the stage's inversion
+ * (mlp non-NULL; the network must outlive its use) or the plain solver (NULL, the
+ * default).  With the network the update and the inversion are two launches, the second
+ * one fusing the MFMA forward pass with the bracketed solve. */
+int phx_pack_set_learned(phx_ctx *ctx, phx_pack *pack, const phx_mlp *mlp);
+/* bracket misses (the plain [0, 1] solve ran) and learned solves since the last reset */
+int phx_pack_learned_stats(phx_pack *pack, int reset, int64_t *misses, int64_t *solves);
 /* One RK stage over the pack (4-10 launches). */
 int phx_pack_stage(phx_ctx *ctx, phx_pack *pack, const phx_pack_stage_args *a,
                    phx_stream stream);

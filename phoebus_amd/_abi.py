@@ -171,6 +171,10 @@ def lib():
         "phx_mesh_get": (i32, [vp, i32, i32, vp]),
         "phx_mesh_get_iters": (i32, [vp, i32, vp]),
         "phx_mesh_autotune": (i32, [vp, i32]),
+        "phx_mesh_set_learned": (i32, [vp, vp]),
+        "phx_mesh_learned_stats": (i32, [vp, i32, P(C.c_int64), P(C.c_int64)]),
+        "phx_pack_set_learned": (i32, [vp, vp, vp]),
+        "phx_pack_learned_stats": (i32, [vp, i32, P(C.c_int64), P(C.c_int64)]),
         "phx_rt_device_count": (i32, [P(i32)]),
         "phx_rt_set_device": (i32, [i32]),
         "phx_rt_malloc": (i32, [P(vp), C.c_size_t]),

@@ -399,6 +399,19 @@ class Mesh:
         self.phx._chk(self.L.phx_mesh_get(self.h, gid, what, out.ctypes.data), "phx_mesh_get")
         return out if what in nv else out[0]
 
+    def set_learned(self, net=None):
+        """every con2prim with the learned bracket of net (an Mlp), or plain (None)."""
+        self._net = net  # kept alive while the mesh uses it
+        self.phx._chk(self.L.phx_mesh_set_learned(self.h, net.h if net is not None else None),
+                      "phx_mesh_set_learned")
+
+    def learned_stats(self, reset=False):
+        """(bracket misses, learned solves) since the last reset."""
+        a, b = C.c_int64(), C.c_int64()
+        self.phx._chk(self.L.phx_mesh_learned_stats(self.h, int(reset), C.byref(a), C.byref(b)),
+                      "phx_mesh_learned_stats")
+        return a.value, b.value
+
     def get_iters(self, gid):
         out = np.zeros(self.shape, np.int32)
         self.phx._chk(self.L.phx_mesh_get_iters(self.h, gid, out.ctypes.data),
@@ -467,6 +480,19 @@ class Pack:
         a = self.args(stage, dt, dt_dev, cfl, dt_out)
         self.phx._chk(self.L.phx_pack_stage(self.phx.h, self.h, C.byref(a), stream),
                       "phx_pack_stage")
+
+    def set_learned(self, net=None):
+        """the stage's con2prim with the learned bracket of net (an Mlp), or plain (None)."""
+        self._net = net
+        self.phx._chk(self.L.phx_pack_set_learned(self.phx.h, self.h,
+                                                  net.h if net is not None else None),
+                      "phx_pack_set_learned")
+
+    def learned_stats(self, reset=False):
+        a, b = C.c_int64(), C.c_int64()
+        self.phx._chk(self.L.phx_pack_learned_stats(self.h, int(reset), C.byref(a), C.byref(b)),
+                      "phx_pack_learned_stats")
+        return a.value, b.value
 
     def set_fixup(self, fx=None):
         """fixup switches after each stage's con2prim (None disables)."""

@@ -6,6 +6,20 @@
 
 namespace phx {
 
+// cell t of the entire block (i fastest); false beyond it
+__device__ __forceinline__ bool entire_cell(const BlockGeom &b, long t, int idx[3]) {
+  if (t >= b.ncell) return false;
+  idx[0] = (int)(t % b.ni);
+  idx[1] = (int)((t / b.ni) % b.nj);
+  idx[2] = (int)(t / ((long)b.ni * b.nj));
+  return true;
+}
+
+__device__ __forceinline__ bool is_interior(const BlockGeom &b, const int idx[3]) {
+  return idx[0] >= b.is[0] && idx[0] <= b.ie[0] && idx[1] >= b.is[1] && idx[1] <= b.ie[1] &&
+         idx[2] >= b.is[2] && idx[2] <= b.ie[2];
+}
+
 // relativity_utils.hpp:54-59 (W from the primitive W v^i)
 template <class M>
 __device__ __forceinline__ double lorentz_factor(const M &g, const double vp[3]) {

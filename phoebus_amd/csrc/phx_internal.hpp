@@ -14,6 +14,7 @@
 #include "phx_device.hpp"
 
 struct phx_ctx;
+struct phx_mlp;
 
 namespace phx {
 
@@ -133,9 +134,15 @@ void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &
 void launch_pack_ct(const PackView &v, long groups, hipStream_t s);
 void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
                          hipStream_t s);
+// do_c2p = false: U_{s+1} only (the inversion then runs in a learned-bracket pass)
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
-                        hipStream_t s);
+                        hipStream_t s, bool do_c2p = true);
+// con2prim with the learned bracket over a pack's cells on an entire-cell work list
+// (phx_mlp.hip): mode 0 the entire blocks, 1 the face-adjacent ghost layer, 2 the interior,
+// 3 the interior and that layer; counters (nullable) accumulate [bracket misses, solves]
+void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
+                             long groups, int mode, unsigned long long *counters, hipStream_t s);
 // the fixups over a pack (phx_fixup.hip): con2prim failure repair on the interior cells,
 // then floors and ceilings (interior cells, or the entire blocks over an entire-cell work
 // list) folded with the interior CFL partial minima (partial may be null)

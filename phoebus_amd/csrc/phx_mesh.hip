@@ -34,20 +34,6 @@ __device__ __forceinline__ void block_loc(const MeshInfo &m, int gid, int lx[3])
   lx[2] = gid / (m.nb[0] * m.nb[1]);
 }
 
-// cell t of the entire block (i fastest); false beyond it
-__device__ __forceinline__ bool entire_cell(const BlockGeom &b, long t, int idx[3]) {
-  if (t >= b.ncell) return false;
-  idx[0] = (int)(t % b.ni);
-  idx[1] = (int)((t / b.ni) % b.nj);
-  idx[2] = (int)(t / ((long)b.ni * b.nj));
-  return true;
-}
-
-__device__ __forceinline__ bool is_interior(const BlockGeom &b, const int idx[3]) {
-  return idx[0] >= b.is[0] && idx[0] <= b.ie[0] && idx[1] >= b.is[1] && idx[1] <= b.ie[1] &&
-         idx[2] >= b.is[2] && idx[2] <= b.ie[2];
-}
-
 // the FillGhost fields of a cell (bc_vars = primitive, fluid.cpp:159-165): primitives,
 // temperature, con2prim root, fail flag and the conserved variables of the container
 constexpr int kNprim = 10, kNcons = 8;
@@ -303,11 +289,22 @@ void launch_exchange_and_boundaries(phx_mesh *m, int st, hipStream_t s) {
   ctx->mark_end(s);
 }
 
+// the robust con2prim of a FillDerived over the stage's output container: mode 0 the
+// entire blocks, 1 the face-adjacent ghost layer, 3 the interior and that layer (learned
+// bracket only: the plain interior inversion is fused into the update)
 void launch_c2p(phx_mesh *m, int st, int mode, hipStream_t s) {
+  phx_pack *k = m->pack;
+  if (k->mlp) {
+    m->ctx->mark_begin(mode == 0 ? "con2prim_learned" : "con2prim_learned_first", s);
+    launch_pack_c2p_learned(m->p.geom.kind, k->dp, k->mlp, pv(m, st, 5), groups(m, 5), mode,
+                            k->counters, s);
+    m->ctx->mark_end(s);
+    return;
+  }
   m->ctx->mark_begin(mode == 0 ? "con2prim" : "con2prim_ghost", s);
   with_geom(m->p.geom.kind, [&](auto G) {
     hipLaunchKernelGGL((k_pack_con2prim<decltype(G)::value>), dim3((unsigned)groups(m, 5)),
-                       dim3(256), 0, s, m->pack->dp, pv(m, st, 5), mode);
+                       dim3(256), 0, s, k->dp, pv(m, st, 5), mode);
   });
   m->ctx->mark_end(s);
 }
@@ -335,13 +332,14 @@ int run_stage(phx_mesh *m, int st, bool last, hipStream_t s) {
                        pv(m, st, 5), m->outflow_mask);
     ctx->mark_end(s);
   }
-  ctx->mark_begin("update_con2prim", s);
+  const bool learned = k->mlp != nullptr;
+  ctx->mark_begin(learned ? "update" : "update_con2prim", s);
   launch_pack_update(k->p.geom.kind, k->dp, pv(m, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
-                     m->dt, nullptr, nullptr, s);
+                     m->dt, nullptr, nullptr, s, !learned);
   ctx->mark_end(s);
-  // the first FillDerived: the interior was inverted by the update; the fixup also reads
-  // the face-adjacent ghost layer
-  launch_c2p(m, st, 1, s);
+  // the first FillDerived: the interior (inverted by the update unless the learned bracket
+  // is on) and the face-adjacent ghost layer the fixup reads
+  launch_c2p(m, st, learned ? 3 : 1, s);
   ctx->mark_begin("c2p_fixup", s);
   launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], s);
   ctx->mark_end(s);
@@ -605,6 +603,16 @@ int phx_mesh_get_iters(const phx_mesh *m, int gid, int32_t *host) {
                    hipMemcpyDeviceToHost) == hipSuccess
              ? PHX_OK
              : PHX_ERR_DEVICE;
+}
+
+int phx_mesh_set_learned(phx_mesh *m, const phx_mlp *mlp) {
+  if (!m) return PHX_ERR_INVALID;
+  return phx_pack_set_learned(m->ctx, m->pack, mlp);
+}
+
+int phx_mesh_learned_stats(phx_mesh *m, int reset, int64_t *misses, int64_t *solves) {
+  if (!m) return PHX_ERR_INVALID;
+  return phx_pack_learned_stats(m->pack, reset, misses, solves);
 }
 
 int phx_mesh_autotune(phx_mesh *m, int reps) {

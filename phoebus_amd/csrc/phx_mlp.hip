@@ -41,6 +41,7 @@
 #include "phx.h"
 #include "phx_device.hpp"
 #include "phx_internal.hpp"
+#include "phx_cells.hpp"
 #include "phx_mlp.h"
 
 namespace phx {
@@ -567,6 +568,121 @@ __global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
   }
 }
 
+// con2prim with the learned bracket over the cells of a pack (work list over entire
+// blocks): the cell-field semantics of the plain pass (con2prim_cell: NaN input -> fail,
+// every other field kept; an unbracketed root -> fail with iters -1), the bracket and
+// fallback of k_con2prim_batch_learned.  mode: 0 entire blocks, 1 the face-adjacent
+// ghost layer, 2 the interior, 3 both.
+template <int GEOM>
+__global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
+    DevParams p, MlpDev m, PackView v, int mode, unsigned long long *counters) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<true>;
+  __shared__ uint64_t wfwd[2 * kFwdFrags * 64];
+  __shared__ float wbias[kNBias];
+  for (int q = threadIdx.x; q < 2 * kFwdFrags * 64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
+  for (int q = threadIdx.x; q < kNBias; q += kLearnedThreads) wbias[q] = m.bias[q];
+  __syncthreads();
+  MlpDev ml = m;
+  ml.fwd = wfwd;
+  ml.bias = wbias;
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  const BlockGeom &bg = E.bg;
+  int idx[3] = {0, 0, 0};
+  bool have = entire_cell(bg, (g - v.start[b]) * kLearnedThreads + threadIdx.x, idx);
+  bool inner = false;
+  if (have) {
+    int out = 0, layer = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] < bg.is[d] || idx[d] > bg.ie[d]) ++out;
+      if (idx[d] == bg.is[d] - 1 || idx[d] == bg.ie[d] + 1) ++layer;
+    }
+    inner = out == 0;
+    const bool face_layer = out == 1 && layer == 1;
+    have = mode == 0 || (mode == 1 && face_layer) || (mode == 2 && inner) ||
+           (mode == 3 && (inner || face_layer));
+  }
+  // a re-inversion of the entire blocks keeps the interior's iteration counts
+  int32_t *iters = (mode == 0 && inner) ? nullptr : E.iters;
+  const long n = bg.ncell;
+  const long c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
+  // phase 1: features of this lane's cell
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  if (have) {
+    const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
+    const M gm(p, X[0], X[1], X[2]);
+    double cs[L::ncons], pr[L::nprim];
+#pragma unroll
+    for (int q = 0; q < L::ncons; ++q) cs[q] = E.u_out[q * n + c];
+    C2PState z;
+    if (c2p_setup<true>(p, gm, X, cs, pr, z)) zone_features(ml, z.res, f);
+  }
+  // phase 2: the network for the wave's 64 cells
+  float o1, o2;
+  mlp_predict_wave(ml, f, o1, o2);
+  if (!have) return;
+  // phase 3: the set-up again, then the bracketed solve
+  double *cons = E.u_out, *prim = E.prim_out;
+  double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+  for (int q = 0; q < L::ncons; ++q) cs[q] = cons[q * n + c];
+  if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
+    E.fail[c] = 0.0;
+    if (iters) iters[c] = 0;
+    return;
+  }
+  const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
+  const M gm(p, X[0], X[1], X[2]);
+#pragma unroll
+  for (int q = 0; q < L::nprim; ++q) pr[q] = prim[q * n + c];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sig[d] = d < bg.ndim ? E.csig[d * n + c] : 0.0;
+  double tt = E.temp ? E.temp[c] : 0.0, root = E.mu[c];
+  C2PState z;
+  c2p_setup<true>(p, gm, X, cs, pr, z);
+  const double cc = exp((double)o1) / (double)prior_z0(ml, f[0], f[2]);
+  const double w = cc * fmax((double)ml.width_scale * exp((double)o2), (double)ml.width_min);
+  const double lo = fmax(cc - w, 0.0), hi = fmin(cc + w, 1.0);
+  const double ylo = z.res(lo), yhi = z.res(hi);
+  int it = 0;
+  bool good = true, fb = false;
+  double mu_new = root;
+  if (ylo * yhi <= 0.0) {
+    mu_new = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
+  } else {
+    fb = true;
+    good = regula_falsi(z.res, 0.0, 1.0, p.tol, cc, p.max_iter, mu_new, it);
+  }
+  double st = 0.0;
+  if (good) {
+    root = mu_new;
+    st = c2p_unwrap<true>(p, gm, z, root, cs, pr, tt, sig);
+  } else {
+    it = -1;
+  }
+  if (counters) {
+    if (fb) atomicAdd(&counters[0], 1ull);
+    const unsigned long long act = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1)
+      atomicAdd(&counters[1], (unsigned long long)__popcll(act));
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) cons[q * n + c] = cs[q];
+#pragma unroll
+  for (int q = 0; q < L::nprim; ++q) prim[q * n + c] = pr[q];
+  for (int d = 0; d < bg.ndim; ++d) E.csig[d * n + c] = sig[d];
+  if (E.temp) E.temp[c] = tt;
+  E.mu[c] = root;
+  E.fail[c] = st;
+  if (iters) iters[c] = it;
+}
+
+void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
+                             long groups, int mode, unsigned long long *counters, hipStream_t s);
+
 // features of batched states (training data and tests)
 template <bool MHD, int GEOM>
 __global__ __launch_bounds__(256) void k_c2p_features(DevParams p, MlpDev m, long n,
@@ -620,6 +736,17 @@ static MlpDev dev_view(const phx_mlp *m) {
   d.width_scale = m->width_scale;
   d.width_min = m->width_min;
   return d;
+}
+
+void phx::launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m,
+                                  const PackView &v, long groups, int mode,
+                                  unsigned long long *counters, hipStream_t s) {
+  if (groups <= 0) return;
+  const MlpDev md = dev_view(m);
+  with_geom(geom, [&](auto G) {
+    hipLaunchKernelGGL((k_pack_con2prim_learned<decltype(G)::value>), dim3((unsigned)groups),
+                       dim3(kLearnedThreads), 0, s, dp, md, v, mode, counters);
+  });
 }
 
 static void repack(phx_mlp *m, hipStream_t s) {

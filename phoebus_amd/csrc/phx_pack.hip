@@ -76,15 +76,22 @@ void launch_stage_fluxes(phx_ctx *ctx, phx_pack *k, int st, hipStream_t s) {
 void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipStream_t s) {
   const int st = a->stage;
   launch_stage_fluxes(ctx, k, st, s);
-  // with fixups on, the CFL partials are taken after them (the repaired and floored cells'
-  // signal speeds), in the floors launch
+  // with fixups on or the learned inversion, the CFL partials are taken after them (the
+  // repaired and floored cells' signal speeds), in the floors launch
+  const bool learned = k->mlp != nullptr;
   const bool fixups = k->fx.c2p_fixup || k->fx.floors;
   double *partial = a->dt_out ? k->partial : nullptr;
-  ctx->mark_begin("update_con2prim", s);
+  ctx->mark_begin(learned ? "update" : "update_con2prim", s);
   launch_pack_update(k->p.geom.kind, k->dp, view(k, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
-                     a->dt, a->dt_dev, fixups ? nullptr : partial, s);
+                     a->dt, a->dt_dev, (fixups || learned) ? nullptr : partial, s, !learned);
   ctx->mark_end(s);
-  if (fixups) {
+  if (learned) {
+    ctx->mark_begin("con2prim_learned", s);
+    launch_pack_c2p_learned(k->p.geom.kind, k->dp, k->mlp, view(k, st, 5), k->groups[5], 2,
+                            k->counters, s);
+    ctx->mark_end(s);
+  }
+  if (fixups || learned) {
     if (k->fx.c2p_fixup) {
       ctx->mark_begin("c2p_fixup", s);
       launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, view(k, st, 4), k->groups[4], s);
@@ -176,6 +183,8 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
   for (int kind = 0; kind < 6 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
   ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max({k->groups[4], k->groups[5], 1L})) ==
                  hipSuccess;
+  ok = ok && hipMalloc(&k->counters, 2 * sizeof(unsigned long long)) == hipSuccess &&
+       hipMemset(k->counters, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
   if (!ok) {
     phx_pack_destroy(k);
     return ctx->fail(PHX_ERR_DEVICE, "pack set-up failed");
@@ -192,6 +201,7 @@ int phx_pack_destroy(phx_pack *k) {
   for (auto &s : k->start)
     if (s) (void)hipFree(s);
   if (k->partial) (void)hipFree(k->partial);
+  if (k->counters) (void)hipFree(k->counters);
   delete k;
   return PHX_OK;
 }
@@ -212,6 +222,24 @@ int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *k, const phx_fixup_params *fx) {
   const int rc = check_fixup(ctx, fx);
   if (rc) return rc;
   k->fx = make_dev_fixup(&k->p, fx);
+  return PHX_OK;
+}
+
+int phx_pack_set_learned(phx_ctx *ctx, phx_pack *k, const phx_mlp *mlp) {
+  if (!ctx || !k) return PHX_ERR_INVALID;
+  k->mlp = mlp;
+  return PHX_OK;
+}
+
+int phx_pack_learned_stats(phx_pack *k, int reset, int64_t *misses, int64_t *solves) {
+  if (!k) return PHX_ERR_INVALID;
+  unsigned long long h[2] = {0, 0};
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(h, k->counters, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+    return PHX_ERR_DEVICE;
+  if (misses) *misses = (int64_t)h[0];
+  if (solves) *solves = (int64_t)h[1];
+  if (reset && hipMemset(k->counters, 0, sizeof(h)) != hipSuccess) return PHX_ERR_DEVICE;
   return PHX_OK;
 }
 

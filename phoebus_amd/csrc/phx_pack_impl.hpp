@@ -29,6 +29,8 @@ struct phx_pack {
   TileShape tiles[3];
   double *partial = nullptr;  // [groups[4]] CFL partial minima
   DevFixup fx{};              // c2p fixup and floors after the update (off by default)
+  const phx_mlp *mlp = nullptr;          // learned con2prim bracket (null: plain solver)
+  unsigned long long *counters = nullptr;  // device [2]: bracket misses, learned solves
 };
 
 struct phx_graph {

@@ -20,7 +20,7 @@ __device__ __forceinline__ double update_con2prim_body(
     const double *__restrict__ src, double *u_stage, const double *u_base, double beta,
     double bdt, double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
-    double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt) {
+    double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt, bool do_c2p = true) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
   constexpr int NC = L::ncons;
@@ -56,6 +56,11 @@ __device__ __forceinline__ double update_con2prim_body(
       const double us = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
       u_stage[(long)l * n + c] = us;
       cs[l] = us + bdt * dudt;
+    }
+    if (!do_c2p) {  // the inversion runs in a separate (learned-bracket) pass
+#pragma unroll
+      for (int l = 0; l < NC; ++l) u_out[(long)l * n + c] = cs[l];
+      return DBL_MAX;
     }
     // ConservedToPrimitive on U_{s+1}
     double pr[L::nprim], sig[3] = {0.0, 0.0, 0.0};
@@ -116,7 +121,7 @@ __global__ __launch_bounds__(256) void k_update_con2prim(
 template <int GEOM>
 __global__ __launch_bounds__(256) void k_pack_update(DevParams p, PackView v, double beta,
                                                      double dt, const double *dt_dev,
-                                                     double *__restrict__ partial) {
+                                                     double *__restrict__ partial, int do_c2p) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
@@ -124,7 +129,7 @@ __global__ __launch_bounds__(256) void k_pack_update(DevParams p, PackView v, do
   const double val = update_con2prim_body<true, GEOM>(
       p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.src, E.u_stage, E.u_base, beta,
       bdt, E.u_out, E.prim_out, E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
-      partial != nullptr);
+      partial != nullptr, do_c2p != 0);
   if (partial) {
     const double m = block_min(val);
     if (threadIdx.x == 0) partial[g] = m;
@@ -133,11 +138,11 @@ __global__ __launch_bounds__(256) void k_pack_update(DevParams p, PackView v, do
 
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
-                        hipStream_t s) {
+                        hipStream_t s, bool do_c2p) {
   if (groups <= 0) return;
   with_geom(geom, [&](auto G) {
     hipLaunchKernelGGL((k_pack_update<decltype(G)::value>), dim3((unsigned)groups), dim3(256), 0,
-                       s, dp, v, beta, dt, dt_dev, partial);
+                       s, dp, v, beta, dt, dt_dev, do_c2p ? partial : nullptr, do_c2p ? 1 : 0);
   });
 }
 

@@ -42,9 +42,19 @@ def features(net, p, data, n):
 def train_bracket(phx, p, rng, n_pairs=1 << 22, steps=6000, batch=16384, lr=2e-3, lam=0.05,
                   width_scale=4.0, width_min=1e-7, seed=3, log=None):
     """A trained Mlp (phoebus_amd.api.Mlp) and the loss history."""
+    data = solver_pairs(phx, p, rng, n_pairs)
+    return train_bracket_on(phx, p, rng, data, steps=steps, batch=batch, lr=lr, lam=lam,
+                            width_scale=width_scale, width_min=width_min, seed=seed, log=log)
+
+
+def train_bracket_on(phx, p, rng, data, steps=6000, batch=16384, lr=2e-3, lam=0.05,
+                     width_scale=4.0, width_min=1e-7, seed=3, log=None):
+    """Train on solver pairs data = dict(coords [2][n] (r, theta or X1, X2), cons [8][n],
+    mu [n], fail [n], d_coords = coords on the device): the features are standardized with
+    the pairs' own mean and spread."""
     from .api import Mlp
 
-    data = solver_pairs(phx, p, rng, n_pairs)
+    n_pairs = data["cons"].shape[1]
     keep = data["fail"] == 1.0
     net = Mlp(phx, seed=seed)
     net.set_normalization([0, 0, 0, 0], [1, 1, 1, 1])
@@ -53,6 +63,7 @@ def train_bracket(phx, p, rng, n_pairs=1 << 22, steps=6000, batch=16384, lr=2e-3
     net.set_normalization(mean, std, width_scale, width_min)
     f = features(net, p, data, n_pairs)[:, keep]
     mu = data["mu"][keep].astype(np.float32)
+    batch = min(batch, f.shape[1])
     nb = max(1, f.shape[1] // batch)
     perm = rng.permutation(f.shape[1])[:nb * batch]
     fb = np.ascontiguousarray(f[:, perm].reshape(4, nb, batch).transpose(1, 0, 2))
@@ -73,3 +84,28 @@ def train_bracket(phx, p, rng, n_pairs=1 << 22, steps=6000, batch=16384, lr=2e-3
     synchronize()
     net.norm = (mean, std)
     return net, losses
+
+
+def mesh_pairs(mesh, gids, stride=1):
+    """Solver pairs from the interior cells of the blocks gids of a mesh (phoebus_amd.api.Mesh)
+    after a step: the conserved state, its con2prim root and fail flag, and the cells'
+    (X1, X2) -- the metric of the torus meshes does not depend on X3."""
+    from . import _abi as A
+
+    out = {"coords": [], "cons": [], "mu": [], "fail": []}
+    for g in gids:
+        b = mesh.blocks[g]
+        ng = b.ng
+        I = (slice(ng, ng + b.nx[2]), slice(ng, ng + b.nx[1]), slice(ng, ng + b.nx[0]))
+        cons = mesh.get(g, A.DRIVER_CONS)[(slice(None),) + I].reshape(8, -1)[:, ::stride]
+        mu = mesh.get(g, A.DRIVER_C2P_MU)[I].reshape(-1)[::stride]
+        fail = mesh.get(g, A.DRIVER_FAIL)[I].reshape(-1)[::stride]
+        x1 = b.xmin[0] + (np.arange(b.nx[0]) + 0.5) * b.dx[0]
+        x2 = b.xmin[1] + (np.arange(b.nx[1]) + 0.5) * b.dx[1]
+        X1 = np.broadcast_to(x1[None, None, :], (b.nx[2], b.nx[1], b.nx[0])).reshape(-1)[::stride]
+        X2 = np.broadcast_to(x2[None, :, None], (b.nx[2], b.nx[1], b.nx[0])).reshape(-1)[::stride]
+        out["coords"].append(np.stack([X1, X2]))
+        out["cons"].append(cons)
+        out["mu"].append(mu)
+        out["fail"].append(fail)
+    return {k: np.concatenate(v, axis=-1) for k, v in out.items()}
