@@ -379,9 +379,58 @@ def c4_uneven_pack(phx, args, log):
 
     ms_blocks = timed(per_block) if args.c4_per_block else None
     g.close()
-    it_h = it_arena.get()
-    its = np.concatenate([it_h[int(offs[i]):int(offs[i + 1])].reshape(shapes[i])[NG:-NG, NG:-NG, NG:-NG].ravel()
-                          for i in range(len(blocks))])
+
+    def interior_iters():
+        it_h = it_arena.get()
+        return np.concatenate([it_h[int(offs[i]):int(offs[i + 1])].reshape(shapes[i])[NG:-NG, NG:-NG, NG:-NG].ravel()
+                               for i in range(len(blocks))])
+
+    its = interior_iters()
+    learned = None
+    if not args.no_c4_learned:
+        # the learned bracket: pairs (conserved state, plain root, (X1, X2)) from the
+        # interiors of every 4th block after the plain steps, the network trained on them
+        # on the device, then the same graph steps with every inversion through it
+        from phoebus_amd.learned import train_bracket_on
+        coords, cons, mu, fl = [], [], [], []
+        for bidx in range(0, len(blocks), 4):
+            b, ab, _, _ = entries[bidx]
+            n = b.nx[0]
+            I = (slice(NG, NG + n),) * 3
+            cons.append(ab.cons.get()[(slice(None),) + I].reshape(8, -1))
+            mu.append(ab.mu.get()[I].reshape(-1))
+            fl.append(ab.fail.get()[I].reshape(-1))
+            x1 = b.xmin[0] + (np.arange(n) + 0.5) * b.dx[0]
+            x2 = b.xmin[1] + (np.arange(n) + 0.5) * b.dx[1]
+            coords.append(np.stack([np.broadcast_to(x1[None, None, :], (n, n, n)).reshape(-1),
+                                    np.broadcast_to(x2[None, :, None], (n, n, n)).reshape(-1)]))
+        data = {"coords": np.ascontiguousarray(np.concatenate(coords, axis=1)),
+                "cons": np.ascontiguousarray(np.concatenate(cons, axis=1)),
+                "mu": np.concatenate(mu), "fail": np.concatenate(fl)}
+        if data["mu"].size > args.c3_pairs:
+            sel = np.random.default_rng(SEED + 6).choice(data["mu"].size, args.c3_pairs, replace=False)
+            data = {k: np.ascontiguousarray(v[..., sel]) for k, v in data.items()}
+        data["d_coords"] = DeviceArray.from_host(data["coords"])
+        t0 = time.perf_counter()
+        net, losses = train_bracket_on(phx, p, np.random.default_rng(SEED + 7), data, steps=6000)
+        train_s = time.perf_counter() - t0
+        pk.set_learned(net)
+        gl = pk.graph(stages, s)
+        pk.learned_stats(reset=True)
+        ms_learned = timed(lambda: gl.launch(s))
+        misses, solves = pk.learned_stats()
+        its_l = interior_iters()
+        gl.close()
+        pk.set_learned(None)
+        learned = {"pairs": int(data["mu"].size), "train_s": train_s,
+                   "loss_first_last": [losses[0], losses[-1]],
+                   "ms_per_rk2_step_graph": ms_learned,
+                   "zone_updates_per_s_graph": 2 * interior / (ms_learned * 1e-3),
+                   "con2prim_mean_iters": float(np.mean(np.maximum(its_l, 0))),
+                   "con2prim_hist": np.bincount(np.maximum(its_l, 0)).tolist(),
+                   "bracket_miss_rate": misses / max(solves, 1)}
+        log(f"c4: learned graph {ms_learned:.3f} ms/step, iters {learned['con2prim_mean_iters']:.2f}"
+            f" vs plain {float(np.mean(np.maximum(its, 0))):.2f}, misses {misses}/{solves}")
     phx.L.phx_rt_stream_destroy(s)
     pk.close()
     zu = 2 * interior
@@ -396,7 +445,9 @@ def c4_uneven_pack(phx, args, log):
            "ms_per_rk2_step_graph": ms_graph, "ms_per_rk2_step_eager_pack": ms_eager,
            "ms_per_rk2_step_per_block_launches": ms_blocks,
            "hbm_frac_stage_bytes": zu / (ms_graph * 1e-3) * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS,
-           "flux_tiles_yz": tiles, "con2prim_mean_iters": float(its.mean()),
+           "flux_tiles_yz": tiles, "con2prim_mean_iters": float(np.mean(np.maximum(its, 0))),
+           "con2prim_hist": np.bincount(np.maximum(its, 0)).tolist(),
+           "learned": learned,
            "con2prim_fail_zones_interior": {"setup": fails_setup, f"after_{steps + 1}_steps": fail_graph},
            "steps": steps}
     log(f"c4: graph {ms_graph:.3f} ms/step, eager {ms_eager:.3f}, per-block {ms_blocks}")
@@ -575,6 +626,7 @@ def main():
     ap.add_argument("--c4-blocks", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=5)
     ap.add_argument("--c4-per-block", type=int, default=1)
+    ap.add_argument("--no-c4-learned", action="store_true", help="skip C4's learned-bracket run")
     ap.add_argument("--no-c5", action="store_true", help="skip the full torus mesh leg")
     ap.add_argument("--c5-steps", type=int, default=200)
     ap.add_argument("--c5-log-steps", type=int, default=50)
