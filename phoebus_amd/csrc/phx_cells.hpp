@@ -20,6 +20,44 @@ __device__ __forceinline__ bool is_interior(const BlockGeom &b, const int idx[3]
          idx[2] >= b.is[2] && idx[2] <= b.ie[2];
 }
 
+// cell t of the ghost layer face-adjacent to the interior of a 3-D block (the cells the c2p
+// fixup reads besides the interior): the x1 faces' layers, then x2's, then x3's, each
+// inner side first; false beyond them
+__device__ __forceinline__ long face_layer_cells(const BlockGeom &b) {
+  const long n0 = b.ie[0] - b.is[0] + 1, n1 = b.ie[1] - b.is[1] + 1, n2 = b.ie[2] - b.is[2] + 1;
+  return 2 * (n1 * n2 + n0 * n2 + n0 * n1);
+}
+__device__ __forceinline__ bool face_layer_cell(const BlockGeom &b, long t, int idx[3]) {
+  const int n[3] = {b.ie[0] - b.is[0] + 1, b.ie[1] - b.is[1] + 1, b.ie[2] - b.is[2] + 1};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int e0 = d == 0 ? 1 : 0, e1 = d == 2 ? 1 : 2;  // the other two dimensions
+    const long face = (long)n[e0] * n[e1];
+    if (t < 2 * face) {
+      const int side = t >= face;
+      const long r = t - side * face;
+      idx[d] = side ? b.ie[d] + 1 : b.is[d] - 1;
+      idx[e0] = (int)(r % n[e0]) + b.is[e0];
+      idx[e1] = (int)(r / n[e0]) + b.is[e1];
+      return true;
+    }
+    t -= 2 * face;
+  }
+  return false;
+}
+
+// work-list kinds of the pack cell kernels: 4 interior, 5 entire blocks, 6 face layer
+__device__ __forceinline__ bool cell_of_kind(int kind, const BlockGeom &b, long t, int idx[3]) {
+  if (kind == 5) return entire_cell(b, t, idx);
+  if (kind == 6) return face_layer_cell(b, t, idx);
+  const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
+  if (t >= (long)ni * nj * nk) return false;
+  idx[0] = (int)(t % ni) + b.is[0];
+  idx[1] = (int)((t / ni) % nj) + b.is[1];
+  idx[2] = (int)(t / ((long)ni * nj)) + b.is[2];
+  return true;
+}
+
 // relativity_utils.hpp:54-59 (W from the primitive W v^i)
 template <class M>
 __device__ __forceinline__ double lorentz_factor(const M &g, const double vp[3]) {
@@ -62,7 +100,7 @@ __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeo
     return;
   }
   const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  const M g(p, X[0], X[1], X[2]);
+  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
 #pragma unroll
@@ -92,7 +130,7 @@ __device__ __forceinline__ void prim_to_con_cell(const DevParams &p, const Block
   using L = Layout<MHD>;
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
-  const M g(p, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
+  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
   double pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];

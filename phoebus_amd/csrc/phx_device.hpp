@@ -47,6 +47,10 @@ struct BlockGeom {
   long ncell;
   double xmin[3], dx[3];
   int istart[3];     // parthenon's istart: ng in active dimensions, 0 otherwise
+  // geometry cached per (X1, X2) plane for metrics that do not depend on X3 (FMKS), or
+  // null: [kGeoCentre][nj][ni] at cell centres (metric, derivatives, inverse 4-metric),
+  // then [kGeoMetric][nj][ni] at x1 faces and at x2 faces
+  const double *geo;
 };
 
 // The hardware hands consecutive workgroups of a launch to the 8 XCDs in turn, and each
@@ -359,6 +363,24 @@ struct FMKSMetric {
   }
   __device__ double detgamma() const { return detgamma_; }
   __device__ double detg() const { return detg_; }
+  // the metric as 19 numbers t[q * s], q = 0..18 (a cached geometry plane)
+  __device__ FMKSMetric() {}
+  __device__ void store(double *t, long s) const {
+    const double v[19] = {alpha_, b0_, b1_, g00_, g01_, g03_, g11_, g12_, g13_, g22_,
+                          g33_, c00_, c01_, c02_, c11_, c12_, c22_, detgamma_, detg_};
+#pragma unroll
+    for (int q = 0; q < 19; ++q) t[q * s] = v[q];
+  }
+  static __device__ FMKSMetric load(const double *__restrict__ t, long s) {
+    FMKSMetric g;
+    g.alpha_ = t[0]; g.b0_ = t[s]; g.b1_ = t[2 * s]; g.g00_ = t[3 * s]; g.g01_ = t[4 * s];
+    g.g03_ = t[5 * s]; g.g11_ = t[6 * s]; g.g12_ = t[7 * s]; g.g13_ = t[8 * s];
+    g.g22_ = t[9 * s]; g.g33_ = t[10 * s]; g.c00_ = t[11 * s]; g.c01_ = t[12 * s];
+    g.c02_ = t[13 * s]; g.c11_ = t[14 * s]; g.c12_ = t[15 * s]; g.c22_ = t[16 * s];
+    g.detgamma_ = t[17 * s]; g.detg_ = t[18 * s];
+    g.j00_ = g.j10_ = g.j11_ = 0.0;
+    return g;
+  }
 };
 
 template <int GEOM>
@@ -369,6 +391,37 @@ template <>
 struct MetricOf<PHX_GEOM_SPHERICAL_KS> { using type = SphericalKSMetric; };
 template <>
 struct MetricOf<PHX_GEOM_FMKS> { using type = FMKSMetric; };
+
+// the cached geometry plane: 19 metric numbers, then (centres only) the x1 and x2
+// derivatives of the 10 g_{mu nu} (upper triangle, row by row), d ln alpha / dX1, dX2, and
+// the 10 g^{mu nu}
+constexpr int kGeoMetric = 19, kGeoCentre = 19 + 20 + 2 + 10;
+constexpr int kGeoPlanes = kGeoCentre + 2 * kGeoMetric;
+
+// The metric of cell-plane point q = j * ni + i of block b at location loc (0 cell centre,
+// 1 x1 face, 2 x2 face; an x3 face has the centre's metric when it does not depend on X3):
+// from the block's cached plane when it has one, else evaluated at X.  CACHED = true
+// promises the plane (packs of FMKS blocks always have one), so no evaluation code is
+// compiled in.
+template <int GEOM, bool CACHED = false>
+__device__ __forceinline__ typename MetricOf<GEOM>::type metric_at(const DevParams &p,
+                                                                   const BlockGeom &b, int loc,
+                                                                   long q, double X1, double X2,
+                                                                   double X3) {
+  using M = typename MetricOf<GEOM>::type;
+  if constexpr (GEOM == PHX_GEOM_FMKS) {
+    const long plane = (long)b.ni * b.nj;
+    const long off = loc == 0 ? 0 : loc == 1 ? kGeoCentre : kGeoCentre + kGeoMetric;
+    if constexpr (CACHED) {
+      return M::load(b.geo + off * plane + q, plane);
+    } else {
+      if (b.geo) return M::load(b.geo + off * plane + q, plane);
+      return M(p, X1, X2, X3);
+    }
+  } else {
+    return M(p, X1, X2, X3);
+  }
+}
 
 // ---- bounds: fixup.hpp:91-134, 188-213 ------------------------------------------------
 __device__ __forceinline__ void get_floors(const DevBounds &b, double x1, double x2,

@@ -72,7 +72,7 @@ __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixu
     for (int m = 0; m < 3; ++m) pr[1 + m] = 0.;
   }
   const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  const M g(p, X[0], X[1], X[2]);
+  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
   const double gamma_max = p.bnd.gam_max;
   double vpcon[3] = {pr[1], pr[2], pr[3]};
   const double W = lorentz_factor(g, vpcon);
@@ -120,7 +120,7 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
   const double gamma_max = p.bnd.gam_max, e_max = p.bnd.sie_max;
   double rho_floor_max = rho_floor;
   double u_floor_max = rho_floor * sie_floor;
-  const M g(p, X[0], X[1], X[2]);
+  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
   double pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];

@@ -31,13 +31,15 @@ struct ReconWidth {
 // which more than doubles the register footprint of the kernel.
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 
-template <bool MHD, int RIEMANN, class M>
+template <bool MHD, int RIEMANN, int GEOM, bool CACHED>
 __device__ __forceinline__ void face_out(const DevParams &p, int dir, const BlockGeom &b,
                                          const double X[3], const double *ql, const double *qr,
-                                         long c, double *__restrict__ flux,
+                                         long c, long q, double *__restrict__ flux,
                                          double *__restrict__ fsig) {
   constexpr int NC = Layout<MHD>::ncons;
-  const M g(p, X[0], X[1], X[2]);
+  // the face's metric (x3 faces share the centre's when the metric ignores X3)
+  const auto g =
+      metric_at<GEOM, CACHED>(p, b, dir == 0 ? 1 : dir == 1 ? 2 : 0, q, X[0], X[1], X[2]);
   double f[NC];
   const double s = riemann_flux<MHD, RIEMANN>(p, dir, g, X, ql, qr, f);
 #pragma unroll
@@ -61,7 +63,7 @@ __host__ __device__ inline long flux_x_groups(const BlockGeom &b, int H) {
 }
 
 // the x sweep for logical group bx of block b; stage = LDS [NP][XB]
-template <int RECON, int RIEMANN, bool MHD, int GEOM>
+template <int RECON, int RIEMANN, bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void flux_x_body(const DevParams &p, const BlockGeom &b, long bx,
                                             const double *__restrict__ prim,
                                             double *__restrict__ flux,
@@ -110,7 +112,7 @@ __device__ __forceinline__ void flux_x_body(const DevParams &p, const BlockGeom 
 #pragma unroll
   for (int v = 0; v < NP; ++v) ql[v] = stage[v][t - 1];
   const double X[3] = {xf(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  face_out<MHD, RIEMANN, M>(p, 0, b, X, ql, lo, c, flux, fsig);
+  face_out<MHD, RIEMANN, GEOM, CACHED>(p, 0, b, X, ql, lo, c, (long)j * b.ni + i, flux, fsig);
 }
 
 template <int RECON, int RIEMANN, bool MHD, int GEOM>
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
 // (narrow ti_n: a row segment of ti_n doubles uses ti_n/16 of a 128-byte line); it is
 // chosen per block size and reconstruction by the autotuner (phx_autotune), else by the
 // default below.  Blocks are renumbered XCD by XCD so neighbouring i chunks share an L2.
-constexpr int kTileMaxThreads = 1024;
+constexpr int kTileMaxThreads = 512;
 constexpr int kTileMaxRows = 256;
 constexpr int kTileMaxLdsBytes = 64 * 1024;  // two blocks per CU
 
@@ -147,7 +149,7 @@ __host__ __device__ inline long flux_tile_groups(const BlockGeom &b, int dir, in
 }
 
 // the DIR sweep for logical tile bx of block b; lds = [NP][ts_n][ti_n] doubles
-template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void flux_tile_body(const DevParams &p, const BlockGeom &b, int bx,
                                                int ti_n, int ts_n,
                                                const double *__restrict__ prim,
@@ -213,11 +215,12 @@ __device__ __forceinline__ void flux_tile_body(const DevParams &p, const BlockGe
   X[0] = xc(b, 0, i);
   X[T] = xc(b, T, t);
   X[DIR] = xf(b, DIR, o);
-  face_out<MHD, RIEMANN, M>(p, DIR, b, X, ql, lo, c, flux, fsig);
+  face_out<MHD, RIEMANN, GEOM, CACHED>(p, DIR, b, X, ql, lo, c,
+                                       (long)(DIR == 1 ? o : t) * b.ni + i, flux, fsig);
 }
 
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
-__global__ __launch_bounds__(1024) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
+__global__ __launch_bounds__(512) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
                                                     int ts_n, const double *__restrict__ prim,
                                                     double *__restrict__ flux,
                                                     double *__restrict__ fsig) {
@@ -233,18 +236,18 @@ __global__ __launch_bounds__(XB) void k_pack_flux_x(DevParams p, PackView v) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
-  flux_x_body<RECON, RIEMANN, true, GEOM>(p, E.bg, g - v.start[b], E.prim, E.flux, E.fsig,
+  flux_x_body<RECON, RIEMANN, true, GEOM, GEOM == PHX_GEOM_FMKS>(p, E.bg, g - v.start[b], E.prim, E.flux, E.fsig,
                                           stage);
 }
 
 template <int DIR, int RECON, int RIEMANN, int GEOM>
-__global__ __launch_bounds__(1024) void k_pack_flux_tile(DevParams p, PackView v, int ti_n,
+__global__ __launch_bounds__(512) void k_pack_flux_tile(DevParams p, PackView v, int ti_n,
                                                          int ts_n) {
   extern __shared__ double lds[];
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
-  flux_tile_body<DIR, RECON, RIEMANN, true, GEOM>(p, E.bg, (int)(g - v.start[b]), ti_n, ts_n,
+  flux_tile_body<DIR, RECON, RIEMANN, true, GEOM, GEOM == PHX_GEOM_FMKS>(p, E.bg, (int)(g - v.start[b]), ti_n, ts_n,
                                                   E.prim, E.flux, E.fsig, lds);
 }
 

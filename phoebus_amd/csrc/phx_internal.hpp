@@ -36,6 +36,7 @@ inline BlockGeom make_block_geom(const phx_block *b) {
   g.nj = n[1];
   g.nk = n[2];
   g.ncell = (long)n[0] * n[1] * n[2];
+  g.geo = nullptr;
   return g;
 }
 
@@ -138,11 +139,11 @@ void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long 
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
                         hipStream_t s, bool do_c2p = true);
-// con2prim with the learned bracket over a pack's cells on an entire-cell work list
-// (phx_mlp.hip): mode 0 the entire blocks, 1 the face-adjacent ghost layer, 2 the interior,
-// 3 the interior and that layer; counters (nullable) accumulate [bracket misses, solves]
+// con2prim with the learned bracket over the cells of a pack work list of kind 4
+// (interior), 5 (entire blocks) or 6 (face-adjacent ghost layer) (phx_mlp.hip); counters
+// (nullable) accumulate [bracket misses, solves]
 void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
-                             long groups, int mode, unsigned long long *counters, hipStream_t s);
+                             long groups, int kind, unsigned long long *counters, hipStream_t s);
 // the fixups over a pack (phx_fixup.hip): con2prim failure repair on the interior cells,
 // then floors and ceilings (interior cells, or the entire blocks over an entire-cell work
 // list) folded with the interior CFL partial minima (partial may be null)
@@ -150,6 +151,8 @@ void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, co
                            long groups, hipStream_t s);
 void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
                         long groups, double *partial, hipStream_t s, bool entire = false);
+// fills a block's cached geometry plane (FMKS; kGeoPlanes x ni x nj doubles at geo)
+void launch_geo_fill(const DevParams &dp, const BlockGeom &bg, double *geo, hipStream_t s);
 // groups per block for each launch kind
 long pack_groups_flux(int d, int recon, const BlockGeom &bg, TileShape shape);
 long pack_groups_cells(const BlockGeom &bg, bool corners);

@@ -227,7 +227,9 @@ __device__ void ks_gcon4(double a, double X1, double X2, double gi[4][4]) {
   const double a2 = a * a;
   const double r = fabs(X1), cth = cos(X2), sth = sin(X2);
   const double sth2 = sth * sth, rho2 = r * r + a2 * cth * cth;
+#pragma unroll
   for (int x = 0; x < 4; ++x)
+#pragma unroll
     for (int y = 0; y < 4; ++y) gi[x][y] = 0.0;
   gi[0][1] = gi[1][0] = ratio(2 * r, rho2);
   gi[0][0] = -1 - gi[0][1];
@@ -275,15 +277,19 @@ __device__ void ks_derivs(double a, double X1, double X2, KSDerivs &o) {
 // (modified_system.hpp:96-112), in the reference's summation order.
 __device__ void fmks_derivs(const DevParams &p, double X1, double X2, double X3,
                             KSDerivs &o) {
+#pragma unroll
   for (int x = 0; x < 4; ++x)
+#pragma unroll
     for (int y = 0; y < 4; ++y) {
       o.gi[x][y] = 0.0;
+#pragma unroll
       for (int z = 0; z < 4; ++z) o.dg[x][y][z] = 0.0;
     }
   o.da[0] = o.da[3] = 0.0;
   const double X[3] = {X1, X2, X3};
   // the FMKS metric does not depend on X3 (FMKSMetric ignores it), so the X3 differences
   // are exactly zero: d < 3 gives the same numbers without evaluating those two points
+#pragma unroll
   for (int d = 1; d < 3; ++d) {
     double L[3] = {X1, X2, X3}, R[3] = {X1, X2, X3};
     const double dx = smax(fabs(X[d - 1]), 1.0) * p.fd_eps;
@@ -291,7 +297,9 @@ __device__ void fmks_derivs(const DevParams &p, double X1, double X2, double X3,
     R[d - 1] += dx;
     const FMKSMetric gr(p, R[0], R[1], R[2]);
     const FMKSMetric gl(p, L[0], L[1], L[2]);
+#pragma unroll
     for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
       for (int nu = 0; nu < 4; ++nu)
         o.dg[mu][nu][d] = ratio(gr.gcov(mu, nu) - gl.gcov(mu, nu), 2 * dx);
     o.da[d] = ratio(gr.alpha() - gl.alpha(), dx * (gl.alpha() + gr.alpha()));
@@ -305,10 +313,14 @@ __device__ void fmks_derivs(const DevParams &p, double X1, double X2, double X3,
   SJ[2][1] = -ratio(m.t1, m.r1 * m.t2);
   SJ[2][2] = ratio(1., m.t2);
   SJ[3][3] = 1.0;
+#pragma unroll
   for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
     for (int nu = 0; nu < 4; ++nu) {
       double acc = 0.0;
+#pragma unroll
       for (int lam = 0; lam < 4; ++lam)
+#pragma unroll
         for (int kap = 0; kap < 4; ++kap) acc += k.gi[lam][kap] * SJ[mu][lam] * SJ[nu][kap];
       o.gi[mu][nu] = acc;
     }
@@ -323,8 +335,76 @@ __device__ __forceinline__ void geom_derivs(const DevParams &p, double X1, doubl
     ks_derivs(p.a, X1, X2, o);
 }
 
+// the (mu, nu) pair e of a symmetric 4x4 in the cached plane's order (upper triangle, row
+// by row)
+__host__ __device__ constexpr int sym_a(int e) { return e < 4 ? 0 : e < 7 ? 1 : e < 9 ? 2 : 3; }
+__host__ __device__ constexpr int sym_b(int e) {
+  return e < 4 ? e : e < 7 ? e - 3 : e < 9 ? e - 5 : 3;
+}
+
+// geom_derivs at cell-plane point q of block b: from the cached plane when there is one
+// (CACHED: always)
+template <int GEOM, bool CACHED = false>
+__device__ __forceinline__ void geom_derivs_at(const DevParams &p, const BlockGeom &b, long q,
+                                               double X1, double X2, double X3, KSDerivs &o) {
+  if constexpr (GEOM == PHX_GEOM_FMKS) {
+    if (CACHED || b.geo) {
+      const long s = (long)b.ni * b.nj;
+      const double *t = b.geo + q;
+      for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y)
+          for (int z = 0; z < 4; ++z) o.dg[x][y][z] = 0.0;
+#pragma unroll
+      for (int d = 1; d < 3; ++d)
+#pragma unroll
+        for (int e = 0; e < 10; ++e) {
+          const double v = t[(kGeoMetric + 10 * (d - 1) + e) * s];
+          o.dg[sym_a(e)][sym_b(e)][d] = v;
+          o.dg[sym_b(e)][sym_a(e)][d] = v;
+        }
+      o.da[0] = o.da[3] = 0.0;
+      o.da[1] = t[(kGeoMetric + 20) * s];
+      o.da[2] = t[(kGeoMetric + 21) * s];
+#pragma unroll
+      for (int e = 0; e < 10; ++e) {
+        const double v = t[(kGeoMetric + 22 + e) * s];
+        o.gi[sym_a(e)][sym_b(e)] = v;
+        o.gi[sym_b(e)][sym_a(e)] = v;
+      }
+      return;
+    }
+  }
+  if constexpr (!(CACHED && GEOM == PHX_GEOM_FMKS)) geom_derivs<GEOM>(p, X1, X2, X3, o);
+}
+
+// fills the cached plane of block b (cells of the entire (i, j) range): the FMKS metric at
+// centres and x1 / x2 faces, and the centres' derivatives and inverse 4-metric
+__global__ __launch_bounds__(256) void k_geo_fill(DevParams p, BlockGeom b, double *geo) {
+  const long s = (long)b.ni * b.nj;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s) return;
+  const int i = (int)(q % b.ni), j = (int)(q / b.ni);
+  const double X1 = xc(b, 0, i), X2 = xc(b, 1, j), X3 = xc(b, 2, b.is[2]);
+  FMKSMetric(p, X1, X2, X3).store(geo + q, s);
+  FMKSMetric(p, xf(b, 0, i), X2, X3).store(geo + kGeoCentre * s + q, s);
+  FMKSMetric(p, X1, xf(b, 1, j), X3).store(geo + (kGeoCentre + kGeoMetric) * s + q, s);
+  KSDerivs o;
+  fmks_derivs(p, X1, X2, X3, o);
+  for (int d = 1; d < 3; ++d)
+    for (int e = 0; e < 10; ++e)
+      geo[(kGeoMetric + 10 * (d - 1) + e) * s + q] = o.dg[sym_a(e)][sym_b(e)][d];
+  geo[(kGeoMetric + 20) * s + q] = o.da[1];
+  geo[(kGeoMetric + 21) * s + q] = o.da[2];
+  for (int e = 0; e < 10; ++e) geo[(kGeoMetric + 22 + e) * s + q] = o.gi[sym_a(e)][sym_b(e)];
+}
+
+void launch_geo_fill(const DevParams &dp, const BlockGeom &bg, double *geo, hipStream_t s) {
+  const long n = (long)bg.ni * bg.nj;
+  hipLaunchKernelGGL(k_geo_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dp, bg, geo);
+}
+
 // geometric source terms of interior cell t of block b
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom &b, long t,
                                              const double *__restrict__ prim,
                                              double *__restrict__ src) {
@@ -338,9 +418,10 @@ __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
   const double X1 = xc(b, 0, i), X2 = xc(b, 1, j), X3 = xc(b, 2, k);
-  const M g(p, X1, X2, X3);
+  const long q = (long)j * b.ni + i;
+  const M g = metric_at<GEOM, CACHED>(p, b, 0, q, X1, X2, X3);
   KSDerivs dv;
-  geom_derivs<GEOM>(p, X1, X2, X3, dv);
+  geom_derivs_at<GEOM, CACHED>(p, b, q, X1, X2, X3, dv);
   double pr[L::nprim];
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
   // tmunu.hpp:94-126
@@ -372,31 +453,40 @@ __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom
   const double A = pr[0] + pr[4] + pr[L::prs] + bsq;
   const double B = pr[L::prs] + 0.5 * bsq;
   double T[4][4];
+#pragma unroll
   for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
     for (int nu = 0; nu < 4; ++nu) T[mu][nu] = A * u[mu] * u[nu] + B * dv.gi[mu][nu] - bb[mu] * bb[nu];
-  // connection Gamma_{abc} (geometry_utils.hpp:72-78) and the source terms (fluid.cpp:604-640)
-  const double gdet = g.detg();
-  for (int l = 0; l < 3; ++l) {
-    double s = 0.0;
+  // fluid.cpp:604-640 with T symmetric and the metric independent of X0 and X3 (both
+  // spherical Kerr-Schild and FMKS): only d_1 and d_2 of g are non-zero.  The momentum
+  // source T^{mn}(d_m g_{nl} - Gamma_{lnm}) equals T^{mn} d_l g_{mn} / 2 (the reference's
+  // form, restated in the oracle, differs by rounding), and the energy source's
+  // T^{mn} Gamma_{rmn} equals sum_{m, n} T^{mn} d_n g_{rm} - T^{mn} d_r g_{mn} / 2.
+  double P[3] = {0.0, 0.0, 0.0};  // P_c = T^{mn} d_c g_{mn}
+#pragma unroll
+  for (int cd = 1; cd < 3; ++cd)
+#pragma unroll
     for (int m = 0; m < 4; ++m)
-      for (int nn = 0; nn < 4; ++nn) {
-        const double gl = 0.5 * (dv.dg[nn][l + 1][m] + dv.dg[m][l + 1][nn] - dv.dg[nn][m][l + 1]);
-        s += T[m][nn] * (dv.dg[nn][l + 1][m] - gl);
-      }
-    src[(long)(1 + l) * n + c] = gdet * s;
-  }
+#pragma unroll
+      for (int nn = 0; nn < 4; ++nn) P[cd] += T[m][nn] * dv.dg[m][nn][cd];
+  const double gdet = g.detg();
+  src[1L * n + c] = gdet * (0.5 * P[1]);
+  src[2L * n + c] = gdet * (0.5 * P[2]);
+  src[3L * n + c] = 0.0;
   const double ia2 = ratio(1, alpha * alpha);
   const double gcon0[4] = {-ia2, ia2 * g.beta(0), ia2 * g.beta(1), ia2 * g.beta(2)};
   double TGam = 0.0;
-  for (int m = 0; m < 4; m++)
-    for (int nn = 0; nn < 4; nn++) {
-      double gam0 = 0;
-      for (int rr = 0; rr < 4; rr++)
-        gam0 += gcon0[rr] * (0.5 * (dv.dg[m][rr][nn] + dv.dg[nn][rr][m] - dv.dg[m][nn][rr]));
-      TGam += T[m][nn] * gam0;
-    }
-  double Ta = 0.0;
-  for (int m = 0; m < 4; m++) Ta += T[m][0] * dv.da[m];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double Q = 0.0;  // sum_{m, n} T^{mn} d_n g_{rm}
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int cd = 1; cd < 3; ++cd) Q += T[m][cd] * dv.dg[r][m][cd];
+    const double half = (r == 1 || r == 2) ? 0.5 * P[r] : 0.0;
+    TGam += gcon0[r] * (Q - half);
+  }
+  const double Ta = T[1][0] * dv.da[1] + T[2][0] * dv.da[2];
   src[4L * n + c] = gdet * alpha * (Ta - TGam);
 }
 
@@ -437,8 +527,11 @@ __global__ __launch_bounds__(128) void k_pack_sources(DevParams p, PackView v) {
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   // two 128-thread halves per 256-cell group
-  sources_body<true, GEOM>(p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.prim, E.src);
-  sources_body<true, GEOM>(p, E.bg, (g - v.start[b]) * 256 + 128 + threadIdx.x, E.prim, E.src);
+  sources_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, E.bg, (g - v.start[b]) * 256 + threadIdx.x,
+                                                 E.prim, E.src);
+  sources_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, E.bg,
+                                                 (g - v.start[b]) * 256 + 128 + threadIdx.x,
+                                                 E.prim, E.src);
 }
 
 // ---- batched con2prim / prim2con over independent states --------------------------------

@@ -131,7 +131,8 @@ __global__ __launch_bounds__(256) void k_mesh_convert(DevParams p, MeshInfo m, P
   const long n = bg.ncell, c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
   if constexpr (GEOM == PHX_GEOM_FMKS) {
     if (outer_x1 && m.inflow) {
-      const M gm(p, xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2]));
+      const M gm = metric_at<GEOM>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], xc(bg, 0, idx[0]),
+                                   xc(bg, 1, idx[1]), xc(bg, 2, idx[2]));
       double vcon[3] = {E.prim_out[n + c], E.prim_out[2 * n + c], E.prim_out[3 * n + c]};
       double W = lorentz_factor(gm, vcon);
       const double alpha = gm.alpha(), beta0 = gm.beta(0);
@@ -183,29 +184,19 @@ __global__ __launch_bounds__(256) void k_mesh_fix_fluxes(MeshInfo m, PackView v,
   }
 }
 
-// the robust con2prim over a pack's blocks on the stage's output container: mode 0 the
-// entire blocks, mode 1 the ghost cells face-adjacent to the interior (what the c2p fixup
-// reads)
+// the robust con2prim over a pack's blocks on the stage's output container, over the
+// cells of work-list kind 5 (entire blocks) or 6 (the face-adjacent ghost layer, what the
+// c2p fixup reads); a re-inversion of the entire blocks keeps the interior's iteration
+// counts (those of the stage's inversion of U_{s+1})
 template <int GEOM>
-__global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v, int mode) {
+__global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v, int kind) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
   int idx[3];
-  if (!entire_cell(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
-  if (mode == 1) {
-    int out = 0, layer = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (idx[d] < bg.is[d] || idx[d] > bg.ie[d]) ++out;
-      if (idx[d] == bg.is[d] - 1 || idx[d] == bg.ie[d] + 1) ++layer;
-    }
-    if (!(out == 1 && layer == 1)) return;
-  }
-  // the interior's iteration counts stay those of the stage's inversion of U_{s+1} (the
-  // update kernel's); the re-inversion after the exchange does not overwrite them
-  int32_t *it = (mode == 0 && is_interior(bg, idx)) ? nullptr : E.iters;
+  if (!cell_of_kind(kind, bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
+  int32_t *it = (kind == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
   con2prim_cell<true, GEOM>(p, bg, idx[0], idx[1], idx[2], E.u_out, E.prim_out, E.temp, E.mu,
                             E.csig, E.fail, it);
 }
@@ -289,22 +280,25 @@ void launch_exchange_and_boundaries(phx_mesh *m, int st, hipStream_t s) {
   ctx->mark_end(s);
 }
 
-// the robust con2prim of a FillDerived over the stage's output container: mode 0 the
-// entire blocks, 1 the face-adjacent ghost layer, 3 the interior and that layer (learned
-// bracket only: the plain interior inversion is fused into the update)
-void launch_c2p(phx_mesh *m, int st, int mode, hipStream_t s) {
+// the robust con2prim of a FillDerived over the stage's output container, on the cells of
+// work-list kind 4 (interior; learned bracket only: the plain one is fused into the
+// update), 5 (entire blocks) or 6 (the face-adjacent ghost layer)
+void launch_c2p(phx_mesh *m, int st, int kind, hipStream_t s) {
   phx_pack *k = m->pack;
+  static const char *kPlain[7] = {"", "", "", "", "con2prim", "con2prim", "con2prim_ghost"};
+  static const char *kLearned[7] = {"", "", "", "", "con2prim_learned_first", "con2prim_learned",
+                                    "con2prim_learned_ghost"};
   if (k->mlp) {
-    m->ctx->mark_begin(mode == 0 ? "con2prim_learned" : "con2prim_learned_first", s);
-    launch_pack_c2p_learned(m->p.geom.kind, k->dp, k->mlp, pv(m, st, 5), groups(m, 5), mode,
-                            k->counters, s);
+    m->ctx->mark_begin(kLearned[kind], s);
+    launch_pack_c2p_learned(m->p.geom.kind, k->dp, k->mlp, pv(m, st, kind), groups(m, kind),
+                            kind, k->counters, s);
     m->ctx->mark_end(s);
     return;
   }
-  m->ctx->mark_begin(mode == 0 ? "con2prim" : "con2prim_ghost", s);
+  m->ctx->mark_begin(kPlain[kind], s);
   with_geom(m->p.geom.kind, [&](auto G) {
-    hipLaunchKernelGGL((k_pack_con2prim<decltype(G)::value>), dim3((unsigned)groups(m, 5)),
-                       dim3(256), 0, s, k->dp, pv(m, st, 5), mode);
+    hipLaunchKernelGGL((k_pack_con2prim<decltype(G)::value>), dim3((unsigned)groups(m, kind)),
+                       dim3(256), 0, s, k->dp, pv(m, st, kind), kind);
   });
   m->ctx->mark_end(s);
 }
@@ -313,7 +307,7 @@ void launch_c2p(phx_mesh *m, int st, int mode, hipStream_t s) {
 // partial: the CFL partial minima (after the last stage) or null
 void launch_fill_derived_entire(phx_mesh *m, int st, double *partial, hipStream_t s) {
   phx_pack *k = m->pack;
-  launch_c2p(m, st, 0, s);
+  launch_c2p(m, st, 5, s);
   m->ctx->mark_begin("c2p_fixup", s);
   launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], s);
   m->ctx->mark_end(s);
@@ -339,7 +333,8 @@ int run_stage(phx_mesh *m, int st, bool last, hipStream_t s) {
   ctx->mark_end(s);
   // the first FillDerived: the interior (inverted by the update unless the learned bracket
   // is on) and the face-adjacent ghost layer the fixup reads
-  launch_c2p(m, st, learned ? 3 : 1, s);
+  if (learned) launch_c2p(m, st, 4, s);
+  launch_c2p(m, st, 6, s);
   ctx->mark_begin("c2p_fixup", s);
   launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], s);
   ctx->mark_end(s);
@@ -525,7 +520,7 @@ int phx_mesh_initialize(phx_mesh *m) {
   }
   // base is the output container of stage 2
   launch_exchange_and_boundaries(m, 2, s);
-  launch_c2p(m, 2, 0, s);
+  launch_c2p(m, 2, 5, s);
   // the "1" container starts as a copy of base
   const long nc = block_cells(m->blocks[0]);
   for (int b = 0; b < m->nblocks; ++b)

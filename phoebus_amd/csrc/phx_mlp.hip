@@ -568,14 +568,14 @@ __global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
   }
 }
 
-// con2prim with the learned bracket over the cells of a pack (work list over entire
-// blocks): the cell-field semantics of the plain pass (con2prim_cell: NaN input -> fail,
-// every other field kept; an unbracketed root -> fail with iters -1), the bracket and
-// fallback of k_con2prim_batch_learned.  mode: 0 entire blocks, 1 the face-adjacent
-// ghost layer, 2 the interior, 3 both.
+// con2prim with the learned bracket over the cells of a pack's work list of kind 4
+// (interior), 5 (entire blocks) or 6 (the face-adjacent ghost layer): the cell-field
+// semantics of the plain pass (con2prim_cell: NaN input -> fail, every other field kept;
+// an unbracketed root -> fail with iters -1), the bracket and fallback of
+// k_con2prim_batch_learned
 template <int GEOM>
 __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
-    DevParams p, MlpDev m, PackView v, int mode, unsigned long long *counters) {
+    DevParams p, MlpDev m, PackView v, int kind, unsigned long long *counters) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<true>;
   __shared__ uint64_t wfwd[2 * kFwdFrags * 64];
@@ -590,30 +590,17 @@ __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
-  int idx[3] = {0, 0, 0};
-  bool have = entire_cell(bg, (g - v.start[b]) * kLearnedThreads + threadIdx.x, idx);
-  bool inner = false;
-  if (have) {
-    int out = 0, layer = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (idx[d] < bg.is[d] || idx[d] > bg.ie[d]) ++out;
-      if (idx[d] == bg.is[d] - 1 || idx[d] == bg.ie[d] + 1) ++layer;
-    }
-    inner = out == 0;
-    const bool face_layer = out == 1 && layer == 1;
-    have = mode == 0 || (mode == 1 && face_layer) || (mode == 2 && inner) ||
-           (mode == 3 && (inner || face_layer));
-  }
+  int idx[3] = {bg.is[0], bg.is[1], bg.is[2]};
+  const bool have = cell_of_kind(kind, bg, (g - v.start[b]) * kLearnedThreads + threadIdx.x, idx);
   // a re-inversion of the entire blocks keeps the interior's iteration counts
-  int32_t *iters = (mode == 0 && inner) ? nullptr : E.iters;
+  int32_t *iters = (kind == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
   const long n = bg.ncell;
   const long c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
   // phase 1: features of this lane's cell
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   if (have) {
     const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
-    const M gm(p, X[0], X[1], X[2]);
+    const M gm = metric_at<GEOM>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
     double cs[L::ncons], pr[L::nprim];
 #pragma unroll
     for (int q = 0; q < L::ncons; ++q) cs[q] = E.u_out[q * n + c];
@@ -635,7 +622,7 @@ __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
     return;
   }
   const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
-  const M gm(p, X[0], X[1], X[2]);
+  const M gm = metric_at<GEOM>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
 #pragma unroll
   for (int q = 0; q < L::nprim; ++q) pr[q] = prim[q * n + c];
 #pragma unroll
@@ -681,7 +668,7 @@ __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
 }
 
 void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
-                             long groups, int mode, unsigned long long *counters, hipStream_t s);
+                             long groups, int kind, unsigned long long *counters, hipStream_t s);
 
 // features of batched states (training data and tests)
 template <bool MHD, int GEOM>
@@ -739,13 +726,13 @@ static MlpDev dev_view(const phx_mlp *m) {
 }
 
 void phx::launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m,
-                                  const PackView &v, long groups, int mode,
+                                  const PackView &v, long groups, int kind,
                                   unsigned long long *counters, hipStream_t s) {
   if (groups <= 0) return;
   const MlpDev md = dev_view(m);
   with_geom(geom, [&](auto G) {
     hipLaunchKernelGGL((k_pack_con2prim_learned<decltype(G)::value>), dim3((unsigned)groups),
-                       dim3(kLearnedThreads), 0, s, dp, md, v, mode, counters);
+                       dim3(kLearnedThreads), 0, s, dp, md, v, kind, counters);
   });
 }
 

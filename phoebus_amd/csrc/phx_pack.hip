@@ -32,9 +32,13 @@ int build_start(phx_pack *k, int kind) {
   std::vector<long> st(k->n + 1, 0);
   for (int b = 0; b < k->n; ++b) {
     const BlockGeom &g = k->geoms[b];
+    const long nf = 2L * ((long)(g.ie[1] - g.is[1] + 1) * (g.ie[2] - g.is[2] + 1) +
+                          (long)(g.ie[0] - g.is[0] + 1) * (g.ie[2] - g.is[2] + 1) +
+                          (long)(g.ie[0] - g.is[0] + 1) * (g.ie[1] - g.is[1] + 1));
     const long c = kind < 3    ? pack_groups_flux(kind, k->p.recon, g, k->tiles[kind])
                    : kind < 5 ? pack_groups_cells(g, kind == 3)
-                              : (g.ncell + 255) / 256;
+                   : kind == 5 ? (g.ncell + 255) / 256
+                               : (nf + 255) / 256;
     st[b + 1] = st[b] + c;
   }
   if (st[k->n] > (long)UINT_MAX) return PHX_ERR_INVALID;
@@ -87,7 +91,7 @@ void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipSt
   ctx->mark_end(s);
   if (learned) {
     ctx->mark_begin("con2prim_learned", s);
-    launch_pack_c2p_learned(k->p.geom.kind, k->dp, k->mlp, view(k, st, 5), k->groups[5], 2,
+    launch_pack_c2p_learned(k->p.geom.kind, k->dp, k->mlp, view(k, st, 4), k->groups[4], 4,
                             k->counters, s);
     ctx->mark_end(s);
   }
@@ -151,7 +155,19 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
       phx_pack_destroy(k);
       return ctx->fail(PHX_ERR_INVALID, "null field in a pack block");
     }
-    const BlockGeom g = make_block_geom(&B.block);
+    BlockGeom g = make_block_geom(&B.block);
+    if (p->geom.kind == PHX_GEOM_FMKS) {
+      // the FMKS metric does not depend on X3: one cached plane per block serves every k
+      double *geo = nullptr;
+      const size_t bytes = sizeof(double) * kGeoPlanes * (size_t)g.ni * g.nj;
+      if (hipMalloc(&geo, bytes) != hipSuccess) {
+        phx_pack_destroy(k);
+        return ctx->fail(PHX_ERR_NOMEM, "geometry cache allocation failed");
+      }
+      k->geo.push_back(geo);
+      launch_geo_fill(k->dp, g, geo, ctx->stream);
+      g.geo = geo;
+    }
     k->geoms.push_back(g);
     k->interior += (long)B.block.nx[0] * B.block.nx[1] * B.block.nx[2];
     for (int st = 1; st <= 2; ++st) {
@@ -180,11 +196,12 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
     ok = hipMalloc(&k->entries[st], sizeof(PackEntry) * nblocks) == hipSuccess &&
          hipMemcpy(k->entries[st], e[st].data(), sizeof(PackEntry) * nblocks,
                    hipMemcpyHostToDevice) == hipSuccess;
-  for (int kind = 0; kind < 6 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
+  for (int kind = 0; kind < 7 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
   ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max({k->groups[4], k->groups[5], 1L})) ==
                  hipSuccess;
   ok = ok && hipMalloc(&k->counters, 2 * sizeof(unsigned long long)) == hipSuccess &&
        hipMemset(k->counters, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
+  ok = ok && hipStreamSynchronize(ctx->stream) == hipSuccess;  // the geometry planes are filled
   if (!ok) {
     phx_pack_destroy(k);
     return ctx->fail(PHX_ERR_DEVICE, "pack set-up failed");
@@ -202,6 +219,7 @@ int phx_pack_destroy(phx_pack *k) {
     if (s) (void)hipFree(s);
   if (k->partial) (void)hipFree(k->partial);
   if (k->counters) (void)hipFree(k->counters);
+  for (double *g : k->geo) (void)hipFree(g);
   delete k;
   return PHX_OK;
 }

@@ -69,7 +69,7 @@ __device__ __forceinline__ double update_con2prim_body(
     const bool nan_in = isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0;
     if (!nan_in) {
       const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-      const M g(p, X[0], X[1], X[2]);
+      const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
 #pragma unroll
       for (int v = 0; v < L::nprim; ++v) pr[v] = prim_out[v * n + c];
       double tt = temp ? temp[c] : 0.0, m = mu[c];
