@@ -28,28 +28,33 @@ __device__ __forceinline__ long face_layer_cells(const BlockGeom &b) {
   return 2 * (n1 * n2 + n0 * n2 + n0 * n1);
 }
 __device__ __forceinline__ bool face_layer_cell(const BlockGeom &b, long t, int idx[3]) {
-  const int n[3] = {b.ie[0] - b.is[0] + 1, b.ie[1] - b.is[1] + 1, b.ie[2] - b.is[2] + 1};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int e0 = d == 0 ? 1 : 0, e1 = d == 2 ? 1 : 2;  // the other two dimensions
-    const long face = (long)n[e0] * n[e1];
-    if (t < 2 * face) {
-      const int side = t >= face;
-      const long r = t - side * face;
-      idx[d] = side ? b.ie[d] + 1 : b.is[d] - 1;
-      idx[e0] = (int)(r % n[e0]) + b.is[e0];
-      idx[e1] = (int)(r / n[e0]) + b.is[e1];
-      return true;
-    }
-    t -= 2 * face;
-  }
-  return false;
+  const int n0 = b.ie[0] - b.is[0] + 1, n1 = b.ie[1] - b.is[1] + 1, n2 = b.ie[2] - b.is[2] + 1;
+  // face d spans the other two dimensions (e0 fastest, e1); one select per index keeps
+  // idx in registers
+  const long f0 = (long)n1 * n2, f1 = (long)n0 * n2, f2 = (long)n0 * n1;
+  const int d = t < 2 * f0 ? 0 : t < 2 * (f0 + f1) ? 1 : 2;
+  t -= d == 0 ? 0 : d == 1 ? 2 * f0 : 2 * (f0 + f1);
+  const long face = d == 0 ? f0 : d == 1 ? f1 : f2;
+  const int ne0 = d == 0 ? n1 : n0;
+  const int side = t >= face;
+  const long r = t - side * face;
+  const int a0 = (int)(r % ne0), a1 = (int)(r / ne0);
+  const int lo = d == 0 ? b.is[0] : d == 1 ? b.is[1] : b.is[2];
+  const int hi = d == 0 ? b.ie[0] : d == 1 ? b.ie[1] : b.ie[2];
+  const int own = side ? hi + 1 : lo - 1;
+  if (t >= 2 * face) return false;  // beyond the layer: idx untouched
+  idx[0] = d == 0 ? own : a0 + b.is[0];
+  idx[1] = d == 1 ? own : (d == 0 ? a0 : a1) + b.is[1];
+  idx[2] = d == 2 ? own : a1 + b.is[2];
+  return true;
 }
 
 // work-list kinds of the pack cell kernels: 4 interior, 5 entire blocks, 6 face layer
-__device__ __forceinline__ bool cell_of_kind(int kind, const BlockGeom &b, long t, int idx[3]) {
-  if (kind == 5) return entire_cell(b, t, idx);
-  if (kind == 6) return face_layer_cell(b, t, idx);
+// (a compile-time kind: a runtime choice between the three keeps idx in scratch memory)
+template <int KIND>
+__device__ __forceinline__ bool cell_of_kind(const BlockGeom &b, long t, int idx[3]) {
+  if constexpr (KIND == 5) return entire_cell(b, t, idx);
+  if constexpr (KIND == 6) return face_layer_cell(b, t, idx);
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
   if (t >= (long)ni * nj * nk) return false;
   idx[0] = (int)(t % ni) + b.is[0];
@@ -77,7 +82,7 @@ __device__ __forceinline__ double eos_bmod_rho_sie(const DevParams &p, double rh
 
 // con2prim of cell c = (i, j, k) of block b, in place on cons / prim / temp / mu / csig;
 // writes the fail flag and the iteration count
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeom &b, int i,
                                               int j, int k, double *__restrict__ cons,
                                               double *__restrict__ prim,
@@ -86,11 +91,10 @@ __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeo
                                               double *__restrict__ csig,
                                               double *__restrict__ fail,
                                               int32_t *__restrict__ iters) {
-  using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
-  double cs[L::ncons], pr[L::nprim], sig[3];
+  double cs[L::ncons], sig[3];
 #pragma unroll
   for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
   // zones rejected up front by the NaN test keep every field (con2prim_robust.hpp:303-304)
@@ -100,27 +104,19 @@ __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeo
     return;
   }
   const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) sig[d] = d < b.ndim ? csig[d * n + c] : 0.0;
-  double tt = temp ? temp[c] : 0.0, m = mu[c];
   int it = 0;
-  const double st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
+  bool solved;
+  const double st = con2prim_fields<MHD, GEOM, CACHED>(p, b, (long)j * b.ni + i, X, cs, prim,
+                                                       temp, mu, cons, n, c, sig, it, solved);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) cons[v * n + c] = cs[v];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) prim[v * n + c] = pr[v];
-  for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
-  if (temp) temp[c] = tt;
-  mu[c] = m;
+  for (int d = 0; d < 3; ++d)
+    if (solved && d < b.ndim) csig[d * n + c] = sig[d];
   fail[c] = st;
   if (iters) iters[c] = it;
 }
 
 // p2c of cell (i, j, k): cons (with B) and, when csig is given, the cell signal speeds
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void prim_to_con_cell(const DevParams &p, const BlockGeom &b, int i,
                                                  int j, int k,
                                                  const double *__restrict__ prim,
@@ -130,7 +126,8 @@ __device__ __forceinline__ void prim_to_con_cell(const DevParams &p, const Block
   using L = Layout<MHD>;
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long n = b.ncell;
-  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
+  const M g =
+      metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
   double pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];

@@ -971,6 +971,7 @@ __device__ __forceinline__ double c2p_unwrap(const DevParams &p, const M &g, C2P
              cons[4], sig);
 #pragma unroll
     for (int i = 0; i < 3; ++i) cons[1 + i] = S[i];
+    if (isnan(cons[0]) + isnan(cons[1]) + isnan(cons[4]) > 0) return 0.0;
   } else {
 #pragma unroll
     for (int m = 0; m < 3; ++m) vel[m] *= iW;
@@ -989,7 +990,8 @@ __device__ __forceinline__ double c2p_unwrap(const DevParams &p, const M &g, C2P
     }
     signal_speeds(prim[0], prim[4], prim[L::prs], bsq2, vel, vsq, prim[L::gm1], g, sig);
   }
-  if (isnan(cons[0]) + isnan(cons[1]) + isnan(cons[4]) > 0) return 0.0;
+  // (the NaN test of the reference, :437-441, can only fire in the branch above: here the
+  // conserved variables are those c2p_setup found free of NaNs)
   if (p.fail_on_floors && (res.used_density_floor || res.used_energy_floor)) return 0.0;
   if (p.fail_on_ceilings && res.used_gamma_max) return 0.0;
   return 1.0;
@@ -1012,6 +1014,61 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   }
   mu_io = mu;
   return c2p_unwrap<MHD>(p, g, z, mu, cons, prim, temp, sig);
+}
+
+// con2prim_zone on a zone's fields in memory ([var][n] planes, cell c; the zone-centre
+// metric of geometry-plane point q at X), for conserved variables cs that passed the NaN
+// test: it writes the magnetic primitives of the set-up, then, when the root is bracketed,
+// the other primitives, the temperature and the root, and the conserved variables only
+// where a floor or ceiling re-derived them (elsewhere they are unchanged).  The input
+// primitives and temperature are never read (the ideal-gas inversion does not use them as a
+// guess).  With CACHED the metric is read from the block's cached geometry for the set-up
+// and again for the unwrap, so it is not held in registers across the solve.  solved tells
+// whether sig holds new signal speeds; iters is -1 for an unbracketed root.
+template <bool MHD, int GEOM, bool CACHED>
+__device__ __forceinline__ double con2prim_fields(const DevParams &p, const BlockGeom &b, long q,
+                                                  const double X[3], double *cs,
+                                                  double *__restrict__ prim,
+                                                  double *__restrict__ temp,
+                                                  double *__restrict__ mu,
+                                                  double *__restrict__ cons, long n, long c,
+                                                  double sig[3], int &iters, bool &solved) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  iters = 0;
+  C2PState z;
+  double pr[L::nprim];
+  const M g = metric_at<GEOM, CACHED>(p, b, 0, q, X[0], X[1], X[2]);
+  c2p_setup<MHD>(p, g, X, cs, pr, z);
+  if constexpr (MHD) {
+#pragma unroll
+    for (int i = 5; i < 8; ++i) prim[i * n + c] = pr[i];
+  }
+  double root;
+  solved = regula_falsi(z.res, 0.0, 1.0, p.tol, mu[c], p.max_iter, root, iters);
+  if (!solved) {
+    iters = -1;
+    return 0.0;
+  }
+  mu[c] = root;
+  double tt = 0.0, st;
+  if constexpr (CACHED) {
+    asm volatile("" ::: "memory");  // compiler-only: the metric is read again below
+    const M g2 = metric_at<GEOM, true>(p, b, 0, q, X[0], X[1], X[2]);
+    st = c2p_unwrap<MHD>(p, g2, z, root, cs, pr, tt, sig);
+  } else {
+    st = c2p_unwrap<MHD>(p, g, z, root, cs, pr, tt, sig);
+  }
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v)
+    if (!MHD || v < 5 || v > 7) prim[v * n + c] = pr[v];
+  if (temp) temp[c] = tt;
+  if (z.res.used_density_floor || z.res.used_energy_floor || z.res.used_energy_max ||
+      z.res.used_gamma_max) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) cons[v * n + c] = cs[v];
+  }
+  return st;
 }
 
 // ---- wavefront / block minimum (64-lane wavefronts) ----------------------------------

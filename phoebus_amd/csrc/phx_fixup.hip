@@ -17,7 +17,7 @@ namespace phx {
 // ConservedToPrimitiveFixupImpl (fixup_c2p.cpp:175-312), fluid part, for interior cell
 // (i, j, k).  A failed neighbour inside the interior is being rewritten by this same pass:
 // its weight (its fail flag) is 0 and it is skipped rather than multiplied.
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixup &fx,
                                                const BlockGeom &b, int i, int j, int k,
                                                double *prim, double *cons, double *temp,
@@ -72,7 +72,7 @@ __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixu
     for (int m = 0; m < 3; ++m) pr[1 + m] = 0.;
   }
   const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
+  const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
   const double gamma_max = p.bnd.gam_max;
   double vpcon[3] = {pr[1], pr[2], pr[3]};
   const double W = lorentz_factor(g, vpcon);
@@ -105,7 +105,7 @@ __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixu
 }
 
 // ApplyFloorsImpl (fixup.cpp:379-539), fluid part, for cell (i, j, k)
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &fx,
                                             const BlockGeom &b, int i, int j, int k,
                                             double *prim, double *cons, double *temp,
@@ -120,7 +120,7 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
   const double gamma_max = p.bnd.gam_max, e_max = p.bnd.sie_max;
   double rho_floor_max = rho_floor;
   double u_floor_max = rho_floor * sie_floor;
-  const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
+  const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
   double pr[L::nprim];
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_pack_c2p_fixup(DevParams p, DevFixup fx
   const PackEntry &E = v.e[b];
   int i, j, k;
   if (!interior_cell(E.bg, (g - v.start[b]) * 256 + threadIdx.x, i, j, k)) return;
-  c2p_fixup_cell<true, GEOM>(p, fx, E.bg, i, j, k, E.prim_out, E.u_out, E.temp, E.csig, E.fail);
+  c2p_fixup_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, E.bg, i, j, k, E.prim_out, E.u_out,
+                                                    E.temp, E.csig, E.fail);
 }
 
 // floors (on the interior cells, or on the entire blocks for a work list over them),
@@ -294,7 +295,8 @@ __global__ __launch_bounds__(256) void k_pack_floors(DevParams p, DevFixup fx, P
   }
   double ldt_min = DBL_MAX;
   if (cell && fx.floors)
-    floors_cell<true, GEOM>(p, fx, bg, i, j, k, E.prim_out, E.u_out, E.temp, E.mu, E.csig);
+    floors_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, bg, i, j, k, E.prim_out, E.u_out,
+                                                   E.temp, E.mu, E.csig);
   if (inner && partial) {
     const long n = bg.ncell, c = ((long)k * bg.nj + j) * bg.ni + i;
     const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};

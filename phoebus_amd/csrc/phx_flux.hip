@@ -69,7 +69,6 @@ __device__ __forceinline__ void flux_x_body(const DevParams &p, const BlockGeom 
                                             double *__restrict__ flux,
                                             double *__restrict__ fsig,
                                             double (*stage)[XB]) {
-  using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
   const int dj = b.ndim > 1, dk = b.ndim > 2;
@@ -155,7 +154,6 @@ __device__ __forceinline__ void flux_tile_body(const DevParams &p, const BlockGe
                                                const double *__restrict__ prim,
                                                double *__restrict__ flux,
                                                double *__restrict__ fsig, double *lds) {
-  using M = typename MetricOf<GEOM>::type;
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
   constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension

@@ -131,8 +131,8 @@ __global__ __launch_bounds__(256) void k_mesh_convert(DevParams p, MeshInfo m, P
   const long n = bg.ncell, c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
   if constexpr (GEOM == PHX_GEOM_FMKS) {
     if (outer_x1 && m.inflow) {
-      const M gm = metric_at<GEOM>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], xc(bg, 0, idx[0]),
-                                   xc(bg, 1, idx[1]), xc(bg, 2, idx[2]));
+      const M gm = metric_at<GEOM, true>(p, bg, 0, (long)idx[1] * bg.ni + idx[0],
+                                         xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2]));
       double vcon[3] = {E.prim_out[n + c], E.prim_out[2 * n + c], E.prim_out[3 * n + c]};
       double W = lorentz_factor(gm, vcon);
       const double alpha = gm.alpha(), beta0 = gm.beta(0);
@@ -153,7 +153,8 @@ __global__ __launch_bounds__(256) void k_mesh_convert(DevParams p, MeshInfo m, P
       }
     }
   }
-  prim_to_con_cell<true, GEOM>(p, bg, idx[0], idx[1], idx[2], E.prim_out, E.u_out, E.csig);
+  prim_to_con_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, bg, idx[0], idx[1], idx[2], E.prim_out,
+                                                      E.u_out, E.csig);
 }
 
 // FixFluxes for gr_outflow (fixup.cpp:759-1062): no mass inflow through a physical face,
@@ -188,17 +189,18 @@ __global__ __launch_bounds__(256) void k_mesh_fix_fluxes(MeshInfo m, PackView v,
 // cells of work-list kind 5 (entire blocks) or 6 (the face-adjacent ghost layer, what the
 // c2p fixup reads); a re-inversion of the entire blocks keeps the interior's iteration
 // counts (those of the stage's inversion of U_{s+1})
-template <int GEOM>
-__global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v, int kind) {
+template <int GEOM, int KIND>
+__global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
   int idx[3];
-  if (!cell_of_kind(kind, bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
-  int32_t *it = (kind == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
-  con2prim_cell<true, GEOM>(p, bg, idx[0], idx[1], idx[2], E.u_out, E.prim_out, E.temp, E.mu,
-                            E.csig, E.fail, it);
+  if (!cell_of_kind<KIND>(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
+  int32_t *it = (KIND == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
+  con2prim_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, bg, idx[0], idx[1], idx[2], E.u_out,
+                                                   E.prim_out, E.temp, E.mu, E.csig, E.fail,
+                                                   it);
 }
 
 }  // namespace phx
@@ -297,8 +299,10 @@ void launch_c2p(phx_mesh *m, int st, int kind, hipStream_t s) {
   }
   m->ctx->mark_begin(kPlain[kind], s);
   with_geom(m->p.geom.kind, [&](auto G) {
-    hipLaunchKernelGGL((k_pack_con2prim<decltype(G)::value>), dim3((unsigned)groups(m, kind)),
-                       dim3(256), 0, s, k->dp, pv(m, st, kind), kind);
+    constexpr int GM = decltype(G)::value;
+    auto kernel = kind == 5 ? k_pack_con2prim<GM, 5> : k_pack_con2prim<GM, 6>;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)groups(m, kind)), dim3(256), 0, s, k->dp,
+                       pv(m, st, kind));
   });
   m->ctx->mark_end(s);
 }

@@ -573,10 +573,11 @@ __global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
 // semantics of the plain pass (con2prim_cell: NaN input -> fail, every other field kept;
 // an unbracketed root -> fail with iters -1), the bracket and fallback of
 // k_con2prim_batch_learned
-template <int GEOM>
+template <int GEOM, int KIND>
 __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
-    DevParams p, MlpDev m, PackView v, int kind, unsigned long long *counters) {
+    DevParams p, MlpDev m, PackView v, unsigned long long *counters) {
   using M = typename MetricOf<GEOM>::type;
+  constexpr bool CACHED = GEOM == PHX_GEOM_FMKS;  // every FMKS block of a pack has its geometry
   using L = Layout<true>;
   __shared__ uint64_t wfwd[2 * kFwdFrags * 64];
   __shared__ float wbias[kNBias];
@@ -591,16 +592,16 @@ __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
   int idx[3] = {bg.is[0], bg.is[1], bg.is[2]};
-  const bool have = cell_of_kind(kind, bg, (g - v.start[b]) * kLearnedThreads + threadIdx.x, idx);
+  const bool have = cell_of_kind<KIND>(bg, (g - v.start[b]) * kLearnedThreads + threadIdx.x, idx);
   // a re-inversion of the entire blocks keeps the interior's iteration counts
-  int32_t *iters = (kind == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
+  int32_t *iters = (KIND == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
   const long n = bg.ncell;
   const long c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
   // phase 1: features of this lane's cell
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   if (have) {
     const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
-    const M gm = metric_at<GEOM>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
+    const M gm = metric_at<GEOM, CACHED>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
     double cs[L::ncons], pr[L::nprim];
 #pragma unroll
     for (int q = 0; q < L::ncons; ++q) cs[q] = E.u_out[q * n + c];
@@ -622,7 +623,7 @@ __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
     return;
   }
   const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
-  const M gm = metric_at<GEOM>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
+  const M gm = metric_at<GEOM, CACHED>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
 #pragma unroll
   for (int q = 0; q < L::nprim; ++q) pr[q] = prim[q * n + c];
 #pragma unroll
@@ -731,8 +732,12 @@ void phx::launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *
   if (groups <= 0) return;
   const MlpDev md = dev_view(m);
   with_geom(geom, [&](auto G) {
-    hipLaunchKernelGGL((k_pack_con2prim_learned<decltype(G)::value>), dim3((unsigned)groups),
-                       dim3(kLearnedThreads), 0, s, dp, md, v, kind, counters);
+    constexpr int GM = decltype(G)::value;
+    auto kernel = kind == 4   ? k_pack_con2prim_learned<GM, 4>
+                  : kind == 5 ? k_pack_con2prim_learned<GM, 5>
+                              : k_pack_con2prim_learned<GM, 6>;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)groups), dim3(kLearnedThreads), 0, s, dp, md, v,
+                       counters);
   });
 }
 

@@ -14,16 +14,14 @@ namespace phx {
 
 // interior cell t of block b: returns the cell's CFL time (DBL_MAX for t beyond the block
 // or when with_dt is false)
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, bool CACHED = false>
 __device__ __forceinline__ double update_con2prim_body(
     const DevParams &p, const BlockGeom &b, long t, const double *__restrict__ flux,
     const double *__restrict__ src, double *u_stage, const double *u_base, double beta,
     double bdt, double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
     double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt, bool do_c2p = true) {
-  using M = typename MetricOf<GEOM>::type;
-  using L = Layout<MHD>;
-  constexpr int NC = L::ncons;
+  constexpr int NC = Layout<MHD>::ncons;
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
   const bool active = t < (long)ni * nj * nk;
   double ldt_min = DBL_MAX;
@@ -57,39 +55,38 @@ __device__ __forceinline__ double update_con2prim_body(
       u_stage[(long)l * n + c] = us;
       cs[l] = us + bdt * dudt;
     }
-    if (!do_c2p) {  // the inversion runs in a separate (learned-bracket) pass
-#pragma unroll
-      for (int l = 0; l < NC; ++l) u_out[(long)l * n + c] = cs[l];
-      return DBL_MAX;
-    }
-    // ConservedToPrimitive on U_{s+1}
-    double pr[L::nprim], sig[3] = {0.0, 0.0, 0.0};
-    int it = 0;
-    double st = 0.0;
-    const bool nan_in = isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0;
-    if (!nan_in) {
-      const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-      const M g = metric_at<GEOM>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
-#pragma unroll
-      for (int v = 0; v < L::nprim; ++v) pr[v] = prim_out[v * n + c];
-      double tt = temp ? temp[c] : 0.0, m = mu[c];
-      st = con2prim_zone<MHD>(p, g, X, cs, pr, tt, m, sig, it);
-#pragma unroll
-      for (int v = 0; v < L::nprim; ++v) prim_out[v * n + c] = pr[v];
-      for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
-      if (temp) temp[c] = tt;
-      mu[c] = m;
-    } else {
-      for (int d = 0; d < b.ndim; ++d) sig[d] = csig[d * n + c];
-    }
+    // U_{s+1} goes out before the inversion, which re-derives it only where it used a floor
+    // or ceiling (con2prim_fields); no conserved variable is live across the solve
 #pragma unroll
     for (int l = 0; l < NC; ++l) u_out[(long)l * n + c] = cs[l];
+    if (!do_c2p) return DBL_MAX;  // the inversion runs in a separate (learned-bracket) pass
+    // ConservedToPrimitive on U_{s+1}
+    double sig[3] = {0.0, 0.0, 0.0};
+    int it = 0;
+    double st = 0.0;
+    bool solved = false;
+    if (!(isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0)) {
+      const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+      st = con2prim_fields<MHD, GEOM, CACHED>(p, b, (long)j * b.ni + i, X, cs, prim_out, temp, mu,
+                                              u_out, n, c, sig, it, solved);
+    }
+    // a zone the inversion left (NaN input, unbracketed root) keeps its signal speeds
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d >= b.ndim) break;
+      if (solved)
+        csig[d * n + c] = sig[d];
+      else if (with_dt)
+        sig[d] = csig[d * n + c];
+    }
     fail[c] = st;
     if (iters) iters[c] = it;
     // EstimateTimestepBlock with face speeds (fluid.cpp:913-928)
     if (with_dt) {
       double ldt = 0.0;
-      for (int d = 0; d < b.ndim; ++d) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (d >= b.ndim) break;
         const long off = d == 0 ? 1 : d == 1 ? sj : sk;
         const double s = smax(sig[d], smax(fsig[d * n + c], fsig[d * n + c + off]));
         ldt += s / b.dx[d];
@@ -117,16 +114,19 @@ __global__ __launch_bounds__(256) void k_update_con2prim(
 }
 
 // packs: the fused update over a work list of 3-D MHD blocks; dt from dt_dev when given
-// (a device-resident time step, so a captured graph needs no host value)
+// (a device-resident time step, so a captured graph needs no host value).  Every FMKS
+// block of a pack carries its cached geometry; the FMKS variant fits four waves per SIMD
+// (128 VGPRs) without spilling.
 template <int GEOM>
-__global__ __launch_bounds__(256) void k_pack_update(DevParams p, PackView v, double beta,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_FMKS ? 4 : 1))) void k_pack_update(DevParams p, PackView v, double beta,
                                                      double dt, const double *dt_dev,
                                                      double *__restrict__ partial, int do_c2p) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   const double bdt = beta * (dt_dev ? dt_dev[0] : dt);
-  const double val = update_con2prim_body<true, GEOM>(
+  const double val = update_con2prim_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(
       p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.src, E.u_stage, E.u_base, beta,
       bdt, E.u_out, E.prim_out, E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
       partial != nullptr, do_c2p != 0);
