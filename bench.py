@@ -554,6 +554,18 @@ def c5_torus_mesh(phx, args, log):
     wall_all = wall_a + wall_b
     log(f"[c5] plain: {1e3 * wall_all / (K - 1):.2f} ms/step, steps {L + 1}-{K} "
         f"{1e3 * wall_b / (K - L):.2f} ms/step, t = {t_plain:.4g}")
+    out = {"workload": "FMKS torus 256x128x128, 32 blocks of 64x64x32, WENO5Z + HLL, "
+                       "fixups + floors (torus deck), gr_outflow x1/x2 + periodic x3",
+           "steps": K, "interior_cells": cells,
+           "plain": {"ms_per_step": 1e3 * wall_all / (K - 1),
+                     "zone_updates_per_s": cells * 2 * (K - 1) / wall_all,
+                     "ms_per_step_after_switch": 1e3 * wall_b / (K - L),
+                     "time": t_plain, "dt": dt_plain, "ncycle": n_plain,
+                     "launch_ms_per_stage": launch_plain, "con2prim_last_stage": it_plain},
+           "conserved_totals_initial": tot0.tolist(),
+           "conserved_drift_plain": (tot_plain - tot0).tolist()}
+    if args.no_c5_learned:
+        return out
     # ---- learned: log pairs over the first L steps, train, switch ------------------------
     m = make()
     logged = []
@@ -584,14 +596,7 @@ def c5_torus_mesh(phx, args, log):
     log(f"[c5] learned: steps {L + 2}-{K} {1e3 * wall_l / (K - L - 1):.2f} ms/step, "
         f"bracket misses {misses}/{solves}")
     rel = lambda a, b: ((a - b) / np.maximum(np.abs(b), 1e-300)).tolist()
-    return {"workload": "FMKS torus 256x128x128, 32 blocks of 64x64x32, WENO5Z + HLL, "
-                        "fixups + floors (torus deck), gr_outflow x1/x2 + periodic x3",
-            "steps": K, "interior_cells": cells,
-            "plain": {"ms_per_step": 1e3 * wall_all / (K - 1),
-                      "zone_updates_per_s": cells * 2 * (K - 1) / wall_all,
-                      "ms_per_step_after_switch": 1e3 * wall_b / (K - L),
-                      "time": t_plain, "dt": dt_plain, "ncycle": n_plain,
-                      "launch_ms_per_stage": launch_plain, "con2prim_last_stage": it_plain},
+    out.update({
             "learned": {"pairs": int(data["cons"].shape[1]), "log_steps": L,
                         "train_steps": args.c5_train_steps, "train_s": train_s,
                         "loss_first_last": [losses[0], losses[-1]],
@@ -601,10 +606,9 @@ def c5_torus_mesh(phx, args, log):
                         "miss_rate": misses / max(solves, 1),
                         "time": t_l, "ncycle": n_l, "launch_ms_per_stage": launch_learned,
                         "con2prim_last_stage": it_l},
-            "conserved_totals_initial": tot0.tolist(),
-            "conserved_drift_plain": (tot_plain - tot0).tolist(),
             "conserved_drift_learned": (tot_l - tot0).tolist(),
-            "learned_vs_plain_totals_rel": rel(tot_l, tot_plain)}
+            "learned_vs_plain_totals_rel": rel(tot_l, tot_plain)})
+    return out
 
 
 def main():
@@ -630,6 +634,7 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the full torus mesh leg")
     ap.add_argument("--c5-steps", type=int, default=200)
     ap.add_argument("--c5-log-steps", type=int, default=50)
+    ap.add_argument("--no-c5-learned", action="store_true", help="skip C5's learned-bracket run")
     ap.add_argument("--c5-pairs", type=int, default=1 << 22)
     ap.add_argument("--c5-train-steps", type=int, default=6000)
     args = ap.parse_args()
