@@ -225,12 +225,16 @@ int phx_prim_to_con_batch(phx_ctx *ctx, const phx_fluid_params *p, int64_t n,
                           phx_stream stream);
 
 /* ---- tile-shape autotuning (no reference counterpart: Kokkos picks team sizes) -----
- * phx_autotune times every admissible y/z flux tile shape (ti cells along i by ts along
- * the sweep) on the caller's fields -- overwriting f->flux and f->face_sig -- and keeps
- * the fastest in the context for this (recon, riemann, mhd, geometry, block size);
- * phx_calculate_fluxes and phx_fluid_stage then use it.  Results do not depend on the
+ * phx_autotune times every admissible y/z flux sweep shape on the caller's fields --
+ * overwriting f->flux and f->face_sig -- and keeps the fastest in the context for this
+ * (recon, riemann, mhd, geometry, block size); phx_calculate_fluxes and phx_fluid_stage
+ * then use it.  A shape (ti, ts) with ts > 0 is a tile of ti cells along i by ts along
+ * the sweep; with ts < 0, one 64-lane wave of ti columns (a power of two) by 64 / ti rows
+ * marching along the sweep in -ts segments of the faces.  Results do not depend on the
  * shape.  reps = timed launches per candidate.  phx_tile_shape reports the shape in use
- * for dir 1 (y) or 2 (z); phx_set_tile_shape fixes one (0 = default). */
+ * for dir 1 (y) or 2 (z); phx_set_tile_shape fixes one (ts = 0: the default, marching
+ * waves of 8 x 8 in segments of at least 24 faces where that makes >= 1024 waves, else
+ * tiles of 8 x up to 64). */
 int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
                  const phx_block_fields *f, int reps, phx_stream stream);
 int phx_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, int dir,

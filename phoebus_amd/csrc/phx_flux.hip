@@ -8,8 +8,10 @@
 //    once per variable and the cell's upper-edge state is published in LDS, so lane l forms
 //    face i from the upper edge of cell i-1 (lane l-1) and the lower edge of cell i (its
 //    own).  Blocks overlap by 2H+1 cells.
-//  * y, z (strided): tiles spanning the whole own-direction extent for a few consecutive i;
-//    rows are read into LDS and each lane reconstructs along its column from LDS.
+//  * y, z (strided): one 64-lane wave of a few columns marches along the sweep with an LDS
+//    ring of primitive rows, passing upper edges to the next row by cross-lane shuffles;
+//    or LDS tiles of a few consecutive i by a stretch of the own direction, overlapping by
+//    2H+1 rows.  The autotuner picks the kind and shape per block size.
 //
 // Results are bit-identical to reconstructing per face: the same arithmetic runs on the
 // same operands.
@@ -129,9 +131,10 @@ __global__ __launch_bounds__(XB) void k_flux_x(DevParams p, BlockGeom b,
 // own direction from LDS and publishes the upper edge for the lane one row up.  Lanes in
 // own-rows H+1 .. ts_n-H-1 emit faces; tiles along the own direction overlap by 2H+1
 // cells.  The shape trades redundant halo work (short ts_n) against partial cache lines
-// (narrow ti_n: a row segment of ti_n doubles uses ti_n/16 of a 128-byte line); it is
-// chosen per block size and reconstruction by the autotuner (phx_autotune), else by the
-// default below.  Blocks are renumbered XCD by XCD so neighbouring i chunks share an L2.
+// (narrow ti_n: a row segment of ti_n doubles uses ti_n/16 of a 128-byte line).  Tiles
+// run when the autotuner (phx_autotune) finds one faster than the marching waves below
+// (the default) or when a caller fixes one.  Blocks are renumbered XCD by XCD so
+// neighbouring i chunks share an L2.
 constexpr int kTileMaxThreads = 512;
 constexpr int kTileMaxRows = 256;
 constexpr int kTileMaxLdsBytes = 64 * 1024;  // two blocks per CU
@@ -217,6 +220,136 @@ __device__ __forceinline__ void flux_tile_body(const DevParams &p, const BlockGe
                                        (long)(DIR == 1 ? o : t) * b.ni + i, flux, fsig);
 }
 
+// ---- y, z marching: one wave per workgroup --------------------------------------------
+// A wave holds ti columns (consecutive in the flattened face range of (i, transverse
+// index), i fastest) by MR = 64 / ti rows along the sweep, and marches over one of nseg
+// segments of its columns' faces, MR cells per step.  The primitives of the rows a step
+// reconstructs (its MR cells and H on either side) sit in an LDS ring of MR + 2H rows, so a
+// step reads only its MR new rows from memory.  Row r takes the upper edge of row r - 1 by
+// a cross-lane shuffle and row 0 the previous step's last row from LDS: no cell is
+// reconstructed twice except the one before each segment (the tiles above redo 2H + 1 of
+// every ts rows).
+constexpr int kMarchLanes = 64;
+constexpr int kMarchMaxLdsBytes = 16 * 1024;
+
+// LDS bytes of a marching workgroup: the ring [NP][MR + 2H][ti] and the carry row [NP][ti]
+__host__ __device__ inline long march_lds_bytes(int np, int h, int ti) {
+  return 8L * np * ti * (kMarchLanes / ti + 2 * h + 1);
+}
+
+// number of marching workgroups of the DIR sweep of block b
+__host__ __device__ inline long flux_march_groups(const BlockGeom &b, int dir, int ti,
+                                                  int nseg) {
+  const int T = dir == 1 ? 2 : 1;
+  const long ncol = (long)(b.ie[0] - b.is[0] + 3) * (b.ie[T] - b.is[T] + 1 + 2 * (b.ndim > T));
+  return (ncol + ti - 1) / ti * nseg;
+}
+
+// the DIR sweep for logical workgroup bx of block b; lds as in march_lds_bytes
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM, bool CACHED = false>
+__device__ __forceinline__ void flux_march_body(const DevParams &p, const BlockGeom &b, long bx,
+                                                int ti, int nseg,
+                                                const double *__restrict__ prim,
+                                                double *__restrict__ flux,
+                                                double *__restrict__ fsig, double *lds) {
+  constexpr int NP = Layout<MHD>::nprim;
+  constexpr int H = ReconWidth<RECON>::H;
+  constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
+  const int MR = kMarchLanes / ti, Q = MR + 2 * H;
+  const int lane = threadIdx.x, cl = lane % ti, r = lane / ti;
+  const int dt_ = b.ndim > T;
+  const int nci = b.ie[0] - b.is[0] + 3;  // i over [is-1, ie+1]
+  const long ncol = (long)nci * (b.ie[T] - b.is[T] + 1 + 2 * dt_);
+  const long nchunk = (ncol + ti - 1) / ti;
+  const int sg = (int)(bx / nchunk);
+  long col = (bx % nchunk) * ti + cl;
+  const bool colv = col < ncol;
+  if (!colv) col = ncol - 1;
+  const int i = b.is[0] - 1 + (int)(col % nci);
+  const int t = b.is[T] - dt_ + (int)(col / nci);
+  const int nown = DIR == 1 ? b.nj : b.nk;
+  // own-direction faces [s-1, e+1] in nseg segments of L; a segment's march starts one
+  // cell early for the upper edge its first face needs
+  const int nf = b.ie[DIR] - b.is[DIR] + 3;
+  const int L = (nf + nseg - 1) / nseg;
+  const int flo = b.is[DIR] - 1 + sg * L;
+  const int fhi = b.is[DIR] - 1 + min((sg + 1) * L, nf) - 1;
+  if (flo > fhi) return;  // an empty trailing segment (the whole workgroup)
+  const int c0 = flo - 1;
+  const int nsteps = (fhi - c0 + MR) / MR;
+  const long sown = DIR == 1 ? (long)b.ni : (long)b.ni * b.nj;
+  const long base = DIR == 1 ? (long)t * b.nj * b.ni + i : (long)t * b.ni + i;
+  double *ring = lds;                // [NP][Q][ti]: row y in slot (y - c0 + H) % Q
+  double *carry = lds + NP * Q * ti; // [NP][ti]: the upper edges of the last row
+  auto load_row = [&](int y) {
+    const long c = base + (long)min(max(y, 0), nown - 1) * sown;
+    const int slot = (y - c0 + H) % Q;
+#pragma unroll
+    for (int v = 0; v < NP; ++v) ring[(v * Q + slot) * ti + cl] = prim[v * b.ncell + c];
+  };
+  const int below = lane >= ti ? lane - ti : lane;
+  for (int m = 0; m < nsteps; ++m) {
+    __syncthreads();  // the previous step's ring reads are done
+    if (m == 0) {
+      for (int y = r; y < Q; y += MR) load_row(c0 - H + y);
+    } else {
+      load_row(c0 + m * MR + H + r);
+    }
+    __syncthreads();
+    const int cc = c0 + m * MR + r;  // this lane's cell
+    // opaque copies: what the face needs is recomputed from them each step instead of
+    // being hoisted out of the loop and held across the Riemann solver
+    int ii = i, tt = t;
+    asm volatile("" : "+v"(ii), "+v"(tt));
+    int sl[2 * H + 1];
+#pragma unroll
+    for (int o = 0; o < 2 * H + 1; ++o) sl[o] = (m * MR + r + o) % Q;
+    double lo[NP], up[NP];
+#pragma unroll
+    for (int v = 0; v < NP; ++v) {
+      double q[2 * H + 1];
+#pragma unroll
+      for (int o = 0; o < 2 * H + 1; ++o) q[o] = ring[(v * Q + sl[o]) * ti + cl];
+      reconstruct_edges<RECON>(q, up[v], lo[v]);
+      pin(up[v]);
+      pin(lo[v]);
+    }
+    // the upper edge of the cell below: row r - 1's, or the previous step's last row
+    double ql[NP];
+#pragma unroll
+    for (int v = 0; v < NP; ++v) {
+      const double sh = __shfl(up[v], below, kMarchLanes);
+      ql[v] = r == 0 ? carry[v * ti + cl] : sh;
+    }
+    if (r == MR - 1) {
+#pragma unroll
+      for (int v = 0; v < NP; ++v) carry[v * ti + cl] = up[v];
+    }
+    if (colv && cc >= flo && cc <= fhi) {
+      double X[3];
+      X[0] = xc(b, 0, ii);
+      X[T] = xc(b, T, tt);
+      X[DIR] = xf(b, DIR, cc);
+      const long bb = DIR == 1 ? (long)tt * b.nj * b.ni + ii : (long)tt * b.ni + ii;
+      face_out<MHD, RIEMANN, GEOM, CACHED>(p, DIR, b, X, ql, lo, bb + (long)cc * sown,
+                                           (long)(DIR == 1 ? cc : tt) * b.ni + ii, flux, fsig);
+    }
+  }
+}
+
+// Minkowski fits four waves per SIMD (128 VGPRs; one 8-byte value spills)
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
+__global__ __launch_bounds__(kMarchLanes)
+__attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 4 : 1))) void k_flux_march(DevParams p, BlockGeom b, int ti,
+                                                            int nseg,
+                                                            const double *__restrict__ prim,
+                                                            double *__restrict__ flux,
+                                                            double *__restrict__ fsig) {
+  extern __shared__ double lds[];
+  flux_march_body<DIR, RECON, RIEMANN, MHD, GEOM>(p, b, xcd_remap(blockIdx.x, gridDim.x), ti,
+                                                  nseg, prim, flux, fsig, lds);
+}
+
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
 __global__ __launch_bounds__(512) void k_flux_tile(DevParams p, BlockGeom b, int ti_n,
                                                     int ts_n, const double *__restrict__ prim,
@@ -249,16 +382,46 @@ __global__ __launch_bounds__(512) void k_pack_flux_tile(DevParams p, PackView v,
                                                   E.prim, E.flux, E.fsig, lds);
 }
 
+template <int DIR, int RECON, int RIEMANN, int GEOM>
+__global__ __launch_bounds__(kMarchLanes) void k_pack_flux_march(DevParams p, PackView v, int ti,
+                                                                 int nseg) {
+  extern __shared__ double lds[];
+  const long g = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = pack_find(v.start, v.n, g);
+  const PackEntry &E = v.e[b];
+  flux_march_body<DIR, RECON, RIEMANN, true, GEOM, GEOM == PHX_GEOM_FMKS>(
+      p, E.bg, g - v.start[b], ti, nseg, E.prim, E.flux, E.fsig, lds);
+}
+
 // ---- dispatch --------------------------------------------------------------------------
 TileShape resolve_tile_shape(int d, int recon, int mhd, const BlockGeom &bg, TileShape want) {
   const int H = recon == PHX_RECON_CONSTANT ? 0 : recon == PHX_RECON_LINEAR ? 1 : 2;
   const int NP = mhd ? 10 : 7;
+  if (want.ts == 0) {
+    // the default: marching waves of 8 columns x 8 rows in segments of at least 24 faces
+    // when that fills the chip with at least 1024 waves; a small block's march would be a
+    // few waves stepping serially, so it keeps 8-wide tiles of up to 64 rows
+    const int nf = bg.ie[d] - bg.is[d] + 3;
+    const int nseg = nf / 24 < 1 ? 1 : nf / 24 > 8 ? 8 : nf / 24;
+    const TileShape m =
+        resolve_tile_shape(d, recon, mhd, bg, TileShape{want.ti > 0 ? want.ti : 8, -nseg});
+    if (flux_march_groups(bg, d, m.ti, -m.ts) >= 1024) return m;
+    want.ts = 64;
+  }
+  if (want.ts < 0) {  // marching: ti a power of two up to 64, -ts segments of the faces
+    const int want_ti = want.ti > 0 ? want.ti : 8;
+    int ti = 1;
+    while (ti * 2 <= want_ti && ti < kMarchLanes) ti *= 2;
+    while (ti > 1 && march_lds_bytes(NP, H, ti) > kMarchMaxLdsBytes) ti /= 2;
+    const int nf = bg.ie[d] - bg.is[d] + 3;
+    return TileShape{ti, -(-want.ts < nf ? -want.ts : nf)};
+  }
   // own-direction cells the face range needs: s-2-H .. e+1+H
   const int need = bg.ie[d] - bg.is[d] + 4 + 2 * H;
   const int cap = need <= kTileMaxRows ? need : kTileMaxRows;
   TileShape t = want;
-  if (t.ts <= 0) t.ts = cap < 64 ? cap : 64;
   if (t.ti <= 0) t.ti = 8;
+  if (t.ts == 0) t.ts = 64;
   if (t.ts > cap) t.ts = cap;
   if (t.ts < 2 * H + 2) t.ts = 2 * H + 2 <= cap ? 2 * H + 2 : cap;
   if (t.ti < 1) t.ti = 1;
@@ -289,6 +452,17 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const do
   const int T = d == 1 ? 2 : 1;
   const int dt_ = bg.ndim > T;
   const TileShape ts = resolve_tile_shape(d, RECON, MHD, bg, shape);
+  if (ts.ts < 0) {
+    const unsigned blocks = (unsigned)flux_march_groups(bg, d, ts.ti, -ts.ts);
+    const size_t lds = (size_t)march_lds_bytes(NP, H, ts.ti);
+    if (d == 1)
+      hipLaunchKernelGGL((k_flux_march<1, RECON, RIEMANN, MHD, GEOM>), dim3(blocks),
+                         dim3(kMarchLanes), lds, s, dp, bg, ts.ti, -ts.ts, prim, flux, fsig);
+    else
+      hipLaunchKernelGGL((k_flux_march<2, RECON, RIEMANN, MHD, GEOM>), dim3(blocks),
+                         dim3(kMarchLanes), lds, s, dp, bg, ts.ti, -ts.ts, prim, flux, fsig);
+    return;
+  }
   const int ts_n = ts.ts, ti_n = ts.ti;
   const int S = ts_n - 2 * H - 1;
   const long nic = (bg.ie[0] - bg.is[0] + 3 + ti_n - 1) / ti_n;
@@ -360,6 +534,7 @@ namespace phx {
 long pack_groups_flux(int d, int recon, const BlockGeom &bg, TileShape shape) {
   const int H = recon == PHX_RECON_CONSTANT ? 0 : recon == PHX_RECON_LINEAR ? 1 : 2;
   if (d == 0) return flux_x_groups(bg, H);
+  if (shape.ts < 0) return flux_march_groups(bg, d, shape.ti, -shape.ts);
   return flux_tile_groups(bg, d, shape.ti, shape.ts, H);
 }
 
@@ -370,6 +545,17 @@ static void pack_dir(int d, const DevParams &dp, const PackView &v, long groups,
   if (d == 0) {
     hipLaunchKernelGGL((k_pack_flux_x<RECON, RIEMANN, GEOM>), dim3((unsigned)groups), dim3(XB), 0,
                        s, dp, v);
+    return;
+  }
+  if (sh.ts < 0) {
+    const size_t lds =
+        (size_t)march_lds_bytes(Layout<true>::nprim, ReconWidth<RECON>::H, sh.ti);
+    if (d == 1)
+      hipLaunchKernelGGL((k_pack_flux_march<1, RECON, RIEMANN, GEOM>), dim3((unsigned)groups),
+                         dim3(kMarchLanes), lds, s, dp, v, sh.ti, -sh.ts);
+    else
+      hipLaunchKernelGGL((k_pack_flux_march<2, RECON, RIEMANN, GEOM>), dim3((unsigned)groups),
+                         dim3(kMarchLanes), lds, s, dp, v, sh.ti, -sh.ts);
     return;
   }
   const int threads = ((sh.ti * sh.ts + 63) / 64) * 64;

@@ -17,12 +17,13 @@ using namespace phx;
 
 namespace phx {
 
-// tile shapes must fit every block of the pack: resolve against the smallest extent
+// tile shapes must fit every block of the pack: resolve against the smallest extent (a
+// marching shape, ts < 0, keeps the fewest segments any block allows)
 TileShape pack_shape(const phx_pack *k, int d, TileShape want) {
   TileShape best = resolve_tile_shape(d, k->p.recon, 1, k->geoms[0], want);
   for (const auto &g : k->geoms) {
     const TileShape t = resolve_tile_shape(d, k->p.recon, 1, g, want);
-    if (t.ts < best.ts) best.ts = t.ts;
+    if (want.ts < 0 ? t.ts > best.ts : t.ts < best.ts) best.ts = t.ts;
     if (t.ti < best.ti) best.ti = t.ti;
   }
   return best;
@@ -278,7 +279,8 @@ int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return ctx->fail(PHX_ERR_DEVICE, "event create failed");
   static const int kTi[] = {1, 2, 4, 8, 16};
-  static const int kTs[] = {16, 24, 32, 48, 64, 96, 128};
+  // tiles of ti x ts, and marching waves of ti columns in -ts segments
+  static const int kTs[] = {16, 24, 32, 48, 64, 96, 128, -1, -2, -3, -4, -6, -8};
   int rc = PHX_OK;
   for (int d = 1; d < 3 && rc == PHX_OK; ++d) {
     float best = 1e30f;

@@ -1,9 +1,12 @@
 // phx_tune.hip -- autotuner of the y/z flux tile shape (include/phx.h, phx_autotune).
 //
-// The y/z flux kernels stage ti x ts cells of every primitive in LDS.  Which shape is
-// fastest depends on the block extent (halo overhead 2H+1 of ts rows per tile), the
-// reconstruction order (H, register pressure), the variable count (LDS per block) and on
-// cache-line use along i.  The tuner times every admissible shape on the caller's own
+// The y/z flux kernels either stage ti x ts cells of every primitive in LDS (tiles,
+// ts > 0) or march one wave of ti columns by 64 / ti rows along the sweep in -ts segments
+// (ts < 0).  Which shape is fastest depends on the block extent (halo overhead 2H+1 of ts
+// rows per tile; one cell per segment and the last step's unused rows when marching), the
+// reconstruction order (H, register pressure), the variable count (LDS per block), the
+// number of waves in flight and cache-line use along i.  The tuner times every admissible
+// shape of both kinds on the caller's own
 // fields with HIP events and keeps the fastest per (recon, riemann, mhd, geometry, block
 // size, direction) in the context; later flux and stage calls with the same key use it.
 // Results are bit-identical for every shape, so tuning changes time only.
@@ -53,6 +56,15 @@ int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
       for (const auto &c : cand) dup = dup || (c.ti == r.ti && c.ts == r.ts);
       if (!dup) cand.push_back(r);
     }
+    // marching waves: ti columns by 64 / ti rows, in nseg segments of the faces
+    static const int kSeg[] = {1, 2, 3, 4, 6, 8};
+    for (int ti : {1, 2, 4, 8, 16})
+      for (int nseg : kSeg) {
+        const TileShape r = resolve_tile_shape(d, p->recon, p->mhd, bg, TileShape{ti, -nseg});
+        bool dup = false;
+        for (const auto &c : cand) dup = dup || (c.ti == r.ti && c.ts == r.ts);
+        if (!dup) cand.push_back(r);
+      }
     float best = 1e30f;
     TileShape best_shape;
     for (const auto &c : cand) {
@@ -95,7 +107,7 @@ int phx_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, 
 
 int phx_set_tile_shape(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b, int dir,
                        int ti, int ts) {
-  if (!ctx || !p || !b || dir < 1 || dir > 2 || ti < 0 || ts < 0) return PHX_ERR_INVALID;
+  if (!ctx || !p || !b || dir < 1 || dir > 2 || ti < 0) return PHX_ERR_INVALID;
   ctx->tiles[phx_ctx::tile_key(p, b, dir)] = TileShape{ti, ts};
   return PHX_OK;
 }

@@ -432,8 +432,8 @@ def test_fused_stage_matches_oracle_sequence(phx, case):
 
 
 def test_tile_shapes_and_autotune_do_not_change_fluxes(phx):
-    # every admissible y/z tile shape (and the tuned one) gives the same bits
-    say("[gpu] flux tile shapes + autotune vs oracle")
+    # every admissible y/z tile and marching shape (and the tuned one) gives the same bits
+    say("[gpu] flux tile and marching shapes + autotune vs oracle")
     rng = np.random.default_rng(77)
     b = block(3, (24, 20, 18), 4, dx=(0.1, 0.2, 0.3))
     p = fluid_params(gamma=13.0 / 9.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL)
@@ -444,7 +444,10 @@ def test_tile_shapes_and_autotune_do_not_change_fluxes(phx):
     f = st.fields()
     ref_flux, ref_fsig = O.calculate_fluxes(op, ob, prim)
     first = None
-    shapes = [(1, 6), (2, 16), (3, 24), (8, 64), (16, 32), (32, 16), (5, 29), (0, 0)]
+    shapes = [(1, 6), (2, 16), (3, 24), (8, 64), (16, 32), (32, 16), (5, 29), (0, 0),
+              # marching waves: ti columns x 64/ti rows in -ts segments (ti rounded down to
+              # a power of two and capped by LDS, segments capped by the face count)
+              (1, -1), (2, -3), (4, -2), (8, -1), (16, -4), (64, -2), (3, -5), (4, -100)]
     for ti, ts in shapes:
         for d in (1, 2):
             phx.set_tile_shape(p, b, d, ti, ts)
