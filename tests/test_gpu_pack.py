@@ -30,9 +30,9 @@ def phx():
 SIZES = [(8, 8, 8), (12, 10, 8), (16, 8, 12), (8, 12, 10), (10, 10, 10)]
 
 
-def _blocks(geom):
+def _blocks(geom, sizes=SIZES):
     out = []
-    for n, nx in enumerate(SIZES):
+    for n, nx in enumerate(sizes):
         if geom == A.GEOM_FMKS:  # X = (log r, x2, phi), blocks side by side in x2
             b = block(3, nx, 4, xmin=(1.5, 0.2 + 0.12 * n, 0.0), dx=(0.04, 0.01, 0.05))
         else:
@@ -95,16 +95,27 @@ def _equal(a, b, what):
             assert np.array_equal(u, v, equal_nan=True), f"{what}: block {n} field {k}"
 
 
-@pytest.mark.parametrize("geom", [A.GEOM_MINKOWSKI, A.GEOM_FMKS])
-def test_pack_matches_single_block_stages(phx, geom):
-    say(f"[gpu] pack of {len(SIZES)} blocks vs single-block stages, geom {geom}")
+# a 64^3 block makes the default y/z sweeps the marching waves (>= 1024 workgroups), for
+# the pack (every block) and for that block's single-block stage
+MARCH_SIZES = [(64, 64, 64), (8, 8, 8), (12, 10, 8)]
+
+
+@pytest.mark.parametrize("geom,sizes", [(A.GEOM_MINKOWSKI, SIZES), (A.GEOM_FMKS, SIZES),
+                                        (A.GEOM_MINKOWSKI, MARCH_SIZES),
+                                        (A.GEOM_FMKS, MARCH_SIZES)],
+                         ids=["mink", "fmks", "mink-march", "fmks-march"])
+def test_pack_matches_single_block_stages(phx, geom, sizes):
+    say(f"[gpu] pack of {len(sizes)} blocks {sizes[0]}... vs single-block stages, geom {geom}")
     rng = np.random.default_rng(31 + geom)
     p = _params(geom)
-    blocks = _blocks(geom)
+    blocks = _blocks(geom, sizes)
     S = Setup(phx, p, blocks, rng, geom)
     pk = Pack(phx, p, [(b, st, c1, it) for b, st, c1, it in
                        zip(blocks, S.states, S.cons1, S.iters)])
-    assert pk.cells() == (len(SIZES), sum(int(np.prod(s)) for s in SIZES))
+    assert pk.cells() == (len(sizes), sum(int(np.prod(s)) for s in sizes))
+    if sizes is MARCH_SIZES:
+        assert all(pk.tile_shape(d)[1] < 0 for d in (1, 2)), "expected marching sweeps"
+        assert all(phx.tile_shape(p, blocks[0], d)[1] < 0 for d in (1, 2))
     dt = 1e-3
     d_dt = DeviceArray((1,))
     # pack: stage 1 then stage 2
