@@ -104,26 +104,15 @@ __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixu
   for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
 }
 
-// ApplyFloorsImpl (fixup.cpp:379-539), fluid part, for cell (i, j, k)
-template <bool MHD, int GEOM, bool CACHED = false>
-__device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &fx,
-                                            const BlockGeom &b, int i, int j, int k,
-                                            double *prim, double *cons, double *temp,
-                                            double *mu, double *csig) {
-  using M = typename MetricOf<GEOM>::type;
-  using L = Layout<MHD>;
-  const long n = b.ncell;
-  const long c = ((long)k * b.nj + j) * b.ni + i;
-  const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
-  double rho_floor, sie_floor;
-  get_floors(p.bnd, X[0], X[1], X[2], rho_floor, sie_floor);
-  const double gamma_max = p.bnd.gam_max, e_max = p.bnd.sie_max;
-  double rho_floor_max = rho_floor;
-  double u_floor_max = rho_floor * sie_floor;
-  const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
-  double pr[L::nprim];
-#pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+// the floors ApplyFloorsImpl targets for a cell (fixup.cpp:397-428): rho and u floors,
+// raised by the MHD b^2 ceilings when they are on
+template <bool MHD, class M>
+__device__ __forceinline__ void floor_targets(const DevParams &p, const DevFixup &fx, const M &g,
+                                              const double *pr, double rho_floor,
+                                              double sie_floor, double &rho_floor_max,
+                                              double &u_floor_max) {
+  rho_floor_max = rho_floor;
+  u_floor_max = rho_floor * sie_floor;
   if constexpr (MHD) {
     if (fx.mhd_floors) {  // :411-428
       double Bsq = 0.0, Bdotv = 0.0;
@@ -143,9 +132,54 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
       const double bsq = (Bsq + g.alpha() * g.alpha() * bcon0 * bcon0) * iW * iW;
       rho_floor_max = smax(rho_floor_max, bsq / fx.bsqorho_max);
       u_floor_max = smax(u_floor_max, bsq / fx.bsqou_max);
-      rho_floor_max = smax(rho_floor_max, smax(pr[4], u_floor_max) / e_max);
+      rho_floor_max = smax(rho_floor_max, smax(pr[4], u_floor_max) / p.bnd.sie_max);
     }
   }
+}
+
+// whether floors_cell changes cell (i, j, k): a floor (:434) or a ceiling (:501-507) acts.
+// It reads only the primitives floors_cell decides on and holds none of its repair state.
+template <bool MHD, int GEOM, bool CACHED = false>
+__device__ __forceinline__ bool floors_needed(const DevParams &p, const DevFixup &fx,
+                                              const BlockGeom &b, int i, int j, int k,
+                                              const double *prim) {
+  using M = typename MetricOf<GEOM>::type;
+  const long n = b.ncell;
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+  double rho_floor, sie_floor;
+  get_floors(p.bnd, X[0], X[1], X[2], rho_floor, sie_floor);
+  const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
+  double pr[8];
+#pragma unroll
+  for (int v = 0; v < (MHD ? 8 : 5); ++v) pr[v] = prim[v * n + c];
+  double rho_floor_max, u_floor_max;
+  floor_targets<MHD>(p, fx, g, pr, rho_floor, sie_floor, rho_floor_max, u_floor_max);
+  if (rho_floor_max - pr[0] > 0. || u_floor_max - pr[4] > 0.) return true;
+  double vpcon[3] = {pr[1], pr[2], pr[3]};
+  return lorentz_factor(g, vpcon) > p.bnd.gam_max || pr[4] / pr[0] > p.bnd.sie_max;
+}
+
+// ApplyFloorsImpl (fixup.cpp:379-539), fluid part, for cell (i, j, k)
+template <bool MHD, int GEOM, bool CACHED = false>
+__device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &fx,
+                                            const BlockGeom &b, int i, int j, int k,
+                                            double *prim, double *cons, double *temp,
+                                            double *mu, double *csig) {
+  using M = typename MetricOf<GEOM>::type;
+  using L = Layout<MHD>;
+  const long n = b.ncell;
+  const long c = ((long)k * b.nj + j) * b.ni + i;
+  const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
+  double rho_floor, sie_floor;
+  get_floors(p.bnd, X[0], X[1], X[2], rho_floor, sie_floor);
+  const double gamma_max = p.bnd.gam_max, e_max = p.bnd.sie_max;
+  const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
+  double pr[L::nprim];
+#pragma unroll
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  double rho_floor_max, u_floor_max;
+  floor_targets<MHD>(p, fx, g, pr, rho_floor, sie_floor, rho_floor_max, u_floor_max);
   double drho = rho_floor_max - pr[0];
   double du = u_floor_max - pr[4];
   bool floor_applied = false;
@@ -270,12 +304,29 @@ __global__ __launch_bounds__(256) void k_pack_c2p_fixup(DevParams p, DevFixup fx
                                                     E.temp, E.csig, E.fail);
 }
 
-// floors (on the interior cells, or on the entire blocks for a work list over them),
-// then the interior cell's CFL time with face speeds (fluid.cpp:913-928) into the
-// workgroup's partial minimum: the time step sees the signal speeds after the fixups
+// the CFL time of interior cell (i, j, k) with face speeds (fluid.cpp:913-928)
+__device__ __forceinline__ double cell_dt(const BlockGeom &bg, const PackEntry &E, int i, int j,
+                                          int k) {
+  const long n = bg.ncell, c = ((long)k * bg.nj + j) * bg.ni + i;
+  const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
+  double ldt = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d >= bg.ndim) break;
+    const double s = smax(E.csig[d * n + c], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
+    ldt += s / bg.dx[d];
+  }
+  return 1.0 / (ldt + 1.e-50);
+}
+
+// the floors' check pass (on the interior cells, or on the entire blocks for a work list
+// over them): a cell a floor or ceiling acts on goes to the queue, every other interior
+// cell's CFL time into the workgroup's partial minimum (so the time step sees the signal
+// speeds after the fixups); q.cell null: no floors, the partials only
 template <int GEOM>
 __global__ __launch_bounds__(256) void k_pack_floors(DevParams p, DevFixup fx, PackView v,
-                                                     int entire, double *__restrict__ partial) {
+                                                     int entire, double *__restrict__ partial,
+                                                     FloorQueue q) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
@@ -293,25 +344,55 @@ __global__ __launch_bounds__(256) void k_pack_floors(DevParams p, DevFixup fx, P
   } else {
     cell = inner = interior_cell(bg, t, i, j, k);
   }
-  double ldt_min = DBL_MAX;
-  if (cell && fx.floors)
-    floors_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, bg, i, j, k, E.prim_out, E.u_out,
-                                                   E.temp, E.mu, E.csig);
-  if (inner && partial) {
-    const long n = bg.ncell, c = ((long)k * bg.nj + j) * bg.ni + i;
-    const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
-    double ldt = 0.0;
-    for (int d = 0; d < bg.ndim; ++d) {
-      const double s =
-          smax(E.csig[d * n + c], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
-      ldt += s / bg.dx[d];
+  const bool defer = cell && q.cell &&
+                     floors_needed<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, bg, i, j, k,
+                                                                        E.prim_out);
+  // one queue slot range per wave
+  const unsigned long long mask = __ballot(defer);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(q.count, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+    if (defer && slot < q.cap) {
+      q.cell[slot] = ((unsigned long long)b << 32) | (unsigned long long)(((long)k * bg.nj + j) * bg.ni + i);
+      q.grp[slot] = (inner && partial) ? (int)g : -1;
     }
-    ldt_min = 1.0 / (ldt + 1.e-50);
   }
   if (partial) {
-    const double m = block_min(ldt_min);
+    const double m = block_min(inner && !defer ? cell_dt(bg, E, i, j, k) : DBL_MAX);
     if (threadIdx.x == 0) partial[g] = m;
   }
+}
+
+// the floors' repair pass over the queued cells (a grid-stride loop over the count the
+// check pass left); a repaired interior cell's CFL time joins its partial by an atomic
+// minimum on the bits (positive doubles order as their bit patterns), so the partials equal
+// those of one pass over every cell
+template <int GEOM>
+__global__ __launch_bounds__(256) void k_pack_floors_apply(DevParams p, DevFixup fx, PackView v,
+                                                           double *partial, FloorQueue q) {
+  const long n = (long)min((unsigned long long)*q.count, (unsigned long long)q.cap);
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const unsigned long long w = q.cell[e];
+    const PackEntry &E = v.e[(int)(w >> 32)];
+    const BlockGeom &bg = E.bg;
+    const long c = (long)(w & 0xffffffffull);
+    const int i = (int)(c % bg.ni), j = (int)((c / bg.ni) % bg.nj);
+    const int k = (int)(c / ((long)bg.ni * bg.nj));
+    floors_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, bg, i, j, k, E.prim_out, E.u_out,
+                                                   E.temp, E.mu, E.csig);
+    const int gg = q.grp[e];
+    if (partial && gg >= 0)
+      atomicMin(reinterpret_cast<unsigned long long *>(partial + gg),
+                (unsigned long long)__double_as_longlong(cell_dt(bg, E, i, j, k)));
+  }
+}
+
+__global__ void k_zero_count(unsigned int *count) {
+  if (threadIdx.x == 0) *count = 0u;
 }
 
 void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
@@ -324,11 +405,19 @@ void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, co
 }
 
 void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
-                        long groups, double *partial, hipStream_t s, bool entire) {
+                        long groups, double *partial, hipStream_t s, bool entire,
+                        const FloorQueue &q) {
   if (groups <= 0 || (!fx.floors && !partial)) return;
+  const FloorQueue qq = fx.floors && q.cell && q.cap >= groups * 256 ? q : FloorQueue{};
+  if (fx.floors && !qq.cell) return;  // the owner sizes the queue: never reached
+  if (qq.cell) hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, qq.count);
   with_geom(geom, [&](auto G) {
     hipLaunchKernelGGL((k_pack_floors<decltype(G)::value>), dim3((unsigned)groups), dim3(256),
-                       0, s, dp, fx, v, entire ? 1 : 0, partial);
+                       0, s, dp, fx, v, entire ? 1 : 0, partial, qq);
+    if (qq.cell)
+      hipLaunchKernelGGL((k_pack_floors_apply<decltype(G)::value>),
+                         dim3((unsigned)(groups < 1024 ? groups : 1024)), dim3(256), 0, s, dp, fx, v,
+                         partial, qq);
   });
 }
 

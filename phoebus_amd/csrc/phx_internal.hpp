@@ -149,8 +149,19 @@ void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, co
 // list) folded with the interior CFL partial minima (partial may be null)
 void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
                            long groups, hipStream_t s);
+// The floors run as a check pass over every cell and a repair pass over the cells it
+// queues (the rare ones a floor or ceiling acts on), so the check pass is not sized for the
+// repair's con2prim.  The queue holds (block << 32 | cell) and the check pass's workgroup
+// (whose CFL partial the repaired cell joins; -1 for none); cap >= groups * 256.
+struct FloorQueue {
+  unsigned long long *cell = nullptr;
+  int *grp = nullptr;
+  unsigned int *count = nullptr;
+  long cap = 0;
+};
 void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
-                        long groups, double *partial, hipStream_t s, bool entire = false);
+                        long groups, double *partial, hipStream_t s, bool entire,
+                        const FloorQueue &q);
 // fills a block's cached geometry plane (FMKS; kGeoPlanes x ni x nj doubles at geo)
 void launch_geo_fill(const DevParams &dp, const BlockGeom &bg, double *geo, hipStream_t s);
 // groups per block for each launch kind

@@ -316,7 +316,8 @@ void launch_fill_derived_entire(phx_mesh *m, int st, double *partial, hipStream_
   launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], s);
   m->ctx->mark_end(s);
   m->ctx->mark_begin("floors", s);
-  launch_pack_floors(k->p.geom.kind, k->dp, k->fx, pv(m, st, 5), k->groups[5], partial, s, true);
+  launch_pack_floors(k->p.geom.kind, k->dp, k->fx, pv(m, st, 5), k->groups[5], partial, s, true,
+                     k->fq);
   m->ctx->mark_end(s);
 }
 
@@ -343,7 +344,8 @@ int run_stage(phx_mesh *m, int st, bool last, hipStream_t s) {
   launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], s);
   ctx->mark_end(s);
   ctx->mark_begin("floors", s);
-  launch_pack_floors(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], nullptr, s);
+  launch_pack_floors(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], nullptr, s,
+                     false, k->fq);
   ctx->mark_end(s);
   launch_exchange_and_boundaries(m, st, s);
   launch_fill_derived_entire(m, st, last ? k->partial : nullptr, s);

@@ -103,7 +103,8 @@ void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipSt
       ctx->mark_end(s);
     }
     ctx->mark_begin("floors", s);
-    launch_pack_floors(k->p.geom.kind, k->dp, k->fx, view(k, st, 4), k->groups[4], partial, s);
+    launch_pack_floors(k->p.geom.kind, k->dp, k->fx, view(k, st, 4), k->groups[4], partial, s,
+                       false, k->fq);
     ctx->mark_end(s);
   }
   if (a->dt_out) {
@@ -220,6 +221,9 @@ int phx_pack_destroy(phx_pack *k) {
     if (s) (void)hipFree(s);
   if (k->partial) (void)hipFree(k->partial);
   if (k->counters) (void)hipFree(k->counters);
+  if (k->fq.cell) (void)hipFree(k->fq.cell);
+  if (k->fq.grp) (void)hipFree(k->fq.grp);
+  if (k->fq.count) (void)hipFree(k->fq.count);
   for (double *g : k->geo) (void)hipFree(g);
   delete k;
   return PHX_OK;
@@ -240,7 +244,24 @@ int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *k, const phx_fixup_params *fx) {
   }
   const int rc = check_fixup(ctx, fx);
   if (rc) return rc;
-  k->fx = make_dev_fixup(&k->p, fx);
+  const DevFixup d = make_dev_fixup(&k->p, fx);
+  if (d.floors && !k->fq.cell) {
+    // room for every cell of the largest floors launch (interior or entire work list)
+    const long cap = 256L * std::max(k->groups[4], k->groups[5]);
+    FloorQueue q;
+    q.cap = cap;
+    const bool ok = hipMalloc(&q.cell, sizeof(unsigned long long) * cap) == hipSuccess &&
+                    hipMalloc(&q.grp, sizeof(int) * cap) == hipSuccess &&
+                    hipMalloc(&q.count, sizeof(unsigned int)) == hipSuccess;
+    if (!ok) {
+      if (q.cell) (void)hipFree(q.cell);
+      if (q.grp) (void)hipFree(q.grp);
+      if (q.count) (void)hipFree(q.count);
+      return ctx->fail(PHX_ERR_NOMEM, "floors queue allocation failed");
+    }
+    k->fq = q;
+  }
+  k->fx = d;
   return PHX_OK;
 }
 

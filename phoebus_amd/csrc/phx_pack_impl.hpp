@@ -33,6 +33,7 @@ struct phx_pack {
   const phx_mlp *mlp = nullptr;          // learned con2prim bracket (null: plain solver)
   std::vector<double *> geo;             // per block: cached geometry plane (FMKS) or empty
   unsigned long long *counters = nullptr;  // device [2]: bracket misses, learned solves
+  phx::FloorQueue fq;  // the floors' repair queue, allocated when floors are set on
 };
 
 struct phx_graph {
