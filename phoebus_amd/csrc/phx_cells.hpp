@@ -49,12 +49,51 @@ __device__ __forceinline__ bool face_layer_cell(const BlockGeom &b, long t, int 
   return true;
 }
 
-// work-list kinds of the pack cell kernels: 4 interior, 5 entire blocks, 6 face layer
+// ghost cell t of a block (every cell outside the interior box): the ghost planes of x3,
+// then for interior k the ghost rows of x2, then for interior (j, k) the ghost cells of x1
+__device__ __forceinline__ long ghost_cells(const BlockGeom &b) {
+  return b.ncell - (long)(b.ie[0] - b.is[0] + 1) * (b.ie[1] - b.is[1] + 1) * (b.ie[2] - b.is[2] + 1);
+}
+__device__ __forceinline__ bool ghost_cell(const BlockGeom &b, long t, int idx[3]) {
+  const int n0 = b.ie[0] - b.is[0] + 1, n1 = b.ie[1] - b.is[1] + 1, n2 = b.ie[2] - b.is[2] + 1;
+  const int g0 = b.ni - n0, g1 = b.nj - n1, g2 = b.nk - n2;  // ghost counts per dimension
+  const long A = (long)g2 * b.nj * b.ni, Bc = (long)n2 * g1 * b.ni, Cc = (long)n2 * n1 * g0;
+  int i, j, k;
+  if (t < A) {
+    const long pl = (long)b.nj * b.ni;
+    const int q = (int)(t / pl);
+    k = q < b.is[2] ? q : b.ie[2] + 1 + (q - b.is[2]);
+    j = (int)((t % pl) / b.ni);
+    i = (int)(t % b.ni);
+  } else if (t < A + Bc) {
+    const long u = t - A, row = (long)g1 * b.ni;
+    k = b.is[2] + (int)(u / row);
+    const int q = (int)((u % row) / b.ni);
+    j = q < b.is[1] ? q : b.ie[1] + 1 + (q - b.is[1]);
+    i = (int)(u % b.ni);
+  } else if (t < A + Bc + Cc) {
+    const long u = t - A - Bc, pl = (long)n1 * g0;
+    k = b.is[2] + (int)(u / pl);
+    j = b.is[1] + (int)((u % pl) / g0);
+    const int q = (int)(u % g0);
+    i = q < b.is[0] ? q : b.ie[0] + 1 + (q - b.is[0]);
+  } else {
+    return false;
+  }
+  idx[0] = i;
+  idx[1] = j;
+  idx[2] = k;
+  return true;
+}
+
+// work-list kinds of the pack cell kernels: 4 interior, 5 entire blocks, 6 face layer,
+// 7 ghost cells
 // (a compile-time kind: a runtime choice between the three keeps idx in scratch memory)
 template <int KIND>
 __device__ __forceinline__ bool cell_of_kind(const BlockGeom &b, long t, int idx[3]) {
   if constexpr (KIND == 5) return entire_cell(b, t, idx);
   if constexpr (KIND == 6) return face_layer_cell(b, t, idx);
+  if constexpr (KIND == 7) return ghost_cell(b, t, idx);
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
   if (t >= (long)ni * nj * nk) return false;
   idx[0] = (int)(t % ni) + b.is[0];

@@ -57,8 +57,7 @@ __global__ __launch_bounds__(256) void k_mesh_exchange(MeshInfo m, PackView v) {
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   int idx[3];
-  if (!entire_cell(E.bg, (g - v.start[b]) * 256 + threadIdx.x, idx) || is_interior(E.bg, idx))
-    return;
+  if (!ghost_cell(E.bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int lx[3];
   block_loc(m, b, lx);
   int src[3], sl[3];
@@ -86,11 +85,13 @@ __global__ __launch_bounds__(256) void k_mesh_outflow(MeshInfo m, PackView v, in
                                                       int outer) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
-  const PackEntry &E = v.e[b];
-  int idx[3];
-  if (!entire_cell(E.bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int lx[3];
   block_loc(m, b, lx);
+  // a block on neither physical face of d has nothing to do (the whole workgroup)
+  if (!(inner && lx[d] == 0) && !(outer && lx[d] == m.nb[d] - 1)) return;
+  const PackEntry &E = v.e[b];
+  int idx[3];
+  if (!ghost_cell(E.bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int ref[3] = {idx[0], idx[1], idx[2]};
   if (idx[d] < E.bg.is[d] && inner && lx[d] == 0)
     ref[d] = E.bg.is[d];
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void k_mesh_convert(DevParams p, MeshInfo m, P
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
   int idx[3];
-  if (!entire_cell(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
+  if (!ghost_cell(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int lx[3];
   block_loc(m, b, lx);
   bool slab = false, outer_x1 = false;
@@ -158,7 +159,9 @@ __global__ __launch_bounds__(256) void k_mesh_convert(DevParams p, MeshInfo m, P
 }
 
 // FixFluxes for gr_outflow (fixup.cpp:759-1062): no mass inflow through a physical face,
-// F(D) <= 0 on inner faces and >= 0 on outer ones, over the face's interior extent
+// F(D) <= 0 on inner faces and >= 0 on outer ones, over the face's interior extent; over
+// the face-adjacent ghost layer, whose cell (is-1, j, k) stands for the inner x1 face
+// stored at (is, j, k) and (ie+1, j, k) for the outer one (likewise in x2, x3)
 __global__ __launch_bounds__(256) void k_mesh_fix_fluxes(MeshInfo m, PackView v,
                                                          int outflow_mask) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
@@ -166,21 +169,17 @@ __global__ __launch_bounds__(256) void k_mesh_fix_fluxes(MeshInfo m, PackView v,
   const PackEntry &E = v.e[b];
   const BlockGeom &bg = E.bg;
   int idx[3];
-  if (!entire_cell(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
+  if (!face_layer_cell(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int lx[3];
   block_loc(m, b, lx);
-  const long n = bg.ncell, c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
+  const long n = bg.ncell, sd[3] = {1, bg.ni, (long)bg.ni * bg.nj};
+  const long c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    bool inside = true;  // the other two indices inside the interior
-#pragma unroll
-    for (int e = 0; e < 3; ++e)
-      if (e != d) inside = inside && idx[e] >= bg.is[e] && idx[e] <= bg.ie[e];
-    if (!inside) continue;
     double *F = E.flux + (long)d * kNcons * n;  // the density flux of direction d
-    if ((outflow_mask >> (2 * d) & 1) && lx[d] == 0 && idx[d] == bg.is[d])
-      F[c] = smin(F[c], 0.0);
-    if ((outflow_mask >> (2 * d + 1) & 1) && lx[d] == m.nb[d] - 1 && idx[d] == bg.ie[d] + 1)
+    if (idx[d] == bg.is[d] - 1 && (outflow_mask >> (2 * d) & 1) && lx[d] == 0)
+      F[c + sd[d]] = smin(F[c + sd[d]], 0.0);
+    if (idx[d] == bg.ie[d] + 1 && (outflow_mask >> (2 * d + 1) & 1) && lx[d] == m.nb[d] - 1)
       F[c] = smax(F[c], 0.0);
   }
 }
@@ -261,8 +260,9 @@ long groups(const phx_mesh *m, int kind) { return m->pack->groups[kind]; }
 
 void launch_exchange_and_boundaries(phx_mesh *m, int st, hipStream_t s) {
   phx_ctx *ctx = m->ctx;
-  const PackView v = pv(m, st, 5);
-  const unsigned gr = (unsigned)groups(m, 5);
+  // every kernel here writes ghost cells only: a work list of them
+  const PackView v = pv(m, st, 7);
+  const unsigned gr = (unsigned)groups(m, 7);
   ctx->mark_begin("exchange", s);
   hipLaunchKernelGGL(k_mesh_exchange, dim3(gr), dim3(256), 0, s, m->mi, v);
   ctx->mark_end(s);
@@ -327,8 +327,8 @@ int run_stage(phx_mesh *m, int st, bool last, hipStream_t s) {
   launch_stage_fluxes(ctx, k, st, s);
   if (m->d.flux_fixup && m->outflow_mask) {
     ctx->mark_begin("fix_fluxes", s);
-    hipLaunchKernelGGL(k_mesh_fix_fluxes, dim3((unsigned)groups(m, 5)), dim3(256), 0, s, m->mi,
-                       pv(m, st, 5), m->outflow_mask);
+    hipLaunchKernelGGL(k_mesh_fix_fluxes, dim3((unsigned)groups(m, 6)), dim3(256), 0, s, m->mi,
+                       pv(m, st, 6), m->outflow_mask);
     ctx->mark_end(s);
   }
   const bool learned = k->mlp != nullptr;

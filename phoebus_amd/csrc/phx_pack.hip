@@ -36,10 +36,13 @@ int build_start(phx_pack *k, int kind) {
     const long nf = 2L * ((long)(g.ie[1] - g.is[1] + 1) * (g.ie[2] - g.is[2] + 1) +
                           (long)(g.ie[0] - g.is[0] + 1) * (g.ie[2] - g.is[2] + 1) +
                           (long)(g.ie[0] - g.is[0] + 1) * (g.ie[1] - g.is[1] + 1));
+    const long nint = (long)(g.ie[0] - g.is[0] + 1) * (g.ie[1] - g.is[1] + 1) *
+                      (g.ie[2] - g.is[2] + 1);
     const long c = kind < 3    ? pack_groups_flux(kind, k->p.recon, g, k->tiles[kind])
                    : kind < 5 ? pack_groups_cells(g, kind == 3)
                    : kind == 5 ? (g.ncell + 255) / 256
-                               : (nf + 255) / 256;
+                   : kind == 6 ? (nf + 255) / 256
+                               : (g.ncell - nint + 255) / 256;
     st[b + 1] = st[b] + c;
   }
   if (st[k->n] > (long)UINT_MAX) return PHX_ERR_INVALID;
@@ -198,7 +201,7 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
     ok = hipMalloc(&k->entries[st], sizeof(PackEntry) * nblocks) == hipSuccess &&
          hipMemcpy(k->entries[st], e[st].data(), sizeof(PackEntry) * nblocks,
                    hipMemcpyHostToDevice) == hipSuccess;
-  for (int kind = 0; kind < 7 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
+  for (int kind = 0; kind < 8 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
   ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max({k->groups[4], k->groups[5], 1L})) ==
                  hipSuccess;
   ok = ok && hipMalloc(&k->counters, 2 * sizeof(unsigned long long)) == hipSuccess &&

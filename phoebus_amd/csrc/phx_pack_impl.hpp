@@ -24,9 +24,9 @@ struct phx_pack {
   std::vector<BlockGeom> geoms;
   PackEntry *entries[2] = {nullptr, nullptr};  // device: stage 1 and stage 2 roles
   // work lists: 0..2 flux sweeps, 3 CT corners, 4 interior cells, 5 entire blocks,
-  // 6 the ghost layer face-adjacent to the interior
-  long *start[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  long groups[7] = {0, 0, 0, 0, 0, 0, 0};
+  // 6 the ghost layer face-adjacent to the interior, 7 every ghost cell
+  long *start[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  long groups[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   TileShape tiles[3];
   double *partial = nullptr;  // [groups[4]] CFL partial minima
   DevFixup fx{};              // c2p fixup and floors after the update (off by default)
