@@ -623,6 +623,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched con2prim leg")
     ap.add_argument("--no-tune", action="store_true", help="default flux tile shapes")
+    ap.add_argument("--yz-shape", default="", metavar="TI,TS",
+                    help="fix the y/z flux sweep shape of the 128^3 stage (TS < 0: marching "
+                         "waves of TI columns in -TS segments) instead of autotuning it")
     ap.add_argument("--c3-states", type=int, default=1 << 26)
     ap.add_argument("--c3-pairs", type=int, default=1 << 22)
     ap.add_argument("--c3-train-steps", type=int, default=30000)
@@ -684,9 +687,13 @@ def main():
     phx.L.phx_rt_stream_create(C.byref(sp))
     s = sp.value
 
-    # pick the y/z flux tile shapes for this block and reconstruction (times every
-    # admissible shape on these fields; results do not depend on it)
-    if not args.no_tune:
+    # pick the y/z flux sweep shapes for this block and reconstruction (times every
+    # admissible shape on these fields; results do not depend on it), or fix them
+    if args.yz_shape:
+        ti, ts = (int(x) for x in args.yz_shape.split(","))
+        for d in (1, 2):
+            phx.set_tile_shape(p, b, d, ti, ts)
+    elif not args.no_tune:
         phx.autotune(p, b, f_stage, reps=3, stream=s)
         phx.L.phx_rt_stream_synchronize(s)
     tiles = {d: list(phx.tile_shape(p, b, d)) for d in (1, 2)}
