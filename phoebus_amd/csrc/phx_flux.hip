@@ -255,55 +255,63 @@ __device__ __forceinline__ void flux_march_body(const DevParams &p, const BlockG
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
   constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
-  const int MR = kMarchLanes / ti, Q = MR + 2 * H;
-  const int lane = threadIdx.x, cl = lane % ti, r = lane / ti;
+  const int MR = kMarchLanes / ti, Q = MR + 2 * H, lg = __builtin_ctz((unsigned)ti);
+  const int lane = threadIdx.x;
   const int dt_ = b.ndim > T;
   const int nci = b.ie[0] - b.is[0] + 3;  // i over [is-1, ie+1]
   const long ncol = (long)nci * (b.ie[T] - b.is[T] + 1 + 2 * dt_);
   const long nchunk = (ncol + ti - 1) / ti;
   const int sg = (int)(bx / nchunk);
-  long col = (bx % nchunk) * ti + cl;
+  long col = (bx % nchunk) * ti + (lane & (ti - 1));
   const bool colv = col < ncol;
   if (!colv) col = ncol - 1;
   const int i = b.is[0] - 1 + (int)(col % nci);
   const int t = b.is[T] - dt_ + (int)(col / nci);
   const int nown = DIR == 1 ? b.nj : b.nk;
-  // own-direction faces [s-1, e+1] in nseg segments of L; a segment's march starts one
-  // cell early for the upper edge its first face needs
+  // own-direction faces [s-1, e+1] in nseg segments; a segment's march starts one cell
+  // early for the upper edge its first face needs.  The march takes K = ceil((nf + nseg) /
+  // MR) steps in all; segment sg takes ks of them (the first K % nseg segments one more)
+  // and covers the ks * MR - 1 faces after the previous segments' (the last one up to
+  // nf), so only the last step of the last segment can have idle rows.
   const int nf = b.ie[DIR] - b.is[DIR] + 3;
-  const int L = (nf + nseg - 1) / nseg;
-  const int flo = b.is[DIR] - 1 + sg * L;
-  const int fhi = b.is[DIR] - 1 + min((sg + 1) * L, nf) - 1;
-  if (flo > fhi) return;  // an empty trailing segment (the whole workgroup)
+  const int K = (nf + nseg + MR - 1) / MR, kq = K / nseg, kr = K % nseg;
+  const int ks = kq + (sg < kr ? 1 : 0);
+  const int f0 = (sg * kq + min(sg, kr)) * MR - sg;  // faces of the segments before
+  const int flo = b.is[DIR] - 1 + f0;
+  const int fhi = b.is[DIR] - 1 + min(f0 + ks * MR - 1, nf) - 1;
+  if (ks == 0 || flo > fhi) return;  // an empty trailing segment (the whole workgroup)
   const int c0 = flo - 1;
   const int nsteps = (fhi - c0 + MR) / MR;
   const long sown = DIR == 1 ? (long)b.ni : (long)b.ni * b.nj;
-  const long base = DIR == 1 ? (long)t * b.nj * b.ni + i : (long)t * b.ni + i;
   double *ring = lds;                // [NP][Q][ti]: row y in slot (y - c0 + H) % Q
   double *carry = lds + NP * Q * ti; // [NP][ti]: the upper edges of the last row
-  auto load_row = [&](int y) {
+  auto load_row = [&](long base, int cl, int y, int slot) {
     const long c = base + (long)min(max(y, 0), nown - 1) * sown;
-    const int slot = (y - c0 + H) % Q;
 #pragma unroll
     for (int v = 0; v < NP; ++v) ring[(v * Q + slot) * ti + cl] = prim[v * b.ncell + c];
   };
-  const int below = lane >= ti ? lane - ti : lane;
+  // the ring slot of this lane's cell, (m MR + r) % Q, stepped by MR with one wrap (MR < Q)
+  int rs = lane >> lg;
   for (int m = 0; m < nsteps; ++m) {
+    // opaque copies: what a step needs of the lane and its column is recomputed from them
+    // each step instead of being hoisted out of the loop and held across the Riemann
+    // solver (ti is a power of two: shifts and masks)
+    int ln = lane, ii = i, tt = t;
+    asm volatile("" : "+v"(ln), "+v"(ii), "+v"(tt));
+    const int r = ln >> lg, cl = ln & (ti - 1);
+    const long base = DIR == 1 ? (long)tt * b.nj * b.ni + ii : (long)tt * b.ni + ii;
     __syncthreads();  // the previous step's ring reads are done
     if (m == 0) {
-      for (int y = r; y < Q; y += MR) load_row(c0 - H + y);
+      for (int y = r; y < Q; y += MR) load_row(base, cl, c0 - H + y, y);
     } else {
-      load_row(c0 + m * MR + H + r);
+      // the new row m MR + 2H + r sits in slot (rs + 2H) % Q
+      load_row(base, cl, c0 + m * MR + H + r, rs + 2 * H < Q ? rs + 2 * H : rs + 2 * H - Q);
     }
     __syncthreads();
     const int cc = c0 + m * MR + r;  // this lane's cell
-    // opaque copies: what the face needs is recomputed from them each step instead of
-    // being hoisted out of the loop and held across the Riemann solver
-    int ii = i, tt = t;
-    asm volatile("" : "+v"(ii), "+v"(tt));
     int sl[2 * H + 1];
 #pragma unroll
-    for (int o = 0; o < 2 * H + 1; ++o) sl[o] = (m * MR + r + o) % Q;
+    for (int o = 0; o < 2 * H + 1; ++o) sl[o] = rs + o < Q ? rs + o : rs + o - Q;
     double lo[NP], up[NP];
 #pragma unroll
     for (int v = 0; v < NP; ++v) {
@@ -316,6 +324,7 @@ __device__ __forceinline__ void flux_march_body(const DevParams &p, const BlockG
     }
     // the upper edge of the cell below: row r - 1's, or the previous step's last row
     double ql[NP];
+    const int below = ln >= ti ? ln - ti : ln;
 #pragma unroll
     for (int v = 0; v < NP; ++v) {
       const double sh = __shfl(up[v], below, kMarchLanes);
@@ -330,14 +339,14 @@ __device__ __forceinline__ void flux_march_body(const DevParams &p, const BlockG
       X[0] = xc(b, 0, ii);
       X[T] = xc(b, T, tt);
       X[DIR] = xf(b, DIR, cc);
-      const long bb = DIR == 1 ? (long)tt * b.nj * b.ni + ii : (long)tt * b.ni + ii;
-      face_out<MHD, RIEMANN, GEOM, CACHED>(p, DIR, b, X, ql, lo, bb + (long)cc * sown,
+      face_out<MHD, RIEMANN, GEOM, CACHED>(p, DIR, b, X, ql, lo, base + (long)cc * sown,
                                            (long)(DIR == 1 ? cc : tt) * b.ni + ii, flux, fsig);
     }
+    rs = rs + MR < Q ? rs + MR : rs + MR - Q;
   }
 }
 
-// Minkowski fits four waves per SIMD (128 VGPRs; one 8-byte value spills)
+// Minkowski fits four waves per SIMD (128 VGPRs) without spilling
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
 __global__ __launch_bounds__(kMarchLanes)
 __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 4 : 1))) void k_flux_march(DevParams p, BlockGeom b, int ti,
@@ -383,7 +392,8 @@ __global__ __launch_bounds__(512) void k_pack_flux_tile(DevParams p, PackView v,
 }
 
 template <int DIR, int RECON, int RIEMANN, int GEOM>
-__global__ __launch_bounds__(kMarchLanes) void k_pack_flux_march(DevParams p, PackView v, int ti,
+__global__ __launch_bounds__(kMarchLanes)
+__attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 4 : 1))) void k_pack_flux_march(DevParams p, PackView v, int ti,
                                                                  int nseg) {
   extern __shared__ double lds[];
   const long g = xcd_remap(blockIdx.x, gridDim.x);
