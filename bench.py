@@ -559,6 +559,8 @@ def c5_torus_mesh(phx, args, log):
            "steps": K, "interior_cells": cells,
            "plain": {"ms_per_step": 1e3 * wall_all / (K - 1),
                      "zone_updates_per_s": cells * 2 * (K - 1) / wall_all,
+                     "hbm_frac_stage_bytes": cells * 2 * (K - 1) / wall_all * B_ALG_STAGE
+                                             / 1e9 / HBM_PEAK_GBS,
                      "ms_per_step_after_switch": 1e3 * wall_b / (K - L),
                      "time": t_plain, "dt": dt_plain, "ncycle": n_plain,
                      "launch_ms_per_stage": launch_plain, "con2prim_last_stage": it_plain},

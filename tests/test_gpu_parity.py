@@ -476,3 +476,34 @@ def test_tile_shapes_and_autotune_do_not_change_fluxes(phx):
     synchronize()
     np.testing.assert_array_equal(st.flux.get(), first[0])
     np.testing.assert_array_equal(st.face_sig.get(), first[1])
+
+
+@pytest.mark.parametrize("geom", [A.GEOM_SPHERICAL_KS, A.GEOM_FMKS])
+def test_sweep_shapes_agree_on_curved_metrics(phx, geom):
+    """Tiles and marching waves run the same arithmetic per face, metric included: every
+    shape gives the same bits on Kerr-Schild and FMKS blocks (the oracle bars for these
+    metrics are checked with the default shapes in test_block_stage_parity)."""
+    say(f"[gpu] y/z sweep shapes on geom {geom}")
+    rng = np.random.default_rng(78 + geom)
+    b = geom_block(3, (20, 18, 16), geom)
+    p = geom_params(geom, gamma=13.0 / 9.0, mhd=True, recon=A.RECON_WENO5Z,
+                    riemann=A.RIEMANN_HLL)
+    st = BlockState(p, b)
+    st.prim.upload(random_block(rng, b, True, 13.0 / 9.0, geom))
+    f = st.fields()
+    first = None
+    for ti, ts in [(4, 32), (2, 16), (8, -1), (4, -3), (16, -2), (1, -5)]:
+        for d in (1, 2):
+            phx.set_tile_shape(p, b, d, ti, ts)
+        st.flux.zero()
+        st.face_sig.zero()
+        phx.calculate_fluxes(p, b, f)
+        synchronize()
+        got = (st.flux.get(), st.face_sig.get())
+        if first is None:
+            first = got
+        np.testing.assert_array_equal(got[0], first[0], err_msg=f"shape {(ti, ts)}")
+        np.testing.assert_array_equal(got[1], first[1], err_msg=f"shape {(ti, ts)}")
+    assert np.isfinite(first[0]).any()
+    for d in (1, 2):
+        phx.set_tile_shape(p, b, d, 0, 0)
