@@ -652,10 +652,15 @@ def main():
     from phoebus_amd.api import BlockState, Phx, block, fluid_params
     from phoebus_amd.device import DeviceArray, Event, synchronize
 
+    def step(msg):  # progress on stderr, flushed before each GPU step
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     nx = args.nx
     rng = np.random.default_rng(SEED)
     prim_h = make_input(rng, nx, NG)
+    step("HIP context")
     phx = Phx(0)
+    step(f"128^3 stage set-up ({args.recon})")
     recon = {"weno5z": A.RECON_WENO5Z, "plm": A.RECON_LINEAR, "mp5": A.RECON_MP5}[args.recon]
     p = fluid_params(gamma=GAMMA, mhd=True, recon=recon, riemann=A.RIEMANN_HLL,
                      c2p_tol=1e-10, c2p_max_iter=100)
@@ -696,9 +701,11 @@ def main():
         for d in (1, 2):
             phx.set_tile_shape(p, b, d, ti, ts)
     elif not args.no_tune:
+        step("autotune y/z sweeps")
         phx.autotune(p, b, f_stage, reps=3, stream=s)
         phx.L.phx_rt_stream_synchronize(s)
     tiles = {d: list(phx.tile_shape(p, b, d)) for d in (1, 2)}
+    step(f"timed stages, shapes {tiles}")
 
     def stage():
         # the warm-start guess of each repeated stage is the input state's own root
