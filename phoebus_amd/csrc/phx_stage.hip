@@ -14,7 +14,7 @@ namespace phx {
 
 // interior cell t of block b: returns the cell's CFL time (DBL_MAX for t beyond the block
 // or when with_dt is false)
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, int NDIM = 3>
 __device__ __forceinline__ double update_con2prim_body(
     const DevParams &p, const BlockGeom &b, long t, const double *__restrict__ flux,
     const double *__restrict__ src, double *u_stage, const double *u_base, double beta,
@@ -36,25 +36,34 @@ __device__ __forceinline__ double update_con2prim_body(
     // UpdateIndependentData -- the operation order of the separate kernels
     const double a1 = b.dx[1] * b.dx[2], a2 = b.dx[0] * b.dx[2], a3 = b.dx[0] * b.dx[1];
     const double vol = b.dx[0] * b.dx[1] * b.dx[2];
-    double cs[NC];
+    // every component is read and computed before anything is stored: U_s may be the same
+    // container as U^n (stage 1) and U_{s+1} the same as U^n (stage 2), so a store would
+    // keep the compiler from issuing the next component's loads ahead of it; with the
+    // stores last, all the loads of the update can be in flight at once.  (The geometric
+    // source terms exist exactly for the curved metrics.)
+    constexpr bool SRC = GEOM != PHX_GEOM_MINKOWSKI;
+    double cs[NC], usv[NC];
 #pragma unroll
     for (int l = 0; l < NC; ++l) {
       const double *F1 = flux + (long)l * n;
       double du = a1 * F1[c + 1] - a1 * F1[c];
-      if (b.ndim >= 2) {
+      if constexpr (NDIM >= 2) {
         const double *F2 = flux + ((long)NC + l) * n;
         du += a2 * F2[c + sj] - a2 * F2[c];
       }
-      if (b.ndim == 3) {
+      if constexpr (NDIM == 3) {
         const double *F3 = flux + ((long)2 * NC + l) * n;
         du += a3 * F3[c + sk] - a3 * F3[c];
       }
       double dudt = -du / vol;
-      if (src != nullptr && l >= 1 && l <= 4) dudt = dudt + src[(long)l * n + c];
-      const double us = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
-      u_stage[(long)l * n + c] = us;
-      cs[l] = us + bdt * dudt;
+      if constexpr (SRC) {
+        if (l >= 1 && l <= 4) dudt = dudt + src[(long)l * n + c];
+      }
+      usv[l] = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
+      cs[l] = usv[l] + bdt * dudt;
     }
+#pragma unroll
+    for (int l = 0; l < NC; ++l) u_stage[(long)l * n + c] = usv[l];
     // U_{s+1} goes out before the inversion, which re-derives it only where it used a floor
     // or ceiling (con2prim_fields); no conserved variable is live across the solve
 #pragma unroll
@@ -97,14 +106,14 @@ __device__ __forceinline__ double update_con2prim_body(
   return ldt_min;
 }
 
-template <bool MHD, int GEOM>
+template <bool MHD, int GEOM, int NDIM>
 __global__ __launch_bounds__(256) void k_update_con2prim(
     DevParams p, BlockGeom b, const double *__restrict__ flux, const double *__restrict__ src,
     double *u_stage, const double *u_base, double beta, double bdt,
     double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
     double *__restrict__ fail, int32_t *__restrict__ iters, double *__restrict__ partial) {
-  const double v = update_con2prim_body<MHD, GEOM>(
+  const double v = update_con2prim_body<MHD, GEOM, false, NDIM>(
       p, b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, src, u_stage, u_base, beta, bdt,
       u_out, prim_out, temp, mu, csig, fsig, fail, iters, partial != nullptr);
   if (partial) {
@@ -197,7 +206,12 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
              f->temp, f->c2p_mu, f->cell_sig, f->face_sig, f->fail, a->iters, partial
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {
-      hipLaunchKernelGGL((k_update_con2prim<decltype(H)::value, decltype(G)::value>), dim3(nb), dim3(256), 0, s, ARGS);
+      constexpr bool MH = decltype(H)::value;
+      constexpr int GM = decltype(G)::value;
+      auto kernel = b->ndim == 3   ? k_update_con2prim<MH, GM, 3>
+                    : b->ndim == 2 ? k_update_con2prim<MH, GM, 2>
+                                   : k_update_con2prim<MH, GM, 1>;
+      hipLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, s, ARGS);
     });
   });
 #undef ARGS
