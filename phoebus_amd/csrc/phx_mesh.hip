@@ -37,16 +37,25 @@ __device__ __forceinline__ void block_loc(const MeshInfo &m, int gid, int lx[3])
 // the FillGhost fields of a cell (bc_vars = primitive, fluid.cpp:159-165): primitives,
 // temperature, con2prim root, fail flag and the conserved variables of the container
 constexpr int kNprim = 10, kNcons = 8;
+// (all 21 values are read before any is written: the source and destination fields share
+// one allocation as far as the compiler can tell, so interleaved copies would each wait a
+// full memory round trip)
 __device__ __forceinline__ void copy_ghost_fields(const PackEntry &D, long cd, const PackEntry &S,
                                                   long cs) {
   const long nd = D.bg.ncell, ns = S.bg.ncell;
+  double pr[kNprim], u[kNcons];
 #pragma unroll
-  for (int v = 0; v < kNprim; ++v) D.prim_out[v * nd + cd] = S.prim_out[v * ns + cs];
+  for (int v = 0; v < kNprim; ++v) pr[v] = S.prim_out[v * ns + cs];
 #pragma unroll
-  for (int v = 0; v < kNcons; ++v) D.u_out[v * nd + cd] = S.u_out[v * ns + cs];
-  if (D.temp) D.temp[cd] = S.temp[cs];
-  D.mu[cd] = S.mu[cs];
-  D.fail[cd] = S.fail[cs];
+  for (int v = 0; v < kNcons; ++v) u[v] = S.u_out[v * ns + cs];
+  const double t = D.temp ? S.temp[cs] : 0.0, m = S.mu[cs], f = S.fail[cs];
+#pragma unroll
+  for (int v = 0; v < kNprim; ++v) D.prim_out[v * nd + cd] = pr[v];
+#pragma unroll
+  for (int v = 0; v < kNcons; ++v) D.u_out[v * nd + cd] = u[v];
+  if (D.temp) D.temp[cd] = t;
+  D.mu[cd] = m;
+  D.fail[cd] = f;
 }
 
 // boundary exchange between blocks of one level (parthenon's same-level ghost fill): a
