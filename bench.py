@@ -751,10 +751,19 @@ def main():
     achieved = kb[dom] / (launch_ms[dom] * 1e-3) / 1e9
     # measured HBM bytes of that launch: rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
     # workload, scaled by the same counters on a copy of known size
+    # (per sweep shape for the y/z sweeps; null for a shape not measured)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_traffic.json")
     if nx == 128 and args.recon == "weno5z" and os.path.exists(tfile):
-        tk = json.load(open(tfile))["kernels"].get(dom)
+        tkern = json.load(open(tfile))["kernels"]
+        key = dom
+        if dom in ("flux_y", "flux_z"):
+            t = tiles[1 if dom == "flux_y" else 2]
+            key = f"{dom}@{t[0]}x{t[1]}"
+            other = f"{'flux_z' if dom == 'flux_y' else 'flux_y'}@{t[0]}x{t[1]}"
+            if key not in tkern and other in tkern:  # the same sweep code and face count
+                key = other
+        tk = tkern.get(key)
         traffic = tk["traffic_bytes"] if tk else None
     # the flux sweeps are bound by VALU issue: FP64 FLOP/s and issue occupancy from the
     # instruction counts of rocprofv3 SQ_INSTS_VALU* runs of this workload (per launch,

@@ -547,7 +547,8 @@ __global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
         root = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
       } else {
         fb = 1;
-        good = regula_falsi(z.res, 0.0, 1.0, p.tol, c, p.max_iter, root, it);
+        // the plain solve with the caller's guess when there is one, else the centre
+        good = regula_falsi(z.res, 0.0, 1.0, p.tol, root > 0.0 ? root : c, p.max_iter, root, it);
         ev += 2;
         if (!good) it = -1;
       }
@@ -642,7 +643,9 @@ __global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
     mu_new = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
   } else {
     fb = true;
-    good = regula_falsi(z.res, 0.0, 1.0, p.tol, cc, p.max_iter, mu_new, it);
+    // the plain solve with the plain solver's own guess, the cell's previous root, when it
+    // has one (a time-dependent run), else the network's centre
+    good = regula_falsi(z.res, 0.0, 1.0, p.tol, root > 0.0 ? root : cc, p.max_iter, mu_new, it);
   }
   double st = 0.0;
   if (good) {

@@ -307,16 +307,19 @@ int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
   static const int kTs[] = {16, 24, 32, 48, 64, 96, 128, -1, -2, -3, -4, -6, -8};
   int rc = PHX_OK;
   for (int d = 1; d < 3 && rc == PHX_OK; ++d) {
-    float best = 1e30f;
-    TileShape best_shape = k->tiles[d];
     std::vector<TileShape> seen;
     for (int ti : kTi)
       for (int ts : kTs) {
         const TileShape c = pack_shape(k, d, TileShape{ti, ts});
         bool dup = false;
         for (const auto &x : seen) dup = dup || (x.ti == c.ti && x.ts == c.ts);
-        if (dup) continue;
-        seen.push_back(c);
+        if (!dup) seen.push_back(c);
+      }
+    // two passes, each candidate kept at its faster pass (as phx_autotune)
+    std::vector<float> t(seen.size(), 1e30f);
+    for (int pass = 0; pass < 2 && rc == PHX_OK; ++pass)
+      for (size_t q = 0; q < seen.size(); ++q) {
+        const TileShape &c = seen[q];
         k->tiles[d] = c;
         if ((rc = build_start(k, d)) != PHX_OK) break;
         const PackView v = view(k, 1, d);
@@ -332,12 +335,12 @@ int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) {
-          best = ms;
-          best_shape = c;
-        }
+        if (ms < t[q]) t[q] = ms;
       }
-    k->tiles[d] = best_shape;
+    size_t best = 0;
+    for (size_t q = 1; q < seen.size(); ++q)
+      if (t[q] < t[best]) best = q;
+    k->tiles[d] = seen[best];
     if (rc == PHX_OK) rc = build_start(k, d);
   }
   (void)hipEventDestroy(e0);

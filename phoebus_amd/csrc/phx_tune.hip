@@ -65,30 +65,33 @@ int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
         for (const auto &c : cand) dup = dup || (c.ti == r.ti && c.ts == r.ts);
         if (!dup) cand.push_back(r);
       }
-    float best = 1e30f;
-    TileShape best_shape;
-    for (const auto &c : cand) {
-      launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
-                    f->face_sig, s, c);  // warm-up
-      (void)hipEventRecord(e0, s);
-      for (int r = 0; r < reps; ++r)
-        launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim,
-                      f->flux, f->face_sig, s, c);
-      (void)hipEventRecord(e1, s);
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        const int lrc = ctx->check_launch();
-        return lrc ? lrc : ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");
+    // two passes over the candidates, each kept at its faster pass: one slow measurement
+    // (a clock step, another process) does not decide the shape
+    std::vector<float> t(cand.size(), 1e30f);
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t k = 0; k < cand.size(); ++k) {
+        const TileShape &c = cand[k];
+        launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
+                      f->face_sig, s, c);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r)
+          launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim,
+                        f->flux, f->face_sig, s, c);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          const int lrc = ctx->check_launch();
+          return lrc ? lrc : ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < t[k]) t[k] = ms;
       }
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) {
-        best = ms;
-        best_shape = c;
-      }
-    }
-    ctx->tiles[phx_ctx::tile_key(p, b, d)] = best_shape;
+    size_t best = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
+      if (t[k] < t[best]) best = k;
+    ctx->tiles[phx_ctx::tile_key(p, b, d)] = cand[best];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
