@@ -37,34 +37,6 @@ CLOCK_HZ = 2.4e9             # peak engine clock
 B_ALG_STAGE = 52 * 8
 
 
-def make_input(rng, n_int, ng):
-    n = n_int + 2 * ng
-    sh = (n, n, n)
-    rho = np.exp(rng.normal(0.0, 1.0, sh))
-    P = rho * rng.uniform(0.01, 1.0, sh)
-    W = rng.uniform(1.0, 5.0, sh)
-    nv = rng.normal(size=(3,) + sh)
-    nv /= np.linalg.norm(nv, axis=0)
-    vp = np.sqrt(W * W - 1.0) * nv
-    A = rng.uniform(-1.0, 1.0, (3,) + sh)
-    d = lambda f, ax: 0.5 * (np.roll(f, -1, axis=ax) - np.roll(f, 1, axis=ax))
-    # axes of (k, j, i): x -> 2, y -> 1, z -> 0
-    Bx = d(A[2], 1) - d(A[1], 0)
-    By = d(A[0], 0) - d(A[2], 2)
-    Bz = d(A[1], 2) - d(A[0], 1)
-    B = np.stack([Bx, By, Bz])
-    sig = np.sum(B * B, axis=0) / rho
-    B *= np.sqrt(0.1 / np.median(sig))
-    prim = np.empty((10,) + sh)
-    prim[0] = rho
-    prim[1:4] = vp
-    prim[4] = P / (GAMMA - 1.0)
-    prim[5:8] = B
-    prim[8] = P
-    prim[9] = GAMMA
-    return np.ascontiguousarray(prim)
-
-
 def kernel_bytes(nx):
     """Algorithmic bytes per launch (each input read once, each output written once)."""
     cells = nx ** 3
@@ -651,13 +623,14 @@ def main():
     from phoebus_amd import _abi as A
     from phoebus_amd.api import BlockState, Phx, block, fluid_params
     from phoebus_amd.device import DeviceArray, Event, synchronize
+    from phoebus_amd.workloads import grmhd_block_prims
 
     def step(msg):  # progress on stderr, flushed before each GPU step
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     nx = args.nx
     rng = np.random.default_rng(SEED)
-    prim_h = make_input(rng, nx, NG)
+    prim_h = grmhd_block_prims(rng, nx, NG, GAMMA)
     step("HIP context")
     phx = Phx(0)
     step(f"128^3 stage set-up ({args.recon})")

@@ -316,7 +316,8 @@ __device__ __forceinline__ double cell_dt(const BlockGeom &bg, const PackEntry &
     const double s = smax(E.csig[d * n + c], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
     ldt += s / bg.dx[d];
   }
-  return 1.0 / (ldt + 1.e-50);
+  const double t = 1.0 / (ldt + 1.e-50);
+  return isnan(t) ? DBL_MAX : t;  // a NaN time constrains nothing (as a serial minimum)
 }
 
 // the floors' check pass (on the interior cells, or on the entire blocks for a work list

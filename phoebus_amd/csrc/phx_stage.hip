@@ -100,7 +100,11 @@ __device__ __forceinline__ double update_con2prim_body(
         const double s = smax(sig[d], smax(fsig[d * n + c], fsig[d * n + c + off]));
         ldt += s / b.dx[d];
       }
-      ldt_min = 1.0 / (ldt + 1.e-50);
+      // a NaN time (a zone whose speeds are NaN, e.g. P = 0 on the energy floor, where
+      // Gamma_1 = 0/0 as in the reference) constrains nothing: the serial minimum of the
+      // reference and of k_dt_partial never lets a NaN replace a number
+      const double t = 1.0 / (ldt + 1.e-50);
+      ldt_min = isnan(t) ? DBL_MAX : t;
     }
   }
   return ldt_min;

@@ -8,6 +8,11 @@ components are drawn isotropic in an orthonormal frame and scaled by 1/sqrt(gamm
 into coordinate components, so W and B^2 = sigma rho hold up to the off-diagonal metric
 terms.  Primitive layout: rho, v1..3 (W v), u, B1..3, P, gamma1.
 
+grmhd_block_prims: the 128^3-class Minkowski GRMHD block of the stage benchmark: rho =
+exp N(0, 1), P = rho U(0.01, 1), Lorentz factor ~ U(1, 5) in isotropic directions, B the
+centred curl of a random vector potential scaled to median b^2/rho = 0.1, over the entire
+block (ghosts included).
+
 shock_tube_prims: the 1-D special-relativistic shock tube (shock_tube.cpp:79-119) on a
 Minkowski block, left state for cell centres x < 0.5, u = P/(Gamma-1) in closed form.
 """
@@ -15,6 +20,35 @@ import numpy as np
 
 GAMMA_C2P = 4.0 / 3.0
 SPIN = 0.9375
+
+
+def grmhd_block_prims(rng, n_int, ng, gamma):
+    """Primitives [10][n][n][n] (n = n_int + 2 ng) of the stage benchmark's block."""
+    n = n_int + 2 * ng
+    sh = (n, n, n)
+    rho = np.exp(rng.normal(0.0, 1.0, sh))
+    P = rho * rng.uniform(0.01, 1.0, sh)
+    W = rng.uniform(1.0, 5.0, sh)
+    nv = rng.normal(size=(3,) + sh)
+    nv /= np.linalg.norm(nv, axis=0)
+    vp = np.sqrt(W * W - 1.0) * nv
+    A = rng.uniform(-1.0, 1.0, (3,) + sh)
+    d = lambda f, ax: 0.5 * (np.roll(f, -1, axis=ax) - np.roll(f, 1, axis=ax))
+    # axes of (k, j, i): x -> 2, y -> 1, z -> 0
+    Bx = d(A[2], 1) - d(A[1], 0)
+    By = d(A[0], 0) - d(A[2], 2)
+    Bz = d(A[1], 2) - d(A[0], 1)
+    B = np.stack([Bx, By, Bz])
+    sig = np.sum(B * B, axis=0) / rho
+    B *= np.sqrt(0.1 / np.median(sig))
+    prim = np.empty((10,) + sh)
+    prim[0] = rho
+    prim[1:4] = vp
+    prim[4] = P / (gamma - 1.0)
+    prim[5:8] = B
+    prim[8] = P
+    prim[9] = gamma
+    return np.ascontiguousarray(prim)
 
 
 def ks_diag_gamma(r, th, a=SPIN):
