@@ -59,8 +59,10 @@ def cpu_baseline(prim, cons, mu0, args):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
-    orecon = {"weno5z": O.RECON_WENO5, "plm": O.RECON_LINEAR, "mp5": O.RECON_MP5}[args.recon]
-    op = O.default_params(gamma=GAMMA, mhd=True, recon=orecon, riemann=O.RIEMANN_HLL,
+    orecon = {"weno5z": O.RECON_WENO5, "plm": O.RECON_LINEAR, "mp5": O.RECON_MP5,
+              "constant": O.RECON_CONSTANT}[args.recon]
+    oriem = {"hll": O.RIEMANN_HLL, "llf": O.RIEMANN_LLF}[args.riemann]
+    op = O.default_params(gamma=GAMMA, mhd=True, recon=orecon, riemann=oriem,
                           tol=1e-10, max_iter=100)
     n = args.cpu_nx
     sl = slice(0, n + 2 * NG)
@@ -591,7 +593,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=NX)
-    ap.add_argument("--recon", choices=("weno5z", "plm", "mp5"), default="weno5z")
+    ap.add_argument("--recon", choices=("weno5z", "plm", "mp5", "constant"), default="weno5z")
+    ap.add_argument("--riemann", choices=("hll", "llf"), default="hll")
     ap.add_argument("--cpu-nx", type=int, default=NX)
     ap.add_argument("--cpu-stages", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -634,8 +637,10 @@ def main():
     step("HIP context")
     phx = Phx(0)
     step(f"128^3 stage set-up ({args.recon})")
-    recon = {"weno5z": A.RECON_WENO5Z, "plm": A.RECON_LINEAR, "mp5": A.RECON_MP5}[args.recon]
-    p = fluid_params(gamma=GAMMA, mhd=True, recon=recon, riemann=A.RIEMANN_HLL,
+    recon = {"weno5z": A.RECON_WENO5Z, "plm": A.RECON_LINEAR, "mp5": A.RECON_MP5,
+             "constant": A.RECON_CONSTANT}[args.recon]
+    riemann = {"hll": A.RIEMANN_HLL, "llf": A.RIEMANN_LLF}[args.riemann]
+    p = fluid_params(gamma=GAMMA, mhd=True, recon=recon, riemann=riemann,
                      c2p_tol=1e-10, c2p_max_iter=100)
     b = block(3, (nx, nx, nx), NG, dx=(1.0 / nx,) * 3)
     st = BlockState(p, b)
@@ -727,7 +732,7 @@ def main():
     # (per sweep shape for the y/z sweeps; null for a shape not measured)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_traffic.json")
-    if nx == 128 and args.recon == "weno5z" and os.path.exists(tfile):
+    if nx == 128 and args.recon == "weno5z" and args.riemann == "hll" and os.path.exists(tfile):
         tkern = json.load(open(tfile))["kernels"]
         key = dom
         if dom in ("flux_y", "flux_z"):
@@ -743,7 +748,7 @@ def main():
     # at the tile shape in use) over the live launch times
     valu = None
     vfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_valu.json")
-    if nx == 128 and args.recon == "weno5z" and os.path.exists(vfile):
+    if nx == 128 and args.recon == "weno5z" and args.riemann == "hll" and os.path.exists(vfile):
         vk = json.load(open(vfile))["kernels"]
         valu = {}
         for name in ("flux_x", "flux_y", "flux_z", "update_con2prim"):
@@ -793,9 +798,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded; see bench.py docstring)",
         "config": {"workload": f"single 3-D GRMHD meshblock {nx}^3 + {NG} ghosts, Minkowski, "
-                               f"Gamma=13/9, {args.recon.upper()} + HLL + FluxCT + divergence/RK "
-                               "update + robust con2prim (tol 1e-10) + CFL, one RK stage",
-                   "zones": nx ** 3, "recon": args.recon, "riemann": "hll",
+                               f"Gamma=13/9, {args.recon.upper()} + {args.riemann.upper()} + "
+                               "FluxCT + divergence/RK update + robust con2prim (tol 1e-10) + "
+                               "CFL, one RK stage",
+                   "zones": nx ** 3, "recon": args.recon, "riemann": args.riemann,
                    "flux_tiles_yz": tiles},
         "roofline": roofline,
         "stage_roofline": {"bytes_per_zone_update": B_ALG_STAGE,

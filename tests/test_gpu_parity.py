@@ -431,12 +431,18 @@ def test_fused_stage_matches_oracle_sequence(phx, case):
         assert abs(d_dt.get()[0] - dt_ref) <= 1e-8 * dt_ref
 
 
-def test_tile_shapes_and_autotune_do_not_change_fluxes(phx):
-    # every admissible y/z tile and marching shape (and the tuned one) gives the same bits
-    say("[gpu] flux tile and marching shapes + autotune vs oracle")
+@pytest.mark.parametrize("recon,riemann", [
+    (A.RECON_WENO5Z, A.RIEMANN_HLL), (A.RECON_CONSTANT, A.RIEMANN_LLF),
+    (A.RECON_LINEAR, A.RIEMANN_HLL), (A.RECON_MP5, A.RIEMANN_LLF)],
+    ids=["weno5z-hll", "constant-llf", "plm-hll", "mp5-llf"])
+def test_tile_shapes_and_autotune_do_not_change_fluxes(phx, recon, riemann):
+    # every admissible y/z tile and marching shape (and the tuned one) gives the same bits,
+    # for each reconstruction's stencil halo (1, 2 or 3 cells)
+    say(f"[gpu] flux tile and marching shapes + autotune vs oracle, recon {recon} "
+        f"riemann {riemann}")
     rng = np.random.default_rng(77)
     b = block(3, (24, 20, 18), 4, dx=(0.1, 0.2, 0.3))
-    p = fluid_params(gamma=13.0 / 9.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL)
+    p = fluid_params(gamma=13.0 / 9.0, mhd=True, recon=recon, riemann=riemann)
     op, ob = oparams(p), oblock(b)
     prim = random_block(rng, b, True, 13.0 / 9.0, A.GEOM_MINKOWSKI)
     st = BlockState(p, b)
