@@ -619,8 +619,9 @@ def main():
     ap.add_argument("--c5-train-steps", type=int, default=6000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
-                                   "of it (see DESIGN.md for how the stage would shard)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not "
+                                   "part of it (the stage would shard by meshblocks, one rank "
+                                   "per GPU, with the ghost exchange between ranks)"}))
         sys.exit(2)
 
     from phoebus_amd import _abi as A
