@@ -170,6 +170,50 @@ def test_batch_p2c_c2p_parity(phx, geom, mhd):
     assert relerr(prim_g[0], prims[0]) < 1e-6
 
 
+def test_batch_edge_sizes(phx):
+    # an empty batch is a no-op (no pointer is read), a negative one is refused, and ragged
+    # sizes (not multiples of a wave or a 256-lane group) match the oracle bit for bit and
+    # write nothing past n
+    say("[gpu] batch p2c/c2p: empty, negative and ragged sizes")
+    gamma = 4.0 / 3.0
+    p = fluid_params(gamma=gamma, mhd=True, c2p_tol=1e-10)
+    op = oparams(p)
+    phx.prim_to_con_batch(p, 0, None, None, None)
+    phx.con2prim_batch(p, 0, None, None, None, None, None)
+    with pytest.raises(A.PhxError):
+        phx.con2prim_batch(p, -1, None, None, None, None, None)
+    pad, sentinel = 64, -7.0
+    for n in (1, 63, 257, 1000):
+        say(f"    n = {n}")
+        rng = np.random.default_rng(n)
+        coords = np.zeros((2, n))
+        prims = np.ascontiguousarray(PH.random_prims(rng, n, True, gamma, wmax=5.0))
+        ref_cons = O.p2c_batch(op, coords, prims)
+        nc = ref_cons.shape[0]
+        d_prim = DeviceArray.from_host(np.concatenate([prims.ravel(), np.full(pad, sentinel)]))
+        d_cons = DeviceArray.from_host(np.full(nc * n + pad, sentinel))
+        phx.prim_to_con_batch(p, n, None, d_prim, d_cons)
+        synchronize()
+        got = d_cons.get()
+        assert np.array_equal(got[:nc * n].reshape(nc, n), ref_cons), n
+        assert np.all(got[nc * n:] == sentinel), n
+        ref = O.c2p_batch(op, coords, ref_cons)
+        npr = ref["prim"].shape[0]
+        d_out = DeviceArray.from_host(np.full(npr * n + pad, sentinel))
+        d_mu = DeviceArray.from_host(np.concatenate([np.zeros(n), np.full(pad, sentinel)]))
+        d_fail = DeviceArray.from_host(np.full(n + pad, sentinel))
+        d_it = DeviceArray.from_host(np.full(n + pad, -7, np.int32))
+        phx.con2prim_batch(p, n, None, d_cons, d_out, d_mu, d_fail, d_it)
+        synchronize()
+        out, mu, fail, it = d_out.get(), d_mu.get(), d_fail.get(), d_it.get()
+        assert np.array_equal(out[:npr * n].reshape(npr, n), ref["prim"]), n
+        assert np.array_equal(mu[:n], ref["mu"]) and np.array_equal(fail[:n], ref["fail"]), n
+        assert np.array_equal(it[:n], ref["iters"]), n
+        for a, s in ((out[npr * n:], sentinel), (mu[n:], sentinel), (fail[n:], sentinel),
+                     (it[n:], -7)):
+            assert np.all(a == s), n
+
+
 # ---------------------------------------------------------------- block tasks ----------
 def random_block(rng, b, mhd, gamma, geom):
     sh = O.block_shape(oblock(b))
