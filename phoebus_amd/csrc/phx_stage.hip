@@ -176,6 +176,10 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
   if (p->mhd && b->ndim > 1 && (!f->emf || b->ng < 2)) return ctx->fail(PHX_ERR_INVALID, "CT needs emf");
   const bool curved = p->geom.kind != PHX_GEOM_MINKOWSKI;
   if (curved && !f->src) return ctx->fail(PHX_ERR_INVALID, "curved metric needs src scratch");
+  const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
+  const long nb = (n + 255) / 256;
+  if (a->dt_out && nb > kDtPartials)  // refused before anything is launched
+    return ctx->fail(PHX_ERR_INVALID, "block too large for the CFL reduction");
   const DevParams dp = make_dev_params(p);
   const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
@@ -196,13 +200,7 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
     launch_sources(p->geom.kind, p->mhd, dp, bg, f->prim, f->src, s);
     ctx->mark_end(s);
   }
-  const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
-  const long nb = (n + 255) / 256;
-  double *partial = nullptr;
-  if (a->dt_out) {
-    if (nb > kDtPartials) return ctx->fail(PHX_ERR_INVALID, "block too large for the CFL reduction");
-    partial = ctx->dt_partial;
-  }
+  double *partial = a->dt_out ? ctx->dt_partial : nullptr;
   const double *src = curved ? f->src : nullptr;
   const double bdt = a->beta * a->dt;
   ctx->mark_begin("update_con2prim", s);
