@@ -477,8 +477,11 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev m, long n,
 // loop invariants into ~270 VGPRs, which costs more than the per-block copy.)
 constexpr int kLearnedThreads = 256;
 
+// three waves per SIMD (the packed weights in LDS allow no more): the MFMA accumulators then
+// live in VGPRs and nothing spills
 template <bool MHD, int GEOM>
-__global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
+__global__ __launch_bounds__(kLearnedThreads)
+__attribute__((amdgpu_waves_per_eu(3))) void k_con2prim_batch_learned(
     DevParams p, MlpDev m, long n, const double *__restrict__ coords, double *__restrict__ cons,
     double *__restrict__ prim, double *__restrict__ mu, double *__restrict__ fail,
     int32_t *__restrict__ iters, int32_t *__restrict__ evals, int32_t *__restrict__ fallback) {
@@ -574,8 +577,12 @@ __global__ __launch_bounds__(kLearnedThreads) void k_con2prim_batch_learned(
 // semantics of the plain pass (con2prim_cell: NaN input -> fail, every other field kept;
 // an unbracketed root -> fail with iters -1), the bracket and fallback of
 // k_con2prim_batch_learned
+// (three waves per SIMD fit without spilling on Minkowski blocks only; the curved metrics
+// would spill 22-34 VGPRs)
 template <int GEOM, int KIND>
-__global__ __launch_bounds__(kLearnedThreads) void k_pack_con2prim_learned(
+__global__ __launch_bounds__(kLearnedThreads)
+__attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 3 : 1))) void
+k_pack_con2prim_learned(
     DevParams p, MlpDev m, PackView v, unsigned long long *counters) {
   using M = typename MetricOf<GEOM>::type;
   constexpr bool CACHED = GEOM == PHX_GEOM_FMKS;  // every FMKS block of a pack has its geometry
