@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <new>
+#include <numeric>
 #include <vector>
 
 #include "phx.h"
@@ -315,31 +316,51 @@ int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
         for (const auto &x : seen) dup = dup || (x.ti == c.ti && x.ts == c.ts);
         if (!dup) seen.push_back(c);
       }
-    // two passes, each candidate kept at its faster pass (as phx_autotune)
+    // ms for n launches of candidate q after one warm-up launch; < 0 on failure (rc set)
+    auto time_cand = [&](size_t q, int n) -> float {
+      const TileShape &c = seen[q];
+      k->tiles[d] = c;
+      if ((rc = build_start(k, d)) != PHX_OK) return -1.f;
+      const PackView v = view(k, 1, d);
+      launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d], c, s);
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < n; ++r)
+        launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d],
+                         c, s);
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");
+        return -1.f;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      return ms;
+    };
+    // as phx_autotune: two passes, each candidate kept at its faster pass, then the fastest
+    // four ranked by the median of three interleaved rounds of 4 x reps launches
     std::vector<float> t(seen.size(), 1e30f);
     for (int pass = 0; pass < 2 && rc == PHX_OK; ++pass)
-      for (size_t q = 0; q < seen.size(); ++q) {
-        const TileShape &c = seen[q];
-        k->tiles[d] = c;
-        if ((rc = build_start(k, d)) != PHX_OK) break;
-        const PackView v = view(k, 1, d);
-        launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d], c, s);
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r)
-          launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d],
-                           c, s);
-        (void)hipEventRecord(e1, s);
-        if (hipEventSynchronize(e1) != hipSuccess) {
-          rc = ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");
-          break;
-        }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < t[q]) t[q] = ms;
+      for (size_t q = 0; q < seen.size() && rc == PHX_OK; ++q) {
+        const float ms = time_cand(q, reps);
+        if (ms >= 0.f && ms < t[q]) t[q] = ms;
       }
-    size_t best = 0;
-    for (size_t q = 1; q < seen.size(); ++q)
-      if (t[q] < t[best]) best = q;
+    std::vector<size_t> order(seen.size());
+    std::iota(order.begin(), order.end(), size_t(0));
+    const size_t nref = std::min<size_t>(4, seen.size());
+    std::partial_sort(order.begin(), order.begin() + nref, order.end(),
+                      [&](size_t a, size_t c) { return t[a] < t[c]; });
+    float m[4][3] = {};
+    for (int r3 = 0; r3 < 3 && rc == PHX_OK; ++r3)
+      for (size_t i = 0; i < nref && rc == PHX_OK; ++i) m[i][r3] = time_cand(order[i], 4 * reps);
+    size_t best = order[0];
+    float best_med = 1e30f;
+    for (size_t i = 0; i < nref && rc == PHX_OK; ++i) {
+      std::sort(m[i], m[i] + 3);
+      if (m[i][1] < best_med) {
+        best_med = m[i][1];
+        best = order[i];
+      }
+    }
     k->tiles[d] = seen[best];
     if (rc == PHX_OK) rc = build_start(k, d);
   }
