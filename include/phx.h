@@ -24,6 +24,10 @@
  *    con2prim_robust.hpp:41-44).
  *  - No entry point allocates device memory after phx_create; scratch lives in the
  *    context.  A context is bound to one device and is not thread-safe.
+ *  - The context's scratch (the CFL partial minima written by phx_fluid_stage and
+ *    phx_estimate_dt when dt_out is set) is one buffer per context: calls that use it
+ *    must be ordered, i.e. issued on one stream, or on streams the caller orders with
+ *    events.  Two such calls in flight at once on different streams race on it.
  */
 #ifndef PHX_H_
 #define PHX_H_

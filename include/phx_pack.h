@@ -51,6 +51,7 @@ typedef struct {
 /* Copies the block table to the device; the fields stay caller-owned. */
 int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
                     const phx_pack_block *blocks, phx_pack **out);
+/* PHX_ERR_INVALID (and nothing freed) while graphs captured from the pack exist. */
 int phx_pack_destroy(phx_pack *pack);
 int phx_pack_nblocks(const phx_pack *pack, int *n, int64_t *interior_cells);
 /* Enables (fx non-NULL) or disables (NULL) fixup::ConservedToPrimitiveFixup and
@@ -69,10 +70,16 @@ int phx_pack_learned_stats(phx_pack *pack, int reset, int64_t *misses, int64_t *
 int phx_pack_stage(phx_ctx *ctx, phx_pack *pack, const phx_pack_stage_args *a,
                    phx_stream stream);
 /* Times every admissible y/z tile shape on the pack's own fields (overwriting fluxes) and
- * keeps the fastest for this pack (results do not depend on it). */
+ * keeps the fastest for this pack (results do not depend on it).  Tuning rewrites the
+ * pack's device work lists, which captured graphs reference with the grid sizes and
+ * shapes of their capture: it returns PHX_ERR_INVALID while any graph of the pack exists
+ * (tune first, then capture; or destroy the graphs, tune, and capture again). */
 int phx_pack_autotune(phx_ctx *ctx, phx_pack *pack, int reps, phx_stream stream);
 int phx_pack_tile_shape(const phx_pack *pack, int dir, int *ti, int *ts);
-/* Captures nstages stages (in order) on a non-default stream into a replayable graph. */
+/* Captures nstages stages (in order) on a non-default stream into a replayable graph.  A
+ * graph replays the stages as configured at capture (tile shapes, fixups, learned
+ * bracket); later phx_pack_set_fixup / phx_pack_set_learned calls affect new captures
+ * and eager stages only.  The pack must outlive its graphs. */
 int phx_pack_graph_create(phx_ctx *ctx, phx_pack *pack, int nstages,
                           const phx_pack_stage_args *stages, phx_stream stream,
                           phx_graph **out);

@@ -420,10 +420,11 @@ class Mesh:
 
 
 class Graph:
-    """A captured, replayable sequence of pack stages (include/phx_pack.h)."""
+    """A captured, replayable sequence of pack stages (include/phx_pack.h).  Holds its pack
+    alive: the library refuses to destroy or autotune a pack while its graphs exist."""
 
-    def __init__(self, L, h):
-        self.L, self.h = L, h
+    def __init__(self, L, h, pack=None):
+        self.L, self.h, self.pack = L, h, pack
 
     def launch(self, stream):
         rc = self.L.phx_graph_launch(self.h, stream)
@@ -463,7 +464,9 @@ class Pack:
 
     def close(self):
         if self.h:
-            self.L.phx_pack_destroy(self.h)
+            rc = self.L.phx_pack_destroy(self.h)
+            if rc != A.PHX_OK:
+                raise A.PhxError(f"phx_pack_destroy: status {rc} (close the pack's graphs first)")
             self.h = None
 
     def __del__(self):
@@ -521,4 +524,4 @@ class Pack:
         h = C.c_void_p()
         self.phx._chk(self.L.phx_pack_graph_create(self.phx.h, self.h, len(stages), arr, stream,
                                                    C.byref(h)), "phx_pack_graph_create")
-        return Graph(self.L, h.value)
+        return Graph(self.L, h.value, self)

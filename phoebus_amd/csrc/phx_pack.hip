@@ -218,6 +218,7 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
 
 int phx_pack_destroy(phx_pack *k) {
   if (!k) return PHX_OK;
+  if (k->graphs > 0) return PHX_ERR_INVALID;  // its graphs still reference the work lists
   (void)hipSetDevice(k->device);
   for (auto &e : k->entries)
     if (e) (void)hipFree(e);
@@ -298,6 +299,10 @@ int phx_pack_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a,
 
 int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
   if (!ctx || !k) return PHX_ERR_INVALID;
+  // tuning rewrites the work lists in place; a live graph would decode them with the grid
+  // size and tile shape it was captured with
+  if (k->graphs > 0)
+    return ctx->fail(PHX_ERR_INVALID, "pack autotune while graphs of the pack exist");
   if (reps < 1) reps = 3;
   hipStream_t s = ctx->stream_of(stream);
   hipEvent_t e0, e1;
@@ -411,6 +416,8 @@ int phx_pack_graph_create(phx_ctx *ctx, phx_pack *k, int nstages,
     phx_graph_destroy(G);
     return ctx->fail(PHX_ERR_DEVICE, "graph instantiation failed");
   }
+  G->pack = k;
+  ++k->graphs;
   *out = G;
   return PHX_OK;
 }
@@ -426,6 +433,7 @@ int phx_graph_destroy(phx_graph *G) {
   if (!G) return PHX_OK;
   if (G->exec) (void)hipGraphExecDestroy(G->exec);
   if (G->graph) (void)hipGraphDestroy(G->graph);
+  if (G->pack) --G->pack->graphs;
   delete G;
   return PHX_OK;
 }

@@ -34,11 +34,16 @@ struct phx_pack {
   std::vector<double *> geo;             // per block: cached geometry plane (FMKS) or empty
   unsigned long long *counters = nullptr;  // device [2]: bracket misses, learned solves
   phx::FloorQueue fq;  // the floors' repair queue, allocated when floors are set on
+  // live graphs captured from this pack: they hold its work-list pointers, grid sizes and
+  // tile shapes, so autotuning (which rewrites the work lists) and destruction wait for
+  // them to be destroyed
+  int graphs = 0;
 };
 
 struct phx_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  phx_pack *pack = nullptr;  // the pack it was captured from
 };
 
 

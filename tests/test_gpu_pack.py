@@ -210,6 +210,42 @@ def test_pack_autotune_keeps_results(phx):
     pk.close()
 
 
+def test_pack_autotune_refused_while_graph_lives(phx):
+    """Autotuning rewrites the pack's work lists in place, which a captured graph decodes
+    with the grid sizes and shapes of its capture: refused (and destroy refused) while a
+    graph of the pack exists; after the graph is closed, tuning and a new capture work and
+    the replay matches the eager stage."""
+    say("[gpu] pack autotune / destroy refused while a graph of the pack exists")
+    geom = A.GEOM_MINKOWSKI
+    rng = np.random.default_rng(9)
+    p = _params(geom)
+    blocks = _blocks(geom)
+    S = Setup(phx, p, blocks, rng, geom)
+    pk = Pack(phx, p, [(b, st, c1, it) for b, st, c1, it in
+                       zip(blocks, S.states, S.cons1, S.iters)])
+    s = C.c_void_p()
+    assert phx.L.phx_rt_stream_create(C.byref(s)) == 0
+    stream = s.value
+    g = pk.graph([Pack.args(1, dt=1e-3)], stream)
+    with pytest.raises(A.PhxError, match="graphs of the pack"):
+        pk.autotune(reps=1)
+    assert phx.L.phx_pack_destroy(pk.h) == A.PHX_ERR_INVALID
+    g.close()
+    pk.autotune(reps=1, stream=stream)
+    S.reset()
+    pk.stage(1, dt=1e-3, stream=stream)
+    phx.L.phx_rt_stream_synchronize(stream)
+    eager = S.snapshot()
+    g2 = pk.graph([Pack.args(1, dt=1e-3)], stream)
+    S.reset()
+    g2.launch(stream)
+    phx.L.phx_rt_stream_synchronize(stream)
+    _equal(S.snapshot(), eager, "replay after re-tune")
+    g2.close()
+    phx.L.phx_rt_stream_destroy(stream)
+    pk.close()
+
+
 def test_pack_rejects_bad_input(phx):
     p = _params(A.GEOM_MINKOWSKI)
     b2 = block(2, (8, 8), 4)
