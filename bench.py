@@ -5,10 +5,13 @@ Workload (single 3-D GRMHD meshblock, Minkowski, ideal gas Gamma = 13/9, 128^3 i
 + 4 ghost zones, SoA FP64): one RK stage = CalculateFluxes (WENO5-Z + HLL, x/y/z) ->
 FluxCT -> FluxDivergence + RK update -> ConservedToPrimitive (robust, tol 1e-10) on the
 interior -> EstimateTimestep.  Synthetic seeded input: rho = exp N(0,1), P = rho U(0.01,1),
-W ~ U(1,5) isotropic, B = centred-difference curl of a random vector potential scaled to a
-median magnetization b^2/rho = 0.1.  Every step repeats the stage on the same input, with
-the con2prim guess taken from the input state's own root (the warm start a time-stepping
-run sees), so the K steps do identical work.
+W ~ U(1,5) isotropic, B = centred-difference curl of a random vector potential under a
+smooth log-uniform amplitude, so the magnetization b^2/rho is log-uniform over about
+1e-3..10 (percentiles reported).  The stage runs at the input state's own CFL step
+(cfl 0.4 over its cell signal speeds).  Every step repeats the stage on the same input,
+with the con2prim guess taken from the input state's own root (the warm start a
+time-stepping run sees), so the K steps do identical work.  The same block is also timed
+with PLM and MP5 reconstruction (recon_legs).
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
 """
@@ -37,6 +40,28 @@ CLOCK_HZ = 2.4e9             # peak engine clock
 B_ALG_STAGE = 52 * 8
 
 
+def sources_digest():
+    """sha256 over the library's kernel sources and headers (the build a profile measured)."""
+    import glob
+    import hashlib
+
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "phoebus_amd", "csrc", "*.hip")) +
+                   glob.glob(os.path.join(ROOT, "phoebus_amd", "csrc", "*.hpp")) +
+                   glob.glob(os.path.join(ROOT, "include", "*.h")))
+    for f in files:
+        h.update(os.path.basename(f).encode())
+        h.update(open(f, "rb").read())
+    return h.hexdigest()
+
+
+def profile_stale(prof):
+    """True when a committed profile was taken on other kernel sources than these (its
+    "sources_sha256" differs), None when it does not record them."""
+    d = prof.get("sources_sha256")
+    return None if d is None else d != sources_digest()
+
+
 def kernel_bytes(nx):
     """Algorithmic bytes per launch (each input read once, each output written once)."""
     cells = nx ** 3
@@ -54,41 +79,99 @@ def kernel_bytes(nx):
     }
 
 
-def cpu_baseline(prim, cons, mu0, args):
-    """The CPU oracle's stage on the same input (all host threads), bounded sample."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle(log):
+    """The oracle rebuilt for this host: -O3 -march=native -ffp-contract=off -fopenmp (the
+    CPU baseline's build); None when no C compiler is usable here (the caller then times
+    the -O2 build the tests use)."""
+    import shutil
+    import tempfile
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        return None
+    d = tempfile.mkdtemp(prefix="phx_oracle_native_")
+    so = os.path.join(d, "libphx_oracle_native.so")
+    cmd = [cc, "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+           "-fPIC", "-std=c99", "-shared", "-o", so,
+           os.path.join(ROOT, "oracle", "phx_oracle.c"), "-lm"]
+    try:
+        import subprocess
+        subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    except Exception as e:  # noqa: BLE001 -- reported, the portable build is timed instead
+        log(f"cpu: native oracle build failed ({e}); timing the portable build")
+        return None
+    return so
+
+
+def cpu_baseline(prim, cons, mu0, dt, args, log=lambda m: None):
+    """The CPU oracle's stage on the same input and dt, built -O3 -march=native
+    -ffp-contract=off on this host: first touch, 3 untimed stages, then the median of
+    cpu_stages (>= 10) timed stages, on all host threads (OMP_PROC_BIND=close) over an
+    --cpu-nx^3 corner of the block and on one thread over a --cpu-nx1^3 corner (bounded
+    samples of the same workload)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
+    so = native_oracle(log)
+    if so:
+        O.LIBPATH = so
+        O._lib = None  # an earlier leg may have loaded the portable build
+    O.lib()
     orecon = {"weno5z": O.RECON_WENO5, "plm": O.RECON_LINEAR, "mp5": O.RECON_MP5,
               "constant": O.RECON_CONSTANT}[args.recon]
     oriem = {"hll": O.RIEMANN_HLL, "llf": O.RIEMANN_LLF}[args.riemann]
     op = O.default_params(gamma=GAMMA, mhd=True, recon=orecon, riemann=oriem,
                           tol=1e-10, max_iter=100)
-    n = args.cpu_nx
-    sl = slice(0, n + 2 * NG)
-    ob = O.make_block(3, [n, n, n], NG, dx=(1.0 / NX,) * 3)
-    p = np.ascontiguousarray(prim[:, sl, sl, sl])
-    u = np.ascontiguousarray(cons[:, sl, sl, sl])
-    m0 = np.ascontiguousarray(mu0[sl, sl, sl])
-    I = O.interior(ob)
-    lo = (NG, NG, NG)
-    hi = (NG + n - 1,) * 3
-    times = []
-    for _ in range(args.cpu_stages):
-        t0 = time.perf_counter()
-        flux, fsig = O.calculate_fluxes(op, ob, p)
-        O.flux_ct(op, ob, flux)
-        out = O.flux_divergence_update(op, ob, flux, None, u.copy(), u, 1.0, 1e-4)
-        pr = p.copy()
-        mu = m0.copy()
-        r = O.con2prim_block(op, ob, out, pr, mu, lo=lo, hi=hi)
-        O.estimate_dt(op, ob, r["sig"], fsig, 0.4)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": n ** 3 / t, "unit": "zone-updates/s", "cores": O.threads(),
+    all_threads = O.threads()
+
+    def run(n, threads):
+        O.set_threads(threads)
+        sl = slice(0, n + 2 * NG)
+        ob = O.make_block(3, [n, n, n], NG, dx=(1.0 / NX,) * 3)
+        p = np.ascontiguousarray(prim[:, sl, sl, sl])
+        u = np.ascontiguousarray(cons[:, sl, sl, sl])
+        m0 = np.ascontiguousarray(mu0[sl, sl, sl])
+        lo = (NG, NG, NG)
+        hi = (NG + n - 1,) * 3
+        times = []
+        for it in range(3 + args.cpu_stages):
+            t0 = time.perf_counter()
+            flux, fsig = O.calculate_fluxes(op, ob, p)
+            O.flux_ct(op, ob, flux)
+            out = O.flux_divergence_update(op, ob, flux, None, u.copy(), u, 1.0, dt)
+            pr = p.copy()
+            mu = m0.copy()
+            r = O.con2prim_block(op, ob, out, pr, mu, lo=lo, hi=hi)
+            O.estimate_dt(op, ob, r["sig"], fsig, 0.4)
+            if it >= 3:
+                times.append(time.perf_counter() - t0)
+        return float(np.median(times))
+
+    t_all = run(args.cpu_nx, all_threads)
+    t_one = run(args.cpu_nx1, 1)
+    O.set_threads(all_threads)
+    return {"value": args.cpu_nx ** 3 / t_all, "unit": "zone-updates/s", "cores": all_threads,
             "kind": "port",
-            "sample": f"{args.cpu_stages} full RK stages of a {n}^3 block of the same input "
-                      f"(C99 restatement, OpenMP, median stage {t:.3f} s)"}
+            "cpu_model": cpu_model(),
+            "build": ("gcc -O3 -march=native -ffp-contract=off -fopenmp" if so else
+                      "gcc -O2 -ffp-contract=off -fopenmp (portable build; native build failed)"),
+            "one_thread": {"value": args.cpu_nx1 ** 3 / t_one, "unit": "zone-updates/s",
+                           "cores": 1, "median_stage_s": t_one, "zones": args.cpu_nx1 ** 3},
+            "sample": f"{args.cpu_nx}^3 corner of the same block and dt, C99 restatement of the "
+                      f"stage (fluxes, CT, update, con2prim, dt) on {all_threads} threads "
+                      f"(OMP_PROC_BIND=close): median of {args.cpu_stages} stages after 3 "
+                      f"warm-ups, {t_all:.3f} s; one thread on a {args.cpu_nx1}^3 corner, "
+                      f"median {t_one:.3f} s"}
 
 
 def c3_learned_vs_plain(phx, args, log):
@@ -247,6 +330,8 @@ def c4_uneven_pack(phx, args, log):
     p = fluid_params(gamma=5.0 / 3.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
                      geom=A.GEOM_FMKS, a=0.9375, c2p_tol=1e-10, c2p_max_iter=100,
                      floors=floors, ceilings=(10.0, 1.0),
+                     flux_arith={"exact": A.FLUX_ARITH_EXACT,
+                                 "fast": A.FLUX_ARITH_FAST}[args.flux_arith],
                      mgr_params=mgr(derefine_poles=False, x0=min(x1min, max(0.0, x1min - NG * dx[0]))))
     shapes = [(n + 2 * NG,) * 3 for n, _ in blocks]
     cells = [int(np.prod(s)) for s in shapes]
@@ -428,6 +513,58 @@ def c4_uneven_pack(phx, args, log):
     return out
 
 
+def c5_cpu_totals(p, fx, N, bnx, xmin, xmax, bc, grid_prim, tot0, tot_gpu, log):
+    """The oracle's mesh driver (the CPU restatement of the same RK2 cycle, all host
+    threads) from the same initial data for len(tot_gpu) steps: the conserved totals
+    (history reducers' sums of D, S_i, tau over the interior, fluid.cpp:319-349) after each
+    step against the GPU run's, as relative differences."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    m = p.geom.mgr
+    op = O.default_params(gamma=p.gamma, mhd=True, recon=O.RECON_WENO5, riemann=O.RIEMANN_HLL,
+                          geom=O.GEOM_FMKS, a=p.geom.a, tol=p.c2p_tol, max_iter=p.c2p_max_iter,
+                          mgr_params=O.mgr(derefine_poles=bool(m.derefine_poles), h=m.h,
+                                           xt=m.xt, alpha=m.alpha, x0=m.x0, smooth=m.smooth))
+    b = p.bounds
+    op.bounds.floor_type = b.floor_type
+    op.bounds.r0, op.bounds.s0, op.bounds.ralpha, op.bounds.salpha = (b.rho0, b.sie0,
+                                                                      b.rho_exp, b.sie_exp)
+    op.bounds.rho_min_eos, op.bounds.sie_min_eos = b.rho_min_eos, b.sie_min_eos
+    op.bounds.gam_max, op.bounds.sie_max = b.gam_max, b.sie_max
+    ofx = O.Fixup(fx.enable_c2p_fixup, fx.failure_strategy, fx.enable_floors,
+                  fx.enable_mhd_floors, fx.bsqorho_max, fx.bsqou_max)
+    om = O.MeshDriver(op, N, bnx, xmin, xmax, bc, fx=ofx, inflow_check=True, cfl=0.8,
+                      dt_init_fact=1e-6)
+    for g, lx in enumerate(om.loc):
+        sl = tuple(slice(lx[d] * bnx[d], lx[d] * bnx[d] + bnx[d] + 2 * NG) for d in (2, 1, 0))
+        om.set_prim(g, np.ascontiguousarray(grid_prim[(slice(None),) + sl]))
+    om.initialize()
+
+    def totals():
+        t = np.zeros(5)
+        for g in range(len(om.blocks)):
+            t += om.cons[g][0][:5, NG:-NG, NG:-NG, NG:-NG].reshape(5, -1).sum(axis=1)
+        return t
+
+    c0 = totals()
+    rel = lambda a, b_: ((a - b_) / np.maximum(np.abs(b_), 1e-300)).tolist()
+    steps = []
+    t0 = time.perf_counter()
+    for s in range(len(tot_gpu)):
+        om.step()
+        c = totals()
+        steps.append({"step": s + 1, "cpu_totals": c.tolist(),
+                      "gpu_minus_cpu_rel": rel(tot_gpu[s], c),
+                      "drift_cpu": (c - c0).tolist(), "drift_gpu": (tot_gpu[s] - tot0).tolist()})
+        log(f"[c5] cpu oracle step {s + 1}: max rel |gpu - cpu| of the totals "
+            f"{np.max(np.abs(rel(tot_gpu[s], c))):.2e}")
+    return {"initial_rel": rel(tot0, c0), "steps": steps,
+            "cpu_s": time.perf_counter() - t0, "threads": O.threads(),
+            "what": "oracle mesh driver (C99 restatement, OpenMP) from the same initial data; "
+                    "totals of D, S_1..3, tau over the interior after each step"}
+
+
 def c5_torus_mesh(phx, args, log):
     """Full 3-D GRMHD torus: the magnetized Fishbone-Moncrief torus of the uneven-batch leg
     on one 256 x 128 x 128 FMKS grid cut into 32 meshblocks of 64 x 64 x 32, advanced
@@ -457,6 +594,8 @@ def c5_torus_mesh(phx, args, log):
     p = fluid_params(gamma=5.0 / 3.0, mhd=True, recon=A.RECON_WENO5Z, riemann=A.RIEMANN_HLL,
                      geom=A.GEOM_FMKS, a=0.9375, c2p_tol=1e-10, c2p_max_iter=100,
                      floors=floors, ceilings=(10.0, 1.0),
+                     flux_arith={"exact": A.FLUX_ARITH_EXACT,
+                                 "fast": A.FLUX_ARITH_FAST}[args.flux_arith],
                      mgr_params=mgr(derefine_poles=False, x0=min(x1min, max(0.0, x1min - NG * dx[0]))))
     fx = fixup_params(c2p_fixup=True, mhd_floors=True)
     bc = (A.BC_OUTFLOW,) * 4 + (A.BC_PERIODIC,) * 2
@@ -519,7 +658,12 @@ def c5_torus_mesh(phx, args, log):
     cells = m.interior_cells
     tot0 = totals(m)
     launch_plain = profiled_step(m)
-    wall_a = timed(m, L - 1)
+    tot_steps = [totals(m)]
+    wall_a = 0.0
+    for _ in range(max(0, args.c5_cpu_steps - 1)):  # totals after each of the first steps
+        wall_a += timed(m, 1)
+        tot_steps.append(totals(m))
+    wall_a += timed(m, L - len(tot_steps))
     wall_b = timed(m, K - L)
     t_plain, dt_plain, n_plain = m.time()
     tot_plain = totals(m)
@@ -540,6 +684,9 @@ def c5_torus_mesh(phx, args, log):
                      "launch_ms_per_stage": launch_plain, "con2prim_last_stage": it_plain},
            "conserved_totals_initial": tot0.tolist(),
            "conserved_drift_plain": (tot_plain - tot0).tolist()}
+    if not args.no_cpu and args.c5_cpu_steps > 0:
+        out["conserved_vs_cpu"] = c5_cpu_totals(p, fx, N, bnx, xmin, xmax, bc, grid_prim,
+                                                tot0, tot_steps, log)
     if args.no_c5_learned:
         return out
     # ---- learned: log pairs over the first L steps, train, switch ------------------------
@@ -587,67 +734,34 @@ def c5_torus_mesh(phx, args, log):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nx", type=int, default=NX)
-    ap.add_argument("--recon", choices=("weno5z", "plm", "mp5", "constant"), default="weno5z")
-    ap.add_argument("--riemann", choices=("hll", "llf"), default="hll")
-    ap.add_argument("--cpu-nx", type=int, default=NX)
-    ap.add_argument("--cpu-stages", type=int, default=4)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c3", action="store_true", help="skip the batched con2prim leg")
-    ap.add_argument("--no-tune", action="store_true", help="default flux tile shapes")
-    ap.add_argument("--yz-shape", default="", metavar="TI,TS",
-                    help="fix the y/z flux sweep shape of the 128^3 stage (TS < 0: marching "
-                         "waves of TI columns in -TS segments) instead of autotuning it")
-    ap.add_argument("--c3-states", type=int, default=1 << 26)
-    ap.add_argument("--c3-pairs", type=int, default=1 << 22)
-    ap.add_argument("--c3-train-steps", type=int, default=30000)
-    ap.add_argument("--no-c4", action="store_true", help="skip the uneven block pack leg")
-    ap.add_argument("--c4-blocks", type=int, default=512)
-    ap.add_argument("--c4-steps", type=int, default=5)
-    ap.add_argument("--c4-per-block", type=int, default=1)
-    ap.add_argument("--no-c4-learned", action="store_true", help="skip C4's learned-bracket run")
-    ap.add_argument("--no-c5", action="store_true", help="skip the full torus mesh leg")
-    ap.add_argument("--c5-steps", type=int, default=200)
-    ap.add_argument("--c5-log-steps", type=int, default=50)
-    ap.add_argument("--no-c5-learned", action="store_true", help="skip C5's learned-bracket run")
-    ap.add_argument("--c5-pairs", type=int, default=1 << 22)
-    ap.add_argument("--c5-train-steps", type=int, default=6000)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not "
-                                   "part of it (the stage would shard by meshblocks, one rank "
-                                   "per GPU, with the ghost exchange between ranks)"}))
-        sys.exit(2)
-
+def c2_stage(phx, args, recon_name, prim_h, step, full=True, arith=None):
+    """One RK stage of the 128^3 GRMHD block with reconstruction recon_name: the stage is
+    timed at the input state's own CFL step (EstimateTimestep's first-call form over the
+    input's cell signal speeds, cfl = 0.4), with the con2prim guess of every repeat reset to
+    the input state's own root (the warm start a time-stepping run sees), so the K steps do
+    identical work.  Returns the leg's numbers; full = True adds the roofline ratings."""
     from phoebus_amd import _abi as A
     from phoebus_amd.api import BlockState, Phx, block, fluid_params
-    from phoebus_amd.device import DeviceArray, Event, synchronize
-    from phoebus_amd.workloads import grmhd_block_prims
-
-    def step(msg):  # progress on stderr, flushed before each GPU step
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    from phoebus_amd.device import DeviceArray, synchronize
+    from phoebus_amd.workloads import magnetization
 
     nx = args.nx
-    rng = np.random.default_rng(SEED)
-    prim_h = grmhd_block_prims(rng, nx, NG, GAMMA)
-    step("HIP context")
-    phx = Phx(0)
-    step(f"128^3 stage set-up ({args.recon})")
+    step(f"{nx}^3 stage set-up ({recon_name})")
     recon = {"weno5z": A.RECON_WENO5Z, "plm": A.RECON_LINEAR, "mp5": A.RECON_MP5,
-             "constant": A.RECON_CONSTANT}[args.recon]
+             "constant": A.RECON_CONSTANT}[recon_name]
     riemann = {"hll": A.RIEMANN_HLL, "llf": A.RIEMANN_LLF}[args.riemann]
+    arith = arith or args.flux_arith
     p = fluid_params(gamma=GAMMA, mhd=True, recon=recon, riemann=riemann,
-                     c2p_tol=1e-10, c2p_max_iter=100)
+                     c2p_tol=1e-10, c2p_max_iter=100,
+                     flux_arith={"exact": A.FLUX_ARITH_EXACT, "fast": A.FLUX_ARITH_FAST}[arith])
     b = block(3, (nx, nx, nx), NG, dx=(1.0 / nx,) * 3)
     st = BlockState(p, b)
     st.prim.upload(prim_h)
     f_all = st.fields()
     phx.prim_to_con(p, b, f_all)            # product p2c builds the conserved input
+    # the stage's dt: cfl x the minimum over the input state's cell signal speeds
+    dt_d = DeviceArray((1,))
+    phx.estimate_dt(p, b, f_all, 0, 0.4, dt_d)
     # the input state's own con2prim root: the warm start of the timed stages
     prim_work = DeviceArray(st.prim.shape)
     mu0 = DeviceArray(st.c2p_mu.shape)
@@ -660,6 +774,7 @@ def main():
     lo, hi = (NG, NG, NG), (NG + nx - 1,) * 3
     phx.con2prim(p, b, f0, lo, hi, iters=it0)
     synchronize()
+    dt = float(dt_d.get()[0])
     cold_iters = it0.get()[I]
     del cons_copy, prim_work
 
@@ -667,7 +782,6 @@ def main():
     u_out = DeviceArray(st.cons.shape)
     prim_out = DeviceArray(st.prim.shape)
     iters = DeviceArray(st.shape, np.int32)
-    dt_d = DeviceArray((1,))
     f_stage = st.fields()
     sp = C.c_void_p()
     phx.L.phx_rt_stream_create(C.byref(sp))
@@ -680,16 +794,15 @@ def main():
         for d in (1, 2):
             phx.set_tile_shape(p, b, d, ti, ts)
     elif not args.no_tune:
-        step("autotune y/z sweeps")
+        step(f"autotune y/z sweeps ({recon_name})")
         phx.autotune(p, b, f_stage, reps=3, stream=s)
         phx.L.phx_rt_stream_synchronize(s)
     tiles = {d: list(phx.tile_shape(p, b, d)) for d in (1, 2)}
-    step(f"timed stages, shapes {tiles}")
+    step(f"timed stages ({recon_name}, dt {dt:.4e}), shapes {tiles}")
 
     def stage():
-        # the warm-start guess of each repeated stage is the input state's own root
         phx.L.phx_rt_memcpy_dtod_async(st.c2p_mu.ptr, mu0.ptr, mu0.nbytes, s)
-        phx.fluid_stage(p, b, f_stage, 1.0, 1e-4, st.cons, u_out, prim_out, iters=iters,
+        phx.fluid_stage(p, b, f_stage, 1.0, dt, st.cons, u_out, prim_out, iters=iters,
                         cfl=0.4, dt_out=dt_d, stream=s)
 
     for _ in range(args.warmup):
@@ -724,17 +837,26 @@ def main():
     phx.L.phx_rt_profile_stop(phx.h)
     phx.L.phx_rt_stream_destroy(s)
     launch_ms = {k: acc[k] / cnt[k] for k in acc}
+    warm_iters = iters.get()[I]
+    value = nx ** 3 / (ms_per_step * 1e-3)
+    leg = {"flux_arith": arith, "ms_per_step": ms_per_step, "zone_updates_per_s": value,
+           "stage_hbm_frac": value * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS,
+           "dt": dt, "flux_tiles_yz": tiles, "launch_ms": launch_ms,
+           "con2prim_mean_iters_warm": float(np.mean(warm_iters)),
+           "fail_count": int(np.sum(st.fail.get()[I] == 0.0))}
+    if not full:
+        return leg
 
     kb = kernel_bytes(nx)
     dom = max(launch_ms, key=lambda k: launch_ms[k])
-    achieved = kb[dom] / (launch_ms[dom] * 1e-3) / 1e9
-    # measured HBM bytes of that launch: rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
-    # workload, scaled by the same counters on a copy of known size
-    # (per sweep shape for the y/z sweeps; null for a shape not measured)
-    traffic = None
+    traffic, stale = None, None
     tfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_traffic.json")
-    if nx == 128 and args.recon == "weno5z" and args.riemann == "hll" and os.path.exists(tfile):
-        tkern = json.load(open(tfile))["kernels"]
+    if (nx == 128 and recon_name == "weno5z" and args.riemann == "hll"
+            and os.path.exists(tfile)
+            and json.load(open(tfile)).get("flux_arith", "exact") == arith):
+        tj = json.load(open(tfile))
+        stale = profile_stale(tj)
+        tkern = tj["kernels"]
         key = dom
         if dom in ("flux_y", "flux_z"):
             t = tiles[1 if dom == "flux_y" else 2]
@@ -749,9 +871,12 @@ def main():
     # at the tile shape in use) over the live launch times
     valu = None
     vfile = os.path.join(ROOT, "profiles", "stage_128cubed_weno5_hll_valu.json")
-    if nx == 128 and args.recon == "weno5z" and args.riemann == "hll" and os.path.exists(vfile):
-        vk = json.load(open(vfile))["kernels"]
-        valu = {}
+    if (nx == 128 and recon_name == "weno5z" and args.riemann == "hll"
+            and os.path.exists(vfile)
+            and json.load(open(vfile)).get("flux_arith", "exact") == arith):
+        vj = json.load(open(vfile))
+        vk = vj["kernels"]
+        valu = {"profile_stale": profile_stale(vj)}
         for name in ("flux_x", "flux_y", "flux_z", "update_con2prim"):
             key = name
             if name in ("flux_y", "flux_z"):
@@ -769,22 +894,128 @@ def main():
                           "fp64_frac": vk[key]["fp64_flop"] / sec / 1e12 / FP64_PEAK_TFLOPS,
                           "valu_issue_frac": vk[key]["valu_wave_instr"] * 4 /
                           (sec * VALU_SIMDS * CLOCK_HZ)}
-    # the dominant launch's roofline: the fused update is HBM-bound (algorithmic bytes over
-    # the 8 TB/s peak); a flux sweep, should one be the longest launch, is bound by FP64
-    # VALU issue and is rated by its measured FP64 FLOP/s against the 78.6 TFLOP/s FP64
-    # peak (the MFMA and vector FP64 peaks are the same figure on this chip)
+    # the dominant launch's roofline.  The fused update is HBM-bound: achieved = the
+    # stage's algorithmic bytes (416 B per zone-update: the compulsory state traffic of a
+    # fully fused stage) times the zones it updates, over its average duration, against
+    # 8 TB/s; the launch's own compulsory bytes (its inputs and outputs, including the face
+    # fluxes a fully fused stage would not move) are reported beside it.  A flux sweep,
+    # should one be the longest launch, is bound by FP64 VALU issue and is rated by its
+    # measured FP64 FLOP/s against the 78.6 TFLOP/s FP64 peak.
+    sec = launch_ms[dom] * 1e-3
+    if dom == "update_con2prim":
+        alg = B_ALG_STAGE * nx ** 3
+    else:
+        alg = kb[dom]
+    achieved = alg / sec / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "bytes_per_launch": kb[dom], "avg_launch_ms": launch_ms[dom]}
+                "traffic_profile_stale": stale,
+                "algorithmic_bytes_per_launch": alg,
+                "launch_bytes": kb[dom], "launch_bytes_frac": kb[dom] / sec / 1e9 / HBM_PEAK_GBS,
+                "avg_launch_ms": launch_ms[dom]}
     if dom.startswith("flux") and valu and dom in valu:
         tf = valu[dom]["fp64_tflops"]
         roofline = {"bound": "mfma", "pipe": "fp64 VALU", "kernel": dom, "achieved": tf,
                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
-                    "traffic": traffic, "valu_issue_frac": valu[dom]["valu_issue_frac"],
+                    "traffic": traffic, "traffic_profile_stale": stale,
+                    "valu_issue_frac": valu[dom]["valu_issue_frac"],
                     "avg_launch_ms": launch_ms[dom]}
-    warm_iters = iters.get()[I]
-    cons_h, mu0_h = st.cons.get(), mu0.get()
-    value = nx ** 3 / (ms_per_step * 1e-3)
+    # the magnetization of the input's interior (b^2 / rho, comoving)
+    sig = magnetization(prim_h[(slice(None),) + I])
+    leg.update({
+        "roofline": roofline, "valu_roofline": valu,
+        "magnetization": {"sigma_percentiles_1_10_50_90_99":
+                          np.percentile(sig, [1, 10, 50, 90, 99]).tolist()},
+        "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
+                     "mean_iters_cold": float(np.mean(cold_iters)),
+                     "hist_warm": np.bincount(np.maximum(warm_iters, 0).ravel()).tolist(),
+                     "hist_cold": np.bincount(np.maximum(cold_iters, 0).ravel()).tolist(),
+                     "fail_count": leg["fail_count"],
+                     "mlp_guess": None},
+        "_cons": st.cons.get(), "_mu0": mu0.get()})
+    return leg
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=NX)
+    ap.add_argument("--recon", choices=("weno5z", "plm", "mp5", "constant"), default="weno5z")
+    ap.add_argument("--riemann", choices=("hll", "llf"), default="hll")
+    ap.add_argument("--flux-arith", choices=("fast", "exact"), default="fast",
+                    help="arithmetic of the flux sweeps: fast (contraction, Newton-refined "
+                         "reciprocals; fluxes within 1e-12 of the reference's rounding) or "
+                         "exact (the reference's rounding, bit for bit)")
+    ap.add_argument("--cpu-nx", type=int, default=NX)
+    ap.add_argument("--cpu-stages", type=int, default=10)
+    ap.add_argument("--cpu-nx1", type=int, default=64,
+                    help="block edge of the one-thread CPU sample")
+    ap.add_argument("--legs", default="plm,mp5",
+                    help="further reconstructions timed on the same block (comma list)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the batched con2prim leg")
+    ap.add_argument("--no-tune", action="store_true", help="default flux tile shapes")
+    ap.add_argument("--yz-shape", default="", metavar="TI,TS",
+                    help="fix the y/z flux sweep shape of the 128^3 stage (TS < 0: marching "
+                         "waves of TI columns in -TS segments) instead of autotuning it")
+    ap.add_argument("--c3-states", type=int, default=1 << 26)
+    ap.add_argument("--c3-pairs", type=int, default=1 << 22)
+    ap.add_argument("--c3-train-steps", type=int, default=30000)
+    ap.add_argument("--no-c4", action="store_true", help="skip the uneven block pack leg")
+    ap.add_argument("--c4-blocks", type=int, default=512)
+    ap.add_argument("--c4-steps", type=int, default=5)
+    ap.add_argument("--c4-per-block", type=int, default=1)
+    ap.add_argument("--no-c4-learned", action="store_true", help="skip C4's learned-bracket run")
+    ap.add_argument("--no-c5", action="store_true", help="skip the full torus mesh leg")
+    ap.add_argument("--c5-steps", type=int, default=200)
+    ap.add_argument("--c5-log-steps", type=int, default=50)
+    ap.add_argument("--no-c5-learned", action="store_true", help="skip C5's learned-bracket run")
+    ap.add_argument("--c5-pairs", type=int, default=1 << 22)
+    ap.add_argument("--c5-train-steps", type=int, default=6000)
+    ap.add_argument("--c5-cpu-steps", type=int, default=2,
+                    help="steps of the CPU oracle mesh run compared with the GPU totals")
+    args = ap.parse_args()
+    # the CPU baseline's thread placement (read when the OpenMP runtime starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not "
+                                   "part of it (the stage would shard by meshblocks, one rank "
+                                   "per GPU, with the ghost exchange between ranks)"}))
+        sys.exit(2)
+
+    from phoebus_amd import _abi as A
+    from phoebus_amd.api import BlockState, Phx, block, fluid_params
+    from phoebus_amd.device import DeviceArray, Event, synchronize
+    from phoebus_amd.workloads import grmhd_block_prims
+
+    def step(msg):  # progress on stderr, flushed before each GPU step
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    nx = args.nx
+    rng = np.random.default_rng(SEED)
+    prim_h = grmhd_block_prims(rng, nx, NG, GAMMA)
+    step("HIP context")
+    phx = Phx(0)
+    lines = {}
+    cons_h = mu0_h = None
+    dt_cfl = None
+    other = "exact" if args.flux_arith == "fast" else "fast"
+    legs = [(args.recon, args.flux_arith), (args.recon, other)] + [
+        (r, args.flux_arith) for r in args.legs.split(",") if r and r != args.recon]
+    for recon, arith in legs:
+        main = (recon, arith) == (args.recon, args.flux_arith)
+        leg = c2_stage(phx, args, recon, prim_h, step, full=main, arith=arith)
+        if main:
+            main_leg = leg
+            cons_h, mu0_h, dt_cfl = leg.pop("_cons"), leg.pop("_mu0"), leg["dt"]
+        else:
+            for k in ("_cons", "_mu0"):
+                leg.pop(k, None)
+        lines[f"{recon}-{arith}"] = leg
+    value = main_leg["zone_updates_per_s"]
+    ms_per_step = main_leg["ms_per_step"]
     out = {
         "metric": "zone-updates/s of GRMHD RK stage; con2prim mean iters, MLP-guess vs plain",
         "value": value,
@@ -801,21 +1032,20 @@ def main():
         "config": {"workload": f"single 3-D GRMHD meshblock {nx}^3 + {NG} ghosts, Minkowski, "
                                f"Gamma=13/9, {args.recon.upper()} + {args.riemann.upper()} + "
                                "FluxCT + divergence/RK update + robust con2prim (tol 1e-10) + "
-                               "CFL, one RK stage",
+                               "CFL, one RK stage at the input state's CFL dt",
                    "zones": nx ** 3, "recon": args.recon, "riemann": args.riemann,
-                   "flux_tiles_yz": tiles},
-        "roofline": roofline,
+                   "flux_arith": args.flux_arith,
+                   "dt": dt_cfl, "cfl": 0.4,
+                   "flux_tiles_yz": main_leg["flux_tiles_yz"]},
+        "roofline": main_leg.pop("roofline"),
         "stage_roofline": {"bytes_per_zone_update": B_ALG_STAGE,
                            "achieved_GBs": value * B_ALG_STAGE / 1e9,
                            "frac": value * B_ALG_STAGE / 1e9 / HBM_PEAK_GBS},
-        "launch_ms": launch_ms,
-        "valu_roofline": valu,
-        "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
-                     "mean_iters_cold": float(np.mean(cold_iters)),
-                     "hist_warm": np.bincount(np.maximum(warm_iters, 0).ravel()).tolist(),
-                     "hist_cold": np.bincount(np.maximum(cold_iters, 0).ravel()).tolist(),
-                     "fail_count": int(np.sum(st.fail.get()[I] == 0.0)),
-                     "mlp_guess": None},
+        "launch_ms": main_leg["launch_ms"],
+        "valu_roofline": main_leg.pop("valu_roofline"),
+        "magnetization": main_leg.pop("magnetization"),
+        "con2prim": main_leg.pop("con2prim"),
+        "recon_legs": {r: lines[r] for r in lines},
     }
     if not args.no_c4:
         if "st" in locals():
@@ -833,7 +1063,8 @@ def main():
         out["con2prim"]["mlp_guess"] = c3_learned_vs_plain(
             phx, args, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prim_h, cons_h, mu0_h, args)
+        out["cpu_baseline"] = cpu_baseline(prim_h, cons_h, mu0_h, dt_cfl, args,
+                                           lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
     phx.close()
 

@@ -91,6 +91,14 @@ typedef struct {
   double gam_max, sie_max;
 } phx_bounds;
 
+/* Arithmetic of the flux sweeps (reconstruction + Riemann flux; no reference counterpart).
+ * EXACT: IEEE division and square root without contraction, rounding as the reference
+ * (fluxes bit-identical to the reference's operation order).  FAST: fused multiply-adds
+ * and Newton-refined hardware reciprocals and reciprocal square roots in the sweeps only
+ * (fluxes within about 1e-15 relative of EXACT, well inside a 1e-12 flux tolerance); every
+ * other kernel is the same in both. */
+enum { PHX_FLUX_ARITH_EXACT = 0, PHX_FLUX_ARITH_FAST = 1 };
+
 /* The fluid package parameters the path reads (fluid.cpp:44-136) plus the ideal-gas EOS
  * (eos_phoebus.cpp:73-95).  Ye is not supported (PHX_ERR_UNSUPPORTED). */
 typedef struct {
@@ -103,6 +111,7 @@ typedef struct {
   int c2p_fail_on_floors, c2p_fail_on_ceilings;
   phx_bounds bounds;
   phx_geometry geom;
+  int flux_arith; /* PHX_FLUX_ARITH_EXACT (0) or PHX_FLUX_ARITH_FAST */
 } phx_fluid_params;
 
 /* One meshblock: interior cells nx[3] (nx = 1 in inactive dimensions), ng ghost layers
@@ -112,6 +121,24 @@ typedef struct {
   int ndim, nx[3], ng;
   double xmin[3], dx[3];
 } phx_block;
+
+/* Per-component device pointers of a block's multi-component fields, for hosts whose
+ * variables are allocated separately: parthenon gives every variable, and every direction
+ * of its flux, an allocation of its own (the recon and flux packs of CalculateFluxes,
+ * PackVariablesAndFluxes at riemann.hpp:223-250, indexed per variable at fluid.cpp:686-707).
+ * A non-NULL entry is the base of that component's [nk][nj][ni] array and is used in place
+ * of the component's slice of the contiguous field (prim + v * cells, flux + (d * ncons +
+ * m) * cells, ...); NULL entries keep the contiguous slice, so a field may be passed as a
+ * table alone (its contiguous pointer NULL).  The components of one parthenon vector
+ * variable sit in its one allocation: component c at base + c * cells.  The kernels take
+ * these bases as launch arguments and index them directly; nothing is gathered or copied. */
+typedef struct {
+  double *prim[10];   /* rho, v1, v2, v3, u, B1, B2, B3, P, gamma1 (hydro: rho, v1..3, u, P,
+                         gamma1 in entries 0..6) */
+  double *cons[8];    /* D, S1, S2, S3, tau, B1, B2, B3 */
+  double *flux[3][8]; /* flux[d][m]: direction d, conserved component m */
+  double *src[8];     /* geometric sources, conserved order */
+} phx_block_vars;
 
 /* Device fields of one block (any may be NULL where the call does not use it). */
 typedef struct {
@@ -125,6 +152,7 @@ typedef struct {
   double *flux;     /* [ndim][ncons][cells]                              */
   double *emf;      /* [3][cells] corner EMFs (MHD, ndim > 1)            */
   double *src;      /* [ncons][cells] geometric sources (S_i, tau slots) */
+  const phx_block_vars *vars; /* per-component overrides of prim, cons, flux, src; nullable */
 } phx_block_fields;
 
 /* ---- context ---------------------------------------------------------------------- */
@@ -208,6 +236,9 @@ typedef struct {
   int32_t *iters;        /* [cells] con2prim iterations, nullable               */
   double cfl;
   double *dt_out;        /* [1] device, nullable                                */
+  /* per-component overrides (as phx_block_fields::vars), nullable: base_vars->cons for
+   * u_base; out_vars->cons for u_out and out_vars->prim for prim_out */
+  const phx_block_vars *base_vars, *out_vars;
 } phx_stage_args;
 
 int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,

@@ -40,6 +40,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ---- reconstruction.hpp:101-244 ------------------------------------------------------ */
 static inline double mc_lim(double dm, double dp, double alpha) { /* :103-107 */
   const double dc = (dm * dp > 0.0) * 0.5 * (dm + dp);

@@ -220,6 +220,7 @@ void orc_inflow_check_block(const orc_params *p, const orc_block *b, const int l
                             const int hi[3], double *prim);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@ PHX_OK, PHX_ERR_INVALID, PHX_ERR_UNSUPPORTED, PHX_ERR_DEVICE, PHX_ERR_NOMEM = 0,
 GEOM_MINKOWSKI, GEOM_SPHERICAL_KS, GEOM_FMKS = 0, 1, 2
 RECON_CONSTANT, RECON_LINEAR, RECON_WENO5Z, RECON_MP5 = 0, 1, 2, 3
 RIEMANN_LLF, RIEMANN_HLL = 0, 1
+FLUX_ARITH_EXACT, FLUX_ARITH_FAST = 0, 1
 BC_OUTFLOW, BC_PERIODIC = 0, 1
 DRIVER_PRIM, DRIVER_CONS, DRIVER_FAIL, DRIVER_C2P_MU = 0, 1, 2, 3
 FAILURE_INTERPOLATE, FAILURE_FLOORS = 0, 1
@@ -46,7 +47,7 @@ class FluidParams(C.Structure):
                 ("recon", C.c_int), ("riemann", C.c_int), ("c2p_tol", C.c_double),
                 ("c2p_max_iter", C.c_int), ("c2p_floor_scale_fac", C.c_double),
                 ("c2p_fail_on_floors", C.c_int), ("c2p_fail_on_ceilings", C.c_int),
-                ("bounds", Bounds), ("geom", Geometry)]
+                ("bounds", Bounds), ("geom", Geometry), ("flux_arith", C.c_int)]
 
 
 class Block(C.Structure):
@@ -54,15 +55,23 @@ class Block(C.Structure):
                 ("xmin", C.c_double * 3), ("dx", C.c_double * 3)]
 
 
+class BlockVars(C.Structure):
+    """phx_block_vars: per-component device pointers (separately allocated variables)."""
+    _fields_ = [("prim", C.c_void_p * 10), ("cons", C.c_void_p * 8),
+                ("flux", (C.c_void_p * 8) * 3), ("src", C.c_void_p * 8)]
+
+
 class BlockFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("prim", "cons", "temp", "c2p_mu", "fail", "cell_sig",
-                                           "face_sig", "flux", "emf", "src")]
+                                           "face_sig", "flux", "emf", "src")] + [
+        ("vars", C.POINTER(BlockVars))]
 
 
 class StageArgs(C.Structure):
     _fields_ = [("beta", C.c_double), ("dt", C.c_double), ("u_base", C.c_void_p),
                 ("u_out", C.c_void_p), ("prim_out", C.c_void_p), ("iters", C.c_void_p),
-                ("cfl", C.c_double), ("dt_out", C.c_void_p)]
+                ("cfl", C.c_double), ("dt_out", C.c_void_p),
+                ("base_vars", C.POINTER(BlockVars)), ("out_vars", C.POINTER(BlockVars))]
 
 
 class FixupParams(C.Structure):

@@ -23,8 +23,11 @@ def mgr(derefine_poles=True, h=0.3, xt=0.82, alpha=14.0, x0=0.0, smooth=0.5, hex
 def fluid_params(gamma=5.0 / 3.0, cv=1.0, mhd=True, recon=A.RECON_WENO5Z,
                  riemann=A.RIEMANN_HLL, c2p_tol=1e-10, c2p_max_iter=100,
                  geom=A.GEOM_MINKOWSKI, a=0.0, floor_scale_fac=1.0, fail_on_floors=False,
-                 fail_on_ceilings=False, floors=None, ceilings=None, mgr_params=None):
-    """FluidParams with floors and ceilings disabled unless given (fixup.hpp defaults)."""
+                 fail_on_ceilings=False, floors=None, ceilings=None, mgr_params=None,
+                 flux_arith=A.FLUX_ARITH_EXACT):
+    """FluidParams with floors and ceilings disabled unless given (fixup.hpp defaults);
+    flux_arith selects the arithmetic of the flux sweeps (exact: the reference's
+    rounding; fast: contraction and Newton-refined reciprocals, about 1e-15 relative)."""
     b = A.Bounds(A.FLOOR_CONSTANT_RHO_SIE, -A.DBL_MAX, -A.DBL_MAX, 0.0, 0.0, 0.0, 0.0,
                  A.DBL_MAX, A.DBL_MAX)
     if floors:
@@ -34,7 +37,8 @@ def fluid_params(gamma=5.0 / 3.0, cv=1.0, mhd=True, recon=A.RECON_WENO5Z,
         b.gam_max, b.sie_max = ceilings
     return A.FluidParams(gamma, cv, int(mhd), recon, riemann, c2p_tol, c2p_max_iter,
                          floor_scale_fac, int(fail_on_floors), int(fail_on_ceilings), b,
-                         A.Geometry(geom, a, mgr_params if mgr_params is not None else mgr()))
+                         A.Geometry(geom, a, mgr_params if mgr_params is not None else mgr()),
+                         flux_arith)
 
 
 def fixup_params(c2p_fixup=True, strategy=A.FAILURE_INTERPOLATE, floors=True, mhd_floors=False,
@@ -89,10 +93,14 @@ class Phx:
         return rc
 
     @staticmethod
-    def fields(**kw):
+    def fields(vars=None, **kw):
+        """BlockFields of device arrays; vars: a BlockVars table of per-component pointers
+        (kept alive by the caller)."""
         f = A.BlockFields()
         for k, v in kw.items():
             setattr(f, k, _p(v))
+        if vars is not None:
+            f.vars = C.pointer(vars)
         return f
 
     # ---- per-block tasks -------------------------------------------------------------------
@@ -144,8 +152,14 @@ class Phx:
                          "phx_estimate_dt")
 
     def fluid_stage(self, p, b, f, beta, dt, u_base, u_out, prim_out, iters=None, cfl=0.4,
-                    dt_out=None, stream=None):
+                    dt_out=None, stream=None, base_vars=None, out_vars=None):
+        """base_vars / out_vars: BlockVars tables of U^n and of (U_{s+1}, prim_out) for
+        separately allocated components (u_base / u_out / prim_out may then be None)."""
         a = A.StageArgs(beta, dt, _p(u_base), _p(u_out), _p(prim_out), _p(iters), cfl, _p(dt_out))
+        if base_vars is not None:
+            a.base_vars = C.pointer(base_vars)
+        if out_vars is not None:
+            a.out_vars = C.pointer(out_vars)
         return self._chk(self.L.phx_fluid_stage(self.h, C.byref(p), C.byref(b), C.byref(f),
                                                 C.byref(a), stream), "phx_fluid_stage")
 

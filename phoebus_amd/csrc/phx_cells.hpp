@@ -121,10 +121,9 @@ __device__ __forceinline__ double eos_bmod_rho_sie(const DevParams &p, double rh
 
 // con2prim of cell c = (i, j, k) of block b, in place on cons / prim / temp / mu / csig;
 // writes the fail flag and the iteration count
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, class CP, class PP>
 __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeom &b, int i,
-                                              int j, int k, double *__restrict__ cons,
-                                              double *__restrict__ prim,
+                                              int j, int k, const CP &cons, const PP &prim,
                                               double *__restrict__ temp,
                                               double *__restrict__ mu,
                                               double *__restrict__ csig,
@@ -135,7 +134,7 @@ __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeo
   const long n = b.ncell;
   double cs[L::ncons], sig[3];
 #pragma unroll
-  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
+  for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v][c];
   // zones rejected up front by the NaN test keep every field (con2prim_robust.hpp:303-304)
   if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
     fail[c] = 0.0;
@@ -146,7 +145,7 @@ __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeo
   int it = 0;
   bool solved;
   const double st = con2prim_fields<MHD, GEOM, CACHED>(p, b, (long)j * b.ni + i, X, cs, prim,
-                                                       temp, mu, cons, n, c, sig, it, solved);
+                                                       temp, mu, cons, c, sig, it, solved);
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (solved && d < b.ndim) csig[d * n + c] = sig[d];
@@ -155,11 +154,9 @@ __device__ __forceinline__ void con2prim_cell(const DevParams &p, const BlockGeo
 }
 
 // p2c of cell (i, j, k): cons (with B) and, when csig is given, the cell signal speeds
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, class PP, class CP>
 __device__ __forceinline__ void prim_to_con_cell(const DevParams &p, const BlockGeom &b, int i,
-                                                 int j, int k,
-                                                 const double *__restrict__ prim,
-                                                 double *__restrict__ cons,
+                                                 int j, int k, const PP &prim, const CP &cons,
                                                  double *__restrict__ csig) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
@@ -169,20 +166,20 @@ __device__ __forceinline__ void prim_to_con_cell(const DevParams &p, const Block
       metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, xc(b, 0, i), xc(b, 1, j), xc(b, 2, k));
   double pr[L::nprim];
 #pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v][c];
   const double vp[3] = {pr[1], pr[2], pr[3]};
   const double bp[3] = {MHD ? pr[5] : 0.0, MHD ? pr[6] : 0.0, MHD ? pr[7] : 0.0};
   double D, S[3], bc[3], tau, sig[3];
   p2c<MHD>(pr[0], vp, bp, pr[4], pr[L::prs], pr[L::gm1], g, D, S, bc, tau, sig);
-  cons[c] = D;
-  cons[n + c] = S[0];
-  cons[2 * n + c] = S[1];
-  cons[3 * n + c] = S[2];
-  cons[4 * n + c] = tau;
+  cons[0][c] = D;
+  cons[1][c] = S[0];
+  cons[2][c] = S[1];
+  cons[3][c] = S[2];
+  cons[4][c] = tau;
   if constexpr (MHD) {
-    cons[5 * n + c] = bc[0];
-    cons[6 * n + c] = bc[1];
-    cons[7 * n + c] = bc[2];
+    cons[5][c] = bc[0];
+    cons[6][c] = bc[1];
+    cons[7][c] = bc[2];
   }
   if (csig)
     for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];

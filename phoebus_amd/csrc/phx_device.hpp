@@ -53,6 +53,55 @@ struct BlockGeom {
   const double *geo;
 };
 
+// ---- per-component field pointers ------------------------------------------------------
+// A multi-component field as the kernels see it: component v's [nk][nj][ni] array starts
+// at p[v].  Both the library's contiguous fields ([nv][cells]) and a host's separately
+// allocated variables (parthenon allocates every variable and every direction of its flux
+// on its own; phx_block_vars) become this on the host, and the kernels index p[v][c], so
+// neither layout is copied.  Flux tables hold flux[d][m] at p[d * 8 + m].
+template <int N>
+struct Comp {
+  double *p[N];
+  __host__ __device__ __forceinline__ double *operator[](int v) const { return p[v]; }
+  // component v of a contiguous field of nv components at base (null beyond nv)
+  static __host__ __device__ __forceinline__ Comp contiguous(const double *base, long ncell,
+                                                            int nv = N) {
+    Comp c;
+#pragma unroll
+    for (int v = 0; v < N; ++v)
+      c.p[v] = base && v < nv ? const_cast<double *>(base) + (long)v * ncell : nullptr;
+    return c;
+  }
+  // flux[d][m] of a contiguous [ndim][nc][cells] flux field into the d * 8 + m slots
+  static __host__ __device__ __forceinline__ Comp flux(const double *base, long ncell, int nc) {
+    static_assert(N == 24, "flux tables have 3 x 8 slots");
+    Comp c;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        c.p[d * 8 + m] =
+            base && m < nc ? const_cast<double *>(base) + ((long)d * nc + m) * ncell : nullptr;
+    return c;
+  }
+};
+using PrimPtrs = Comp<10>;
+using ConsPtrs = Comp<8>;
+using FluxPtrs = Comp<24>;
+// The same view of a field the library laid out contiguously ([nv][cells] at base; a flux
+// [3][8][cells] is one too, slot d * 8 + m): component v at base + v * n, computed where it
+// is used.  The kernels over packs and meshes, whose blocks are always contiguous, take this
+// form (their code is then what the contiguous layout compiles to); the per-block entry
+// points take Comp tables.
+struct Contig {
+  double *base;
+  long n;
+  __device__ __forceinline__ double *operator[](int v) const { return base + (long)v * n; }
+};
+__device__ __forceinline__ Contig contig(const double *base, long n) {
+  return Contig{const_cast<double *>(base), n};
+}
+
 // The hardware hands consecutive workgroups of a launch to the 8 XCDs in turn, and each
 // XCD has its own L2.  Renumbering so that each XCD receives a contiguous range of
 // logical blocks keeps neighbouring tiles (which share cache lines at their edges and
@@ -83,6 +132,66 @@ __device__ __forceinline__ double ratio(double a, double b) {
 __device__ __forceinline__ double make_positive(double v) { return smax(v, kEps); }
 __device__ __forceinline__ double make_bounded(double v, double lo, double hi) {
   return smin(smax(v, lo + kEps), hi * (1.0 - kEps));
+}
+
+// ---- arithmetic of the face states and fluxes ------------------------------------------
+// The reconstruction, prim_to_flux and the Riemann solvers divide and take square roots
+// through these.  By default they are IEEE division and sqrt, which round exactly as the
+// reference.  A translation unit that defines PHX_FLUX_FAST as 1 (the sweeps of the fast
+// flux arithmetic, phx_flux_fast.hip, also built with contraction on) gets reciprocals and
+// reciprocal square roots from the hardware estimates (v_rcp_f64, v_rsq_f64) refined by
+// Newton-Raphson steps (Markstein, IBM J. Res. Develop. 34 (1990) 111-119), with the
+// quotient formed as a times the reciprocal: within a few ulp of the IEEE results for
+// finite positive denominators and radicands, which is what these code paths divide by
+// (the WENO weights are bounded below by their 1e-100 epsilon, Lorentz factors by 1,
+// signal-speed radicands by make_positive's EPS), in about half the instructions.  A zero
+// or infinite denominator gives NaN where IEEE division gives +-inf.
+#ifndef PHX_FLUX_FAST
+#define PHX_FLUX_FAST 0
+#endif
+constexpr bool kFluxFast = PHX_FLUX_FAST != 0;
+
+__device__ __forceinline__ double frcp(double b) {
+  if constexpr (kFluxFast) {
+    // r <- r + r (1 - b r): each step squares the relative error of the estimate
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return fma(fma(-b, r, 1.0), r, r);
+  } else {
+    return 1.0 / b;
+  }
+}
+__device__ __forceinline__ double fdiv(double a, double b) {
+  if constexpr (kFluxFast)
+    return a * frcp(b);
+  else
+    return a / b;
+}
+// 1 / sqrt(x): y <- y + y (1 - x y^2) / 2, twice
+__device__ __forceinline__ double frsqrt(double x) {
+  if constexpr (kFluxFast) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  } else {
+    return 1.0 / sqrt(x);
+  }
+}
+// sqrt(x): one reciprocal-square-root step, s = x y, then s <- s + y (x - s^2) / 2
+__device__ __forceinline__ double fsqrt(double x) {
+  if constexpr (kFluxFast) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    const double s = x * y;
+    return fma(0.5 * y, fma(-s, s, x), s);
+  } else {
+    return sqrt(x);
+  }
+}
+// robust::ratio with the flux arithmetic
+__device__ __forceinline__ double fratio(double a, double b) {
+  const double sgn = b >= 0 ? 1.0 : -1.0;
+  return fdiv(a, b + sgn * kSmall);
 }
 
 // ---- reconstruction.hpp:101-244 -------------------------------------------------------
@@ -119,31 +228,31 @@ __device__ __forceinline__ void recon_weno5z(double q0, double q1, double q2, do
   b = q4 - 4.0 * q3 + 3.0 * q2;
   double beta2 = tt * a * a + b * b + eps;
   const double tau5 = fabs(beta2 - beta0);
-  beta0 = (beta0 + tau5) / beta0;
-  beta1 = (beta1 + tau5) / beta1;
-  beta2 = (beta2 + tau5) / beta2;
+  beta0 = fdiv(beta0 + tau5, beta0);
+  beta1 = fdiv(beta1 + tau5, beta1);
+  beta2 = fdiv(beta2 + tau5, beta2);
 
   double w0 = g0 * beta0 + eps, w1 = g1 * beta1 + eps, w2 = g2 * beta2 + eps;
-  double wsum = 1.0 / (w0 + w1 + w2);
+  double wsum = frcp(w0 + w1 + w2);
   double l = w0 * (c00 * q0 + c01 * q1 + c02 * q2);
   l += w1 * (c10 * q1 + c11 * q2 + c12 * q3);
   l += w2 * (c20 * q2 + c21 * q3 + c22 * q4);
   l *= wsum;
-  const double al = 3.0 * wsum * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
+  const double al = fdiv(3.0 * wsum * w0 * w1 * w2, g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
 
   w0 = g0 * beta2 + eps;
   w1 = g1 * beta1 + eps;
   w2 = g2 * beta0 + eps;
-  wsum = 1.0 / (w0 + w1 + w2);
+  wsum = frcp(w0 + w1 + w2);
   double r = w0 * (c00 * q4 + c01 * q3 + c02 * q2);
   r += w1 * (c10 * q3 + c11 * q2 + c12 * q1);
   r += w2 * (c20 * q2 + c21 * q1 + c22 * q0);
   r *= wsum;
-  const double ar = 3.0 * wsum * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
+  const double ar = fdiv(3.0 * wsum * w0 * w1 * w2, g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
 
   double dq = q3 - q2;
   dq = mc_limiter(q2 - q1, dq, 2.0);
-  const double alin = 2.0 * al * ar / (al + ar);
+  const double alin = fdiv(2.0 * al * ar, al + ar);
   ql = alin * l + (1.0 - alin) * (q2 + 0.5 * dq);
   qr = alin * r + (1.0 - alin) * (q2 - 0.5 * dq);
 }
@@ -156,7 +265,7 @@ __device__ __forceinline__ double mp5_edge(double fm2, double fm1, double f0, do
                                            double fp2) {
   constexpr double alpha = 4.0, epsm = 1.e-12;
   double f = 2.0 * fm2 - 13.0 * fm1 + 47.0 * f0 + 27.0 * fp1 - 3.0 * fp2;
-  f /= 60.0;
+  f = kFluxFast ? f * (1.0 / 60.0) : f / 60.0;
   const double fmp = f0 + minmod2(fp1 - f0, alpha * (f0 - fm1));
   if ((f - f0) * (f - fmp) <= epsm) return f;
   const double d2m = fm2 + f0 - 2.0 * fm1;
@@ -493,13 +602,13 @@ __device__ __forceinline__ void prim_to_flux(const DevParams &p, const int dir, 
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
       if (M::nz_cov(ii + 1, jj + 1)) vsq0 += g.gcov(ii + 1, jj + 1) * vpcon[ii] * vpcon[jj];
-  double W = sqrt(1. + vsq0);
+  double W = fsqrt(1. + vsq0);
   if (W > gam_max) {
-    const double rescale = sqrt((gam_max * gam_max - 1.) / (W * W - 1.));
+    const double rescale = fsqrt(fdiv(gam_max * gam_max - 1., W * W - 1.));
 #pragma unroll
     for (int ii = 0; ii < 3; ++ii) vpcon[ii] *= rescale;
   }
-  double vcon[3] = {vpcon[0] / W, vpcon[1] / W, vpcon[2] / W};
+  double vcon[3] = {fdiv(vpcon[0], W), fdiv(vpcon[1], W), fdiv(vpcon[2], W)};
   const double vel = vcon[dir];
   double Bcon[3] = {0.0, 0.0, 0.0};
   const double u = (q[4] / rho > sie_floor ? (q[4] / rho > sie_max ? rho * sie_max : q[4])
@@ -525,58 +634,58 @@ __device__ __forceinline__ void prim_to_flux(const DevParams &p, const int dir, 
     }
   const double vsq_max = 1.0 - 1.0 / (gam_max * gam_max);
   if (vsq > vsq_max) {
-    double scale = vsq_max / vsq;
+    double scale = fdiv(vsq_max, vsq);
     vsq *= scale;
-    scale = sqrt(scale);
+    scale = fsqrt(scale);
     Bdotv *= scale;
 #pragma unroll
     for (int m = 0; m < 3; m++) vcon[m] *= scale;
   }
-  W = 1.0 / sqrt(1 - vsq);
+  W = frsqrt(1 - vsq);
   const double ialpha_beta_dir = M::nz_beta(0) || M::nz_beta(1) || M::nz_beta(2)
-                                     ? g.beta(dir) / g.alpha() : 0.0;
+                                     ? fdiv(g.beta(dir), g.alpha()) : 0.0;
   const double vtil = vel - ialpha_beta_dir;
 
   U[0] = rho * W;
   F[0] = U[0] * vtil;
   const double b0 = W * Bdotv;
-  const double bsq = (BdotB + b0 * b0) / (W * W);
+  const double bsq = fdiv(BdotB + b0 * b0, W * W);
   const double rhohWsq = (rho + u + P) * W * W;
   const double bsqWsq = bsq * W * W;
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     double bcovm = 0.0;
-    if (MHD && M::nz_cov(m + 1, 0)) bcovm = g.gcov(m + 1, 0) * b0 / g.alpha();
+    if (MHD && M::nz_cov(m + 1, 0)) bcovm = fdiv(g.gcov(m + 1, 0) * b0, g.alpha());
     double vcovm = 0.0;
 #pragma unroll
     for (int n = 1; n < 4; n++) {
       if (!M::nz_cov(m + 1, n)) continue;
       if constexpr (MHD) {
-        const double sh = M::nz_beta(n - 1) ? g.beta(n - 1) / g.alpha() : 0.0;
-        bcovm += g.gcov(m + 1, n) * (Bcon[n - 1] / W + b0 * (vcon[n - 1] - sh));
+        const double sh = M::nz_beta(n - 1) ? fdiv(g.beta(n - 1), g.alpha()) : 0.0;
+        bcovm += g.gcov(m + 1, n) * (fdiv(Bcon[n - 1], W) + b0 * (vcon[n - 1] - sh));
       }
       vcovm += g.gcov(m + 1, n) * vcon[n - 1];
     }
     U[1 + m] = (rhohWsq + bsqWsq) * vcovm - b0 * bcovm;
-    F[1 + m] = U[1 + m] * vtil + (P + 0.5 * bsq) * (double)(dir == m) - bcovm * Bcon[dir] / W;
+    F[1 + m] = U[1 + m] * vtil + (P + 0.5 * bsq) * (double)(dir == m) - fdiv(bcovm * Bcon[dir], W);
   }
   U[4] = rhohWsq + bsqWsq - (P + 0.5 * bsq) - b0 * b0 - U[0];
   F[4] = U[4] * vtil + (P + 0.5 * bsq) * vel - Bdotv * Bcon[dir];
   if constexpr (MHD) {
 #pragma unroll
     for (int m = 5; m <= 7; m++) {
-      const double sh = M::nz_beta(m - 5) ? g.beta(m - 5) / g.alpha() : 0.0;
+      const double sh = M::nz_beta(m - 5) ? fdiv(g.beta(m - 5), g.alpha()) : 0.0;
       U[m] = Bcon[m - 5];
       F[m] = U[m] * vtil - Bcon[dir] * (vcon[m - 5] - sh);
     }
   }
-  const double vasq = bsqWsq / make_positive(rhohWsq + bsqWsq);
-  const double cssq = make_bounded(gamma1 * P * W * W / make_positive(rhohWsq), 0.0, 1.0);
+  const double vasq = fdiv(bsqWsq, make_positive(rhohWsq + bsqWsq));
+  const double cssq = make_bounded(fdiv(gamma1 * P * W * W, make_positive(rhohWsq)), 0.0, 1.0);
   const double cmsq = make_bounded(cssq + vasq * (1.0 - cssq), 0.0, 1.0);
-  const double vcoff = g.alpha() / (1.0 - vsq * cmsq);
+  const double vcoff = fdiv(g.alpha(), 1.0 - vsq * cmsq);
   const double v0 = vel * (1.0 - cmsq);
   const double vpm =
-      sqrt(make_positive(cmsq * (1.0 - vsq) * (g.gcon(dir, dir) * (1.0 - vsq * cmsq) - vel * v0)));
+      fsqrt(make_positive(cmsq * (1.0 - vsq) * (g.gcon(dir, dir) * (1.0 - vsq * cmsq) - vel * v0)));
   const double bd = M::nz_beta(dir) ? g.beta(dir) : 0.0;
   vp = vcoff * (v0 + vpm) - bd;
   vm = vcoff * (v0 - vpm) - bd;
@@ -602,7 +711,7 @@ __device__ __forceinline__ double riemann_flux(const DevParams &p, int dir, cons
     const double cl = smin(smin(vml, vmr), 0.0);
     const double cr = smax(smax(vpl, vpr), 0.0);
     const double crcl = cr * cl;
-    const double inv = ratio(1.0, cr - cl);
+    const double inv = fratio(1.0, cr - cl);
 #pragma unroll
     for (int m = 0; m < NC; m++)
       flux[m] = ((cr * Fl[m] - cl * Fr[m]) * g.detg() + crcl * (Ur[m] - Ul[m]) * g.detgamma()) * inv;
@@ -1016,7 +1125,7 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
   return c2p_unwrap<MHD>(p, g, z, mu, cons, prim, temp, sig);
 }
 
-// con2prim_zone on a zone's fields in memory ([var][n] planes, cell c; the zone-centre
+// con2prim_zone on a zone's fields in memory (component planes, cell c; the zone-centre
 // metric of geometry-plane point q at X), for conserved variables cs that passed the NaN
 // test: it writes the magnetic primitives of the set-up, then, when the root is bracketed,
 // the other primitives, the temperature and the root, and the conserved variables only
@@ -1025,13 +1134,13 @@ __device__ __forceinline__ double con2prim_zone(const DevParams &p, const M &g,
 // guess).  With CACHED the metric is read from the block's cached geometry for the set-up
 // and again for the unwrap, so it is not held in registers across the solve.  solved tells
 // whether sig holds new signal speeds; iters is -1 for an unbracketed root.
-template <bool MHD, int GEOM, bool CACHED>
+template <bool MHD, int GEOM, bool CACHED, class PP, class CP>
 __device__ __forceinline__ double con2prim_fields(const DevParams &p, const BlockGeom &b, long q,
                                                   const double X[3], double *cs,
-                                                  double *__restrict__ prim,
+                                                  const PP &prim,
                                                   double *__restrict__ temp,
                                                   double *__restrict__ mu,
-                                                  double *__restrict__ cons, long n, long c,
+                                                  const CP &cons, long c,
                                                   double sig[3], int &iters, bool &solved) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
@@ -1042,7 +1151,7 @@ __device__ __forceinline__ double con2prim_fields(const DevParams &p, const Bloc
   c2p_setup<MHD>(p, g, X, cs, pr, z);
   if constexpr (MHD) {
 #pragma unroll
-    for (int i = 5; i < 8; ++i) prim[i * n + c] = pr[i];
+    for (int i = 5; i < 8; ++i) prim[i][c] = pr[i];
   }
   double root;
   solved = regula_falsi(z.res, 0.0, 1.0, p.tol, mu[c], p.max_iter, root, iters);
@@ -1061,12 +1170,12 @@ __device__ __forceinline__ double con2prim_fields(const DevParams &p, const Bloc
   }
 #pragma unroll
   for (int v = 0; v < L::nprim; ++v)
-    if (!MHD || v < 5 || v > 7) prim[v * n + c] = pr[v];
+    if (!MHD || v < 5 || v > 7) prim[v][c] = pr[v];
   if (temp) temp[c] = tt;
   if (z.res.used_density_floor || z.res.used_energy_floor || z.res.used_energy_max ||
       z.res.used_gamma_max) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) cons[v * n + c] = cs[v];
+    for (int v = 0; v < 5; ++v) cons[v][c] = cs[v];
   }
   return st;
 }

@@ -17,11 +17,11 @@ namespace phx {
 // ConservedToPrimitiveFixupImpl (fixup_c2p.cpp:175-312), fluid part, for interior cell
 // (i, j, k).  A failed neighbour inside the interior is being rewritten by this same pass:
 // its weight (its fail flag) is 0 and it is skipped rather than multiplied.
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, class PP, class CP>
 __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixup &fx,
                                                const BlockGeom &b, int i, int j, int k,
-                                               double *prim, double *cons, double *temp,
-                                               double *csig, const double *fail) {
+                                               const PP &prim, const CP &cons,
+                                               double *temp, double *csig, const double *fail) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
   const long n = b.ncell;
@@ -48,14 +48,14 @@ __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixu
   if (b.ndim == 3) num_valid += w[4] + w[5];
   double pr[L::nprim];
 #pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v][c];
   if (num_valid > 0.5 && fx.strategy == PHX_FAILURE_INTERPOLATE) {  // :225-243
     const double norm = 1.0 / num_valid;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {  // rho, v^1..3, u
       double t[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) t[q] = skip[q] ? 0.0 : w[q] * prim[m * n + nb[q]];
+      for (int q = 0; q < 6; ++q) t[q] = skip[q] ? 0.0 : w[q] * prim[m][nb[q]];
       double acc = t[0] + t[1];
       if (b.ndim > 1) {
         acc += t[2] + t[3];
@@ -95,12 +95,12 @@ __device__ __forceinline__ void c2p_fixup_cell(const DevParams &p, const DevFixu
   double D, S[3], bc[3], tau, sig[3];
   p2c<MHD>(pr[0], vpcon, bp, pr[4], pr[L::prs], pr[L::gm1], g, D, S, bc, tau, sig);
 #pragma unroll
-  for (int v = 0; v < L::nprim; ++v) prim[v * n + c] = pr[v];
+  for (int v = 0; v < L::nprim; ++v) prim[v][c] = pr[v];
   if (temp) temp[c] = t;
-  cons[c] = D;
+  cons[0][c] = D;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) cons[(1 + m) * n + c] = S[m];
-  cons[4 * n + c] = tau;
+  for (int m = 0; m < 3; ++m) cons[1 + m][c] = S[m];
+  cons[4][c] = tau;
   for (int d = 0; d < b.ndim; ++d) csig[d * n + c] = sig[d];
 }
 
@@ -139,10 +139,10 @@ __device__ __forceinline__ void floor_targets(const DevParams &p, const DevFixup
 
 // whether floors_cell changes cell (i, j, k): a floor (:434) or a ceiling (:501-507) acts.
 // It reads only the primitives floors_cell decides on and holds none of its repair state.
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, class PP>
 __device__ __forceinline__ bool floors_needed(const DevParams &p, const DevFixup &fx,
                                               const BlockGeom &b, int i, int j, int k,
-                                              const double *prim) {
+                                              const PP &prim) {
   using M = typename MetricOf<GEOM>::type;
   const long n = b.ncell;
   const long c = ((long)k * b.nj + j) * b.ni + i;
@@ -152,7 +152,7 @@ __device__ __forceinline__ bool floors_needed(const DevParams &p, const DevFixup
   const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
   double pr[8];
 #pragma unroll
-  for (int v = 0; v < (MHD ? 8 : 5); ++v) pr[v] = prim[v * n + c];
+  for (int v = 0; v < (MHD ? 8 : 5); ++v) pr[v] = prim[v][c];
   double rho_floor_max, u_floor_max;
   floor_targets<MHD>(p, fx, g, pr, rho_floor, sie_floor, rho_floor_max, u_floor_max);
   if (rho_floor_max - pr[0] > 0. || u_floor_max - pr[4] > 0.) return true;
@@ -161,11 +161,11 @@ __device__ __forceinline__ bool floors_needed(const DevParams &p, const DevFixup
 }
 
 // ApplyFloorsImpl (fixup.cpp:379-539), fluid part, for cell (i, j, k)
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, class PP, class CP>
 __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &fx,
                                             const BlockGeom &b, int i, int j, int k,
-                                            double *prim, double *cons, double *temp,
-                                            double *mu, double *csig) {
+                                            const PP &prim, const CP &cons,
+                                            double *temp, double *mu, double *csig) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
   const long n = b.ncell;
@@ -177,7 +177,7 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
   const M g = metric_at<GEOM, CACHED>(p, b, 0, (long)j * b.ni + i, X[0], X[1], X[2]);
   double pr[L::nprim];
 #pragma unroll
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v][c];
   double rho_floor_max, u_floor_max;
   floor_targets<MHD>(p, fx, g, pr, rho_floor, sie_floor, rho_floor_max, u_floor_max);
   double drho = rho_floor_max - pr[0];
@@ -197,7 +197,7 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
   double t = temp ? temp[c] : 0.0;
   if (floor_applied) {  // :448-498
 #pragma unroll
-    for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
+    for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v][c];
     const double vzero[3] = {0.0, 0.0, 0.0};
     const double dprs = eos_p_rho_sie(p, drho, ratio(du, drho));
     const double dgm1 = ratio(eos_bmod_rho_sie(p, drho, ratio(du, drho)), dprs);
@@ -234,7 +234,7 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
   if (ceiling_applied) {  // :509-526
     if (!floor_applied) {
 #pragma unroll
-      for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v * n + c];
+      for (int v = 0; v < L::ncons; ++v) cs[v] = cons[v][c];
     }
     const double rescale = sqrt(gamma_max * gamma_max - 1.) / (W * W - 1.);
 #pragma unroll
@@ -253,9 +253,9 @@ __device__ __forceinline__ void floors_cell(const DevParams &p, const DevFixup &
     pr[L::prs] = eos_p_rho_t(p, pr[0], t);
     pr[L::gm1] = ratio(eos_bmod_rho_t(p, pr[0], t), pr[L::prs]);
 #pragma unroll
-    for (int v = 0; v < L::nprim; ++v) prim[v * n + c] = pr[v];
+    for (int v = 0; v < L::nprim; ++v) prim[v][c] = pr[v];
 #pragma unroll
-    for (int v = 0; v < L::ncons; ++v) cons[v * n + c] = cs[v];
+    for (int v = 0; v < L::ncons; ++v) cons[v][c] = cs[v];
     if (temp) temp[c] = t;
   }
 }
@@ -273,7 +273,7 @@ __device__ __forceinline__ bool interior_cell(const BlockGeom &b, long t, int &i
 
 template <bool MHD, int GEOM>
 __global__ __launch_bounds__(256) void k_c2p_fixup(DevParams p, DevFixup fx, BlockGeom b,
-                                                   double *prim, double *cons, double *temp,
+                                                   PrimPtrs prim, ConsPtrs cons, double *temp,
                                                    double *csig, const double *fail) {
   int i, j, k;
   if (!interior_cell(b, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void k_c2p_fixup(DevParams p, DevFixup fx, Blo
 
 template <bool MHD, int GEOM>
 __global__ __launch_bounds__(256) void k_apply_floors(DevParams p, DevFixup fx, BlockGeom b,
-                                                      Range3 rg, double *prim, double *cons,
+                                                      Range3 rg, PrimPtrs prim, ConsPtrs cons,
                                                       double *temp, double *mu, double *csig) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rg.n[0] * rg.n[1] * rg.n[2]) return;
@@ -300,8 +300,9 @@ __global__ __launch_bounds__(256) void k_pack_c2p_fixup(DevParams p, DevFixup fx
   const PackEntry &E = v.e[b];
   int i, j, k;
   if (!interior_cell(E.bg, (g - v.start[b]) * 256 + threadIdx.x, i, j, k)) return;
-  c2p_fixup_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, E.bg, i, j, k, E.prim_out, E.u_out,
-                                                    E.temp, E.csig, E.fail);
+  c2p_fixup_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(
+      p, fx, E.bg, i, j, k, contig(E.prim_out, E.bg.ncell), contig(E.u_out, E.bg.ncell),
+      E.temp, E.csig, E.fail);
 }
 
 // the CFL time of interior cell (i, j, k) with face speeds (fluid.cpp:913-928)
@@ -346,8 +347,8 @@ __global__ __launch_bounds__(256) void k_pack_floors(DevParams p, DevFixup fx, P
     cell = inner = interior_cell(bg, t, i, j, k);
   }
   const bool defer = cell && q.cell &&
-                     floors_needed<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, bg, i, j, k,
-                                                                        E.prim_out);
+                     floors_needed<true, GEOM, GEOM == PHX_GEOM_FMKS>(
+                         p, fx, bg, i, j, k, contig(E.prim_out, bg.ncell));
   // one queue slot range per wave
   const unsigned long long mask = __ballot(defer);
   if (mask) {
@@ -383,8 +384,9 @@ __global__ __launch_bounds__(256) void k_pack_floors_apply(DevParams p, DevFixup
     const long c = (long)(w & 0xffffffffull);
     const int i = (int)(c % bg.ni), j = (int)((c / bg.ni) % bg.nj);
     const int k = (int)(c / ((long)bg.ni * bg.nj));
-    floors_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, fx, bg, i, j, k, E.prim_out, E.u_out,
-                                                   E.temp, E.mu, E.csig);
+    floors_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(
+        p, fx, bg, i, j, k, contig(E.prim_out, bg.ncell), contig(E.u_out, bg.ncell), E.temp,
+        E.mu, E.csig);
     const int gg = q.grp[e];
     if (partial && gg >= 0)
       atomicMin(reinterpret_cast<unsigned long long *>(partial + gg),
@@ -455,19 +457,23 @@ extern "C" int phx_c2p_fixup(phx_ctx *ctx, const phx_fluid_params *p,
   if (rc) return rc;
   if ((rc = check_fixup(ctx, fx))) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
-  if (!f || !f->prim || !f->cons || !f->fail || !f->cell_sig)
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const ConsPtrs cons = cons_ptrs(f, bg.ncell, nc);
+  if (!complete(prim, np) || !complete(cons, nc) || !f->fail || !f->cell_sig)
     return ctx->fail(PHX_ERR_INVALID, "null field");
   const DevFixup dfx = make_dev_fixup(p, fx);
   if (!dfx.c2p_fixup) return PHX_OK;
   const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
   const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {
       hipLaunchKernelGGL((k_c2p_fixup<decltype(H)::value, decltype(G)::value>),
                          dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dp, dfx, bg,
-                         f->prim, f->cons, f->temp, f->cell_sig, f->fail);
+                         prim, cons, f->temp, f->cell_sig, f->fail);
     });
   });
   return ctx->check_launch();
@@ -482,21 +488,25 @@ extern "C" int phx_apply_floors(phx_ctx *ctx, const phx_fluid_params *p,
   if (rc) return rc;
   if ((rc = check_fixup(ctx, fx))) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
-  if (!f || !f->prim || !f->cons || !f->c2p_mu || !f->cell_sig)
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const ConsPtrs cons = cons_ptrs(f, bg.ncell, nc);
+  if (!complete(prim, np) || !complete(cons, nc) || !f->c2p_mu || !f->cell_sig)
     return ctx->fail(PHX_ERR_INVALID, "null field");
   Range3 rg;
   if ((rc = make_range(ctx, b, lo, hi, rg))) return rc;
   const DevFixup dfx = make_dev_fixup(p, fx);
   if (!dfx.floors) return PHX_OK;
   const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
   const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {
       hipLaunchKernelGGL((k_apply_floors<decltype(H)::value, decltype(G)::value>),
                          dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dp, dfx, bg, rg,
-                         f->prim, f->cons, f->temp, f->c2p_mu, f->cell_sig);
+                         prim, cons, f->temp, f->c2p_mu, f->cell_sig);
     });
   });
   return ctx->check_launch();

@@ -67,14 +67,57 @@ struct TileShape {
 };
 
 // flux launches (phx_flux.hip): direction d of fluid.cpp:753-770 with the fused
-// reconstruction; writes flux[d][m][cells] and fsig[d][cells]
+// reconstruction; writes flux[d][m][cells] and fsig[d][cells].  fast selects the build of
+// the sweeps with the fast flux arithmetic (phx_fluid_params::flux_arith).
 void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevParams &dp,
-                   const BlockGeom &bg, const double *prim, double *flux, double *fsig,
-                   hipStream_t s, TileShape shape = TileShape());
+                   const BlockGeom &bg, const PrimPtrs &prim, const FluxPtrs &flux, double *fsig,
+                   hipStream_t s, TileShape shape, bool fast);
+inline bool flux_fast_of(const phx_fluid_params *p) { return p->flux_arith == PHX_FLUX_ARITH_FAST; }
 // the shape launch_fluxes uses for direction d (after defaults and limits)
 TileShape resolve_tile_shape(int d, int recon, int mhd, const BlockGeom &bg, TileShape want);
 
 DevParams make_dev_params(const phx_fluid_params *p);
+
+// ---- per-component pointers of a call's fields -----------------------------------------
+// component v of a contiguous field at base, unless the table has its own pointer
+template <int N>
+inline Comp<N> comps(const double *base, long ncell, int nv, double *const *table) {
+  Comp<N> c = Comp<N>::contiguous(base, ncell, nv);
+  if (table)
+    for (int v = 0; v < nv && v < N; ++v)
+      if (table[v]) c.p[v] = table[v];
+  return c;
+}
+inline PrimPtrs prim_ptrs(const phx_block_fields *f, long ncell, int nprim) {
+  return comps<10>(f->prim, ncell, nprim, f->vars ? f->vars->prim : nullptr);
+}
+inline ConsPtrs cons_ptrs(const phx_block_fields *f, long ncell, int ncons) {
+  return comps<8>(f->cons, ncell, ncons, f->vars ? f->vars->cons : nullptr);
+}
+inline ConsPtrs src_ptrs(const phx_block_fields *f, long ncell, int ncons) {
+  return comps<8>(f->src, ncell, ncons, f->vars ? f->vars->src : nullptr);
+}
+inline FluxPtrs flux_ptrs(const phx_block_fields *f, long ncell, int ncons) {
+  FluxPtrs c = FluxPtrs::flux(f->flux, ncell, ncons);
+  if (f->vars)
+    for (int d = 0; d < 3; ++d)
+      for (int m = 0; m < ncons; ++m)
+        if (f->vars->flux[d][m]) c.p[d * 8 + m] = f->vars->flux[d][m];
+  return c;
+}
+// every one of the first n components present
+template <int N>
+inline bool complete(const Comp<N> &c, int n) {
+  for (int v = 0; v < n && v < N; ++v)
+    if (!c.p[v]) return false;
+  return true;
+}
+inline bool complete_flux(const FluxPtrs &c, int ndim, int ncons) {
+  for (int d = 0; d < ndim; ++d)
+    for (int m = 0; m < ncons; ++m)
+      if (!c.p[d * 8 + m]) return false;
+  return true;
+}
 
 // a cell range lo..lo+n-1 per dimension (i, j, k)
 struct Range3 {
@@ -93,9 +136,9 @@ int check_fixup(phx_ctx *ctx, const phx_fixup_params *fx);
 int check_params(phx_ctx *ctx, const phx_fluid_params *p);
 int check_block(phx_ctx *ctx, const phx_block *b, int recon);
 
-void launch_flux_ct(const BlockGeom &bg, double *flux, double *emf, hipStream_t s);
+void launch_flux_ct(const BlockGeom &bg, const FluxPtrs &flux, double *emf, hipStream_t s);
 void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
-                    const double *prim, double *src, hipStream_t s);
+                    const PrimPtrs &prim, const ConsPtrs &src, hipStream_t s);
 void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s);
 
 // ---- block packs (a MeshBlockPack-style work list over blocks of different sizes) -------
@@ -131,7 +174,22 @@ __device__ __forceinline__ int pack_find(const long *start, int n, long g) {
 
 // pack launches (phx_flux.hip, phx_kernels.hip, phx_stage.hip); 3-D MHD blocks
 void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
-                      const PackView &v, long groups, TileShape shape, hipStream_t s);
+                      const PackView &v, long groups, TileShape shape, hipStream_t s, bool fast);
+// the two builds of the sweeps (phx_flux_sweeps.hpp in phx_flux.hip and phx_flux_fast.hip)
+namespace flux_exact {
+void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevParams &dp,
+                   const BlockGeom &bg, const PrimPtrs &prim, const FluxPtrs &flux, double *fsig,
+                   hipStream_t s, TileShape sh);
+void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
+                      const PackView &v, long groups, TileShape sh, hipStream_t s);
+}  // namespace flux_exact
+namespace flux_fast {
+void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevParams &dp,
+                   const BlockGeom &bg, const PrimPtrs &prim, const FluxPtrs &flux, double *fsig,
+                   hipStream_t s, TileShape sh);
+void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
+                      const PackView &v, long groups, TileShape sh, hipStream_t s);
+}  // namespace flux_fast
 void launch_pack_ct(const PackView &v, long groups, hipStream_t s);
 void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
                          hipStream_t s);
@@ -187,12 +245,13 @@ struct phx_ctx {
 
   int device = 0;
   hipStream_t stream = nullptr;  // default stream of the context
-  // autotuned y/z flux tile shapes per (recon, riemann, mhd, geom, ndim, nx, ny, nz, ng, d)
-  using TileKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
+  // autotuned y/z flux tile shapes per (recon, riemann, mhd, geom, ndim, nx, ny, nz, ng, d,
+  // flux arithmetic)
+  using TileKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
   std::map<TileKey, phx::TileShape> tiles;
   static TileKey tile_key(const phx_fluid_params *p, const phx_block *b, int d) {
     return TileKey(p->recon, p->riemann, p->mhd, p->geom.kind, b->ndim, b->nx[0], b->nx[1],
-                   b->nx[2], b->ng, d);
+                   b->nx[2], b->ng, d, p->flux_arith);
   }
   phx::TileShape tile_for(const phx_fluid_params *p, const phx_block *b, int d) const {
     const auto it = tiles.find(tile_key(p, b, d));

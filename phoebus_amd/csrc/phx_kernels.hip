@@ -23,9 +23,8 @@ namespace phx {
 
 // ---- FluxCT (fluid.cpp:775-841) --------------------------------------------------------
 // corner EMFs of a 3-D block: thread t of (nx+1)(ny+1)(nz+1)
-template <int NC>
-__device__ __forceinline__ void ct_emf_3d_body(const BlockGeom &b, long t,
-                                               const double *__restrict__ flux,
+template <int NC, class FP>
+__device__ __forceinline__ void ct_emf_3d_body(const BlockGeom &b, long t, const FP &flux,
                                                double *__restrict__ emf) {
   const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 2;
   if (t >= (long)ni * nj * nk) return;
@@ -34,22 +33,21 @@ __device__ __forceinline__ void ct_emf_3d_body(const BlockGeom &b, long t,
   const int k = (int)(t / ((long)ni * nj)) + b.is[2];
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long sj = b.ni, sk = (long)b.ni * b.nj, n = b.ncell;
-  auto F = [&](int d, int comp, long cc) { return flux[((long)d * NC + 5 + comp) * n + cc]; };
+  auto F = [&](int d, int comp, long cc) { return flux[d * 8 + 5 + comp][cc]; };
   emf[c] = 0.25 * (F(1, 2, c) + F(1, 2, c - sk) - F(2, 1, c) - F(2, 1, c - sj));
   emf[n + c] = -0.25 * (F(0, 2, c) + F(0, 2, c - sk) - F(2, 0, c) - F(2, 0, c - 1));
   emf[2 * n + c] = 0.25 * (F(0, 1, c) + F(0, 1, c - sj) - F(1, 0, c) - F(1, 0, c - 1));
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, const double *__restrict__ flux,
+__global__ __launch_bounds__(256) void k_ct_emf_3d(BlockGeom b, FluxPtrs flux,
                                                    double *__restrict__ emf) {
   ct_emf_3d_body<NC>(b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, emf);
 }
 
 // B fluxes from the corner EMFs of a 3-D block: thread t of (nx+1)(ny+1)(nz+1)
-template <int NC>
-__device__ __forceinline__ void ct_flux_3d_body(const BlockGeom &b, long t,
-                                                double *__restrict__ flux,
+template <int NC, class FP>
+__device__ __forceinline__ void ct_flux_3d_body(const BlockGeom &b, long t, const FP &flux,
                                                 const double *__restrict__ emf) {
   const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 2;
   if (t >= (long)ni * nj * nk) return;
@@ -59,7 +57,7 @@ __device__ __forceinline__ void ct_flux_3d_body(const BlockGeom &b, long t,
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const long sj = b.ni, sk = (long)b.ni * b.nj, n = b.ncell;
   const double *e0 = emf, *e1 = emf + n, *e2 = emf + 2 * n;
-  auto F = [&](int d, int comp) -> double & { return flux[((long)d * NC + 5 + comp) * n + c]; };
+  auto F = [&](int d, int comp) -> double & { return flux[d * 8 + 5 + comp][c]; };
   F(0, 0) = 0.0;
   F(0, 1) = 0.5 * (e2[c] + e2[c + sj]);
   F(0, 2) = -0.5 * (e1[c] + e1[c + sk]);
@@ -72,13 +70,13 @@ __device__ __forceinline__ void ct_flux_3d_body(const BlockGeom &b, long t,
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_ct_flux_3d(BlockGeom b, double *__restrict__ flux,
+__global__ __launch_bounds__(256) void k_ct_flux_3d(BlockGeom b, FluxPtrs flux,
                                                     const double *__restrict__ emf) {
   ct_flux_3d_body<NC>(b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, emf);
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_ct_emf_2d(BlockGeom b, const double *__restrict__ flux,
+__global__ __launch_bounds__(256) void k_ct_emf_2d(BlockGeom b, FluxPtrs flux,
                                                    double *__restrict__ emf) {
   const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,13 +85,13 @@ __global__ __launch_bounds__(256) void k_ct_emf_2d(BlockGeom b, const double *__
   const int j = (int)((t / ni) % nj) + b.is[1];
   const int k = (int)(t / ((long)ni * nj)) + b.is[2];
   const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long n = b.ncell, sj = b.ni;
-  auto F = [&](int d, int comp, long cc) { return flux[((long)d * NC + 5 + comp) * n + cc]; };
+  const long sj = b.ni;
+  auto F = [&](int d, int comp, long cc) { return flux[d * 8 + 5 + comp][cc]; };
   emf[c] = 0.25 * (F(0, 1, c) + F(0, 1, c - sj) - F(1, 0, c) - F(1, 0, c - 1));
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_ct_flux_2d(BlockGeom b, double *__restrict__ flux,
+__global__ __launch_bounds__(256) void k_ct_flux_2d(BlockGeom b, FluxPtrs flux,
                                                     const double *__restrict__ emf) {
   const int ni = b.ie[0] - b.is[0] + 2, nj = b.ie[1] - b.is[1] + 2, nk = b.ie[2] - b.is[2] + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,20 +100,18 @@ __global__ __launch_bounds__(256) void k_ct_flux_2d(BlockGeom b, double *__restr
   const int j = (int)((t / ni) % nj) + b.is[1];
   const int k = (int)(t / ((long)ni * nj)) + b.is[2];
   const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long n = b.ncell, sj = b.ni;
-  flux[((long)0 * NC + 5) * n + c] = 0.0;
-  flux[((long)0 * NC + 6) * n + c] = 0.5 * (emf[c] + emf[c + sj]);
-  flux[((long)1 * NC + 5) * n + c] = -0.5 * (emf[c] + emf[c + 1]);
-  flux[((long)1 * NC + 6) * n + c] = 0.0;
+  const long sj = b.ni;
+  flux[0 * 8 + 5][c] = 0.0;
+  flux[0 * 8 + 6][c] = 0.5 * (emf[c] + emf[c + sj]);
+  flux[1 * 8 + 5][c] = -0.5 * (emf[c] + emf[c + 1]);
+  flux[1 * 8 + 6][c] = 0.0;
 }
 
 // ---- flux divergence + sources + RK average/update (phoebus_driver.cpp:560-576) -------
 template <int NC>
-__global__ __launch_bounds__(256) void k_update(BlockGeom b, const double *__restrict__ flux,
-                                                const double *__restrict__ src,
-                                                double *u_stage,
-                                                const double *u_base, double beta, double bdt,
-                                                double *u_out) {
+__global__ __launch_bounds__(256) void k_update(BlockGeom b, FluxPtrs flux, ConsPtrs src,
+                                                ConsPtrs u_stage, ConsPtrs u_base, double beta,
+                                                double bdt, ConsPtrs u_out) {
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nk) return;
@@ -123,34 +119,32 @@ __global__ __launch_bounds__(256) void k_update(BlockGeom b, const double *__res
   const int j = (int)((t / ni) % nj) + b.is[1];
   const int k = (int)(t / ((long)ni * nj)) + b.is[2];
   const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long n = b.ncell;
   const double a1 = b.dx[1] * b.dx[2], a2 = b.dx[0] * b.dx[2], a3 = b.dx[0] * b.dx[1];
   const double vol = b.dx[0] * b.dx[1] * b.dx[2];
 #pragma unroll
   for (int l = 0; l < NC; ++l) {
-    const double *F1 = flux + (long)l * n;
+    const double *F1 = flux[l];
     double du = a1 * F1[c + 1] - a1 * F1[c];
     if (b.ndim >= 2) {
-      const double *F2 = flux + ((long)NC + l) * n;
+      const double *F2 = flux[8 + l];
       du += a2 * F2[c + b.ni] - a2 * F2[c];
     }
     if (b.ndim == 3) {
-      const double *F3 = flux + ((long)2 * NC + l) * n;
+      const double *F3 = flux[16 + l];
       du += a3 * F3[c + (long)b.ni * b.nj] - a3 * F3[c];
     }
     double dudt = -du / vol;
-    if (src != nullptr && l >= 1 && l <= 4) dudt = dudt + src[(long)l * n + c];
-    const double us = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
-    u_stage[(long)l * n + c] = us;
-    u_out[(long)l * n + c] = us + bdt * dudt;
+    if (src[l] != nullptr && l >= 1 && l <= 4) dudt = dudt + src[l][c];
+    const double us = beta * u_stage[l][c] + (1 - beta) * u_base[l][c];
+    u_stage[l][c] = us;
+    u_out[l][c] = us + bdt * dudt;
   }
 }
 
 // ---- con2prim over a cell range (fluid.cpp:478-517) ------------------------------------
 template <bool MHD, int GEOM>
 __global__ __launch_bounds__(256) void k_con2prim(DevParams p, BlockGeom b, Range3 rg,
-                                                  double *__restrict__ cons,
-                                                  double *__restrict__ prim,
+                                                  ConsPtrs cons, PrimPtrs prim,
                                                   double *__restrict__ temp,
                                                   double *__restrict__ mu,
                                                   double *__restrict__ csig,
@@ -167,8 +161,7 @@ __global__ __launch_bounds__(256) void k_con2prim(DevParams p, BlockGeom b, Rang
 // ---- prim -> cons over a cell range (fluid.cpp:359-449) --------------------------------
 template <bool MHD, int GEOM>
 __global__ __launch_bounds__(256) void k_prim_to_con(DevParams p, BlockGeom b, Range3 rg,
-                                                     const double *__restrict__ prim,
-                                                     double *__restrict__ cons,
+                                                     PrimPtrs prim, ConsPtrs cons,
                                                      double *__restrict__ csig) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rg.n[0] * rg.n[1] * rg.n[2]) return;
@@ -404,10 +397,9 @@ void launch_geo_fill(const DevParams &dp, const BlockGeom &bg, double *geo, hipS
 }
 
 // geometric source terms of interior cell t of block b
-template <bool MHD, int GEOM, bool CACHED = false>
+template <bool MHD, int GEOM, bool CACHED = false, class PP, class SP>
 __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom &b, long t,
-                                             const double *__restrict__ prim,
-                                             double *__restrict__ src) {
+                                             const PP &prim, const SP &src) {
   using L = Layout<MHD>;
   using M = typename MetricOf<GEOM>::type;
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
@@ -423,7 +415,7 @@ __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom
   KSDerivs dv;
   geom_derivs_at<GEOM, CACHED>(p, b, q, X1, X2, X3, dv);
   double pr[L::nprim];
-  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v * n + c];
+  for (int v = 0; v < L::nprim; ++v) pr[v] = prim[v][c];
   // tmunu.hpp:94-126
   double gam[3][3];
   for (int x = 0; x < 3; ++x)
@@ -470,9 +462,9 @@ __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom
 #pragma unroll
       for (int nn = 0; nn < 4; ++nn) P[cd] += T[m][nn] * dv.dg[m][nn][cd];
   const double gdet = g.detg();
-  src[1L * n + c] = gdet * (0.5 * P[1]);
-  src[2L * n + c] = gdet * (0.5 * P[2]);
-  src[3L * n + c] = 0.0;
+  src[1][c] = gdet * (0.5 * P[1]);
+  src[2][c] = gdet * (0.5 * P[2]);
+  src[3][c] = 0.0;
   const double ia2 = ratio(1, alpha * alpha);
   const double gcon0[4] = {-ia2, ia2 * g.beta(0), ia2 * g.beta(1), ia2 * g.beta(2)};
   double TGam = 0.0;
@@ -487,21 +479,20 @@ __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom
     TGam += gcon0[r] * (Q - half);
   }
   const double Ta = T[1][0] * dv.da[1] + T[2][0] * dv.da[2];
-  src[4L * n + c] = gdet * alpha * (Ta - TGam);
+  src[4][c] = gdet * alpha * (Ta - TGam);
 }
 
 template <bool MHD, int GEOM>
-__global__ __launch_bounds__(128) void k_sources(DevParams p, BlockGeom b,
-                                                 const double *__restrict__ prim,
-                                                 double *__restrict__ src) {
+__global__ __launch_bounds__(128) void k_sources(DevParams p, BlockGeom b, PrimPtrs prim,
+                                                 ConsPtrs src) {
   sources_body<MHD, GEOM>(p, b, (long)blockIdx.x * blockDim.x + threadIdx.x, prim, src);
 }
 
 template <int NC>
-__global__ void k_zero_sources(BlockGeom b, double *__restrict__ src) {
+__global__ void k_zero_sources(BlockGeom b, ConsPtrs src) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= b.ncell) return;
-  for (int l = 1; l <= 4; ++l) src[(long)l * b.ncell + t] = 0.0;
+  for (int l = 1; l <= 4; ++l) src[l][t] = 0.0;
 }
 
 // ---- packs: CT and sources over a work list of 3-D blocks -------------------------------
@@ -510,7 +501,8 @@ __global__ __launch_bounds__(256) void k_pack_ct_emf(PackView v) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
-  ct_emf_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.emf);
+  ct_emf_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, E.bg.ncell),
+                     E.emf);
 }
 
 template <int NC>
@@ -518,7 +510,8 @@ __global__ __launch_bounds__(256) void k_pack_ct_flux(PackView v) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
-  ct_flux_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.emf);
+  ct_flux_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, E.bg.ncell),
+                      E.emf);
 }
 
 template <int GEOM>
@@ -527,11 +520,12 @@ __global__ __launch_bounds__(128) void k_pack_sources(DevParams p, PackView v) {
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   // two 128-thread halves per 256-cell group
+  const Contig prim = contig(E.prim, E.bg.ncell), src = contig(E.src, E.bg.ncell);
   sources_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, E.bg, (g - v.start[b]) * 256 + threadIdx.x,
-                                                 E.prim, E.src);
+                                                 prim, src);
   sources_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, E.bg,
                                                  (g - v.start[b]) * 256 + 128 + threadIdx.x,
-                                                 E.prim, E.src);
+                                                 prim, src);
 }
 
 // ---- batched con2prim / prim2con over independent states --------------------------------
@@ -655,6 +649,8 @@ int check_params(phx_ctx *ctx, const phx_fluid_params *p) {
   if (p->bounds.floor_type < 0 || p->bounds.floor_type > PHX_FLOOR_R_RHO_SIE)
     return ctx->fail(PHX_ERR_INVALID, "invalid floor type");
   if (p->c2p_max_iter < 0) return ctx->fail(PHX_ERR_INVALID, "c2p_max_iter < 0");
+  if (p->flux_arith != PHX_FLUX_ARITH_EXACT && p->flux_arith != PHX_FLUX_ARITH_FAST)
+    return ctx->fail(PHX_ERR_INVALID, "invalid flux arithmetic");
   return PHX_OK;
 }
 
@@ -697,7 +693,7 @@ void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long 
   });
 }
 
-void launch_flux_ct(const BlockGeom &bg, double *flux, double *emf, hipStream_t s) {
+void launch_flux_ct(const BlockGeom &bg, const FluxPtrs &flux, double *emf, hipStream_t s) {
   const int nx0 = bg.ie[0] - bg.is[0] + 1, nx1 = bg.ie[1] - bg.is[1] + 1;
   const int nx2 = bg.ie[2] - bg.is[2] + 1;
   if (bg.ndim == 3) {
@@ -712,7 +708,7 @@ void launch_flux_ct(const BlockGeom &bg, double *flux, double *emf, hipStream_t 
 }
 
 void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
-                    const double *prim, double *src, hipStream_t s) {
+                    const PrimPtrs &prim, const ConsPtrs &src, hipStream_t s) {
   if (geom == PHX_GEOM_MINKOWSKI) {
     if (mhd)
       hipLaunchKernelGGL(k_zero_sources<8>, dim3(nblocks(bg.ncell, 256)), dim3(256), 0, s, bg, src);
@@ -760,13 +756,18 @@ int phx_calculate_fluxes(phx_ctx *ctx, const phx_fluid_params *p, const phx_bloc
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = check_block(ctx, b, p->recon))) return rc;
-  if (!f || !f->prim || !f->flux || !f->face_sig) return ctx->fail(PHX_ERR_INVALID, "null field");
-  const DevParams dp = make_dev_params(p);
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
   const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const FluxPtrs flux = flux_ptrs(f, bg.ncell, nc);
+  if (!complete(prim, np) || !complete_flux(flux, b->ndim, nc) || !f->face_sig)
+    return ctx->fail(PHX_ERR_INVALID, "null field");
+  const DevParams dp = make_dev_params(p);
   hipStream_t s = ctx->stream_of(stream);
   for (int d = 0; d < b->ndim; ++d)
-    launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
-                  f->face_sig, s, ctx->tile_for(p, b, d));
+    launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, prim, flux,
+                  f->face_sig, s, ctx->tile_for(p, b, d), flux_fast_of(p));
   return ctx->check_launch();
 }
 
@@ -777,8 +778,11 @@ int phx_flux_ct(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   if (rc) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_LINEAR))) return rc;  // CT reads 2 ghost layers
   if (!p->mhd || b->ndim == 1) return PHX_OK;
-  if (!f || !f->flux || !f->emf) return ctx->fail(PHX_ERR_INVALID, "null field");
-  launch_flux_ct(make_block_geom(b), f->flux, f->emf, ctx->stream_of(stream));
+  if (!f || !f->emf) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const FluxPtrs flux = flux_ptrs(f, bg.ncell, 8);
+  if (!complete_flux(flux, b->ndim, 8)) return ctx->fail(PHX_ERR_INVALID, "null field");
+  launch_flux_ct(bg, flux, f->emf, ctx->stream_of(stream));
   return ctx->check_launch();
 }
 
@@ -788,9 +792,13 @@ int phx_fluid_source_terms(phx_ctx *ctx, const phx_fluid_params *p, const phx_bl
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
-  if (!f || !f->src || !f->prim) return ctx->fail(PHX_ERR_INVALID, "null field");
-  launch_sources(p->geom.kind, p->mhd, make_dev_params(p), make_block_geom(b), f->prim, f->src,
-                 ctx->stream_of(stream));
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const ConsPtrs src = src_ptrs(f, bg.ncell, nc);
+  if (!complete(prim, np) || !complete(src, nc)) return ctx->fail(PHX_ERR_INVALID, "null field");
+  launch_sources(p->geom.kind, p->mhd, make_dev_params(p), bg, prim, src, ctx->stream_of(stream));
   return ctx->check_launch();
 }
 
@@ -807,10 +815,15 @@ int phx_flux_divergence_update(phx_ctx *ctx, const phx_fluid_params *p, const ph
   hipStream_t s = ctx->stream_of(stream);
   const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
   const double bdt = beta * dt;
+  const int nc = p->mhd ? 8 : 5;
+  const long m = bg.ncell;
+  const FluxPtrs F = FluxPtrs::flux(flux, m, nc);
+  const ConsPtrs S = ConsPtrs::contiguous(src, m, nc), Us = ConsPtrs::contiguous(u_stage, m, nc);
+  const ConsPtrs Ub = ConsPtrs::contiguous(u_base, m, nc), Uo = ConsPtrs::contiguous(u_out, m, nc);
   if (p->mhd)
-    hipLaunchKernelGGL(k_update<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, src, u_stage, u_base, beta, bdt, u_out);
+    hipLaunchKernelGGL(k_update<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, F, S, Us, Ub, beta, bdt, Uo);
   else
-    hipLaunchKernelGGL(k_update<5>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, src, u_stage, u_base, beta, bdt, u_out);
+    hipLaunchKernelGGL(k_update<5>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, F, S, Us, Ub, beta, bdt, Uo);
   return ctx->check_launch();
 }
 
@@ -822,16 +835,20 @@ int phx_con2prim(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
-  if (!f || !f->cons || !f->prim || !f->c2p_mu || !f->cell_sig || !f->fail)
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const ConsPtrs cons = cons_ptrs(f, bg.ncell, nc);
+  if (!complete(cons, nc) || !complete(prim, np) || !f->c2p_mu || !f->cell_sig || !f->fail)
     return ctx->fail(PHX_ERR_INVALID, "null field");
   Range3 rg;
   if ((rc = make_range(ctx, b, lo, hi, rg))) return rc;
   const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
   const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
   const dim3 grid(nblocks(n, 256)), blk(256);
-#define C2P_ARGS dp, bg, rg, f->cons, f->prim, f->temp, f->c2p_mu, f->cell_sig, f->fail, iters
+#define C2P_ARGS dp, bg, rg, cons, prim, f->temp, f->c2p_mu, f->cell_sig, f->fail, iters
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {
       hipLaunchKernelGGL((k_con2prim<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, C2P_ARGS);
@@ -848,17 +865,21 @@ int phx_prim_to_con(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = check_block(ctx, b, PHX_RECON_CONSTANT))) return rc;
-  if (!f || !f->cons || !f->prim) return ctx->fail(PHX_ERR_INVALID, "null field");
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const ConsPtrs cons = cons_ptrs(f, bg.ncell, nc);
+  if (!complete(cons, nc) || !complete(prim, np)) return ctx->fail(PHX_ERR_INVALID, "null field");
   Range3 rg;
   if ((rc = make_range(ctx, b, lo, hi, rg))) return rc;
   const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
   const long n = (long)rg.n[0] * rg.n[1] * rg.n[2];
   const dim3 grid(nblocks(n, 256)), blk(256);
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {
-      hipLaunchKernelGGL((k_prim_to_con<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, dp, bg, rg, f->prim, f->cons, f->cell_sig);
+      hipLaunchKernelGGL((k_prim_to_con<decltype(H)::value, decltype(G)::value>), grid, blk, 0, s, dp, bg, rg, prim, cons, f->cell_sig);
     });
   });
   return ctx->check_launch();

@@ -163,8 +163,9 @@ __global__ __launch_bounds__(256) void k_mesh_convert(DevParams p, MeshInfo m, P
       }
     }
   }
-  prim_to_con_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, bg, idx[0], idx[1], idx[2], E.prim_out,
-                                                      E.u_out, E.csig);
+  prim_to_con_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(
+      p, bg, idx[0], idx[1], idx[2], contig(E.prim_out, bg.ncell), contig(E.u_out, bg.ncell),
+      E.csig);
 }
 
 // FixFluxes for gr_outflow (fixup.cpp:759-1062): no mass inflow through a physical face,
@@ -206,8 +207,10 @@ __global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v) 
   int idx[3];
   if (!cell_of_kind<KIND>(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int32_t *it = (KIND == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
-  con2prim_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, bg, idx[0], idx[1], idx[2], E.u_out,
-                                                   E.prim_out, E.temp, E.mu, E.csig, E.fail,
+  con2prim_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, bg, idx[0], idx[1], idx[2],
+                                                   contig(E.u_out, bg.ncell),
+                                                   contig(E.prim_out, bg.ncell),
+                                                   E.temp, E.mu, E.csig, E.fail,
                                                    it);
 }
 

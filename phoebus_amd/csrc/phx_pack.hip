@@ -69,7 +69,7 @@ void launch_stage_fluxes(phx_ctx *ctx, phx_pack *k, int st, hipStream_t s) {
   for (int d = 0; d < 3; ++d) {
     ctx->mark_begin(kName[d], s);
     launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, view(k, st, d),
-                     k->groups[d], k->tiles[d], s);
+                     k->groups[d], k->tiles[d], s, flux_fast_of(&k->p));
     ctx->mark_end(s);
   }
   ctx->mark_begin("flux_ct", s);
@@ -156,6 +156,10 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
       return rc ? rc : ctx->fail(PHX_ERR_UNSUPPORTED, "packs hold 3-D MHD blocks");
     }
     const phx_block_fields &f = B.f;
+    if (f.vars) {
+      phx_pack_destroy(k);
+      return ctx->fail(PHX_ERR_UNSUPPORTED, "pack blocks take contiguous fields (no vars table)");
+    }
     if (!f.prim || !f.cons || !B.cons1 || !f.flux || !f.face_sig || !f.cell_sig || !f.fail ||
         !f.c2p_mu || !f.emf || (curved && !f.src)) {
       phx_pack_destroy(k);
@@ -327,11 +331,12 @@ int phx_pack_autotune(phx_ctx *ctx, phx_pack *k, int reps, phx_stream stream) {
       k->tiles[d] = c;
       if ((rc = build_start(k, d)) != PHX_OK) return -1.f;
       const PackView v = view(k, 1, d);
-      launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d], c, s);
+      launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d], c, s,
+                       flux_fast_of(&k->p));
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < n; ++r)
         launch_pack_flux(d, k->p.recon, k->p.riemann, k->p.geom.kind, k->dp, v, k->groups[d],
-                         c, s);
+                         c, s, flux_fast_of(&k->p));
       (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) {
         rc = ctx->fail(PHX_ERR_DEVICE, "tuning launch failed");

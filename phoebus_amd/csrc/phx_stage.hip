@@ -14,11 +14,11 @@ namespace phx {
 
 // interior cell t of block b: returns the cell's CFL time (DBL_MAX for t beyond the block
 // or when with_dt is false)
-template <bool MHD, int GEOM, bool CACHED = false, int NDIM = 3>
+template <bool MHD, int GEOM, bool CACHED = false, int NDIM = 3, class FP, class CP, class PP>
 __device__ __forceinline__ double update_con2prim_body(
-    const DevParams &p, const BlockGeom &b, long t, const double *__restrict__ flux,
-    const double *__restrict__ src, double *u_stage, const double *u_base, double beta,
-    double bdt, double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
+    const DevParams &p, const BlockGeom &b, long t, const FP &flux, const CP &src,
+    const CP &u_stage, const CP &u_base, double beta, double bdt,
+    const CP &u_out, const PP &prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
     double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt, bool do_c2p = true) {
   constexpr int NC = Layout<MHD>::ncons;
@@ -45,29 +45,29 @@ __device__ __forceinline__ double update_con2prim_body(
     double cs[NC], usv[NC];
 #pragma unroll
     for (int l = 0; l < NC; ++l) {
-      const double *F1 = flux + (long)l * n;
+      const double *F1 = flux[l];
       double du = a1 * F1[c + 1] - a1 * F1[c];
       if constexpr (NDIM >= 2) {
-        const double *F2 = flux + ((long)NC + l) * n;
+        const double *F2 = flux[8 + l];
         du += a2 * F2[c + sj] - a2 * F2[c];
       }
       if constexpr (NDIM == 3) {
-        const double *F3 = flux + ((long)2 * NC + l) * n;
+        const double *F3 = flux[16 + l];
         du += a3 * F3[c + sk] - a3 * F3[c];
       }
       double dudt = -du / vol;
       if constexpr (SRC) {
-        if (l >= 1 && l <= 4) dudt = dudt + src[(long)l * n + c];
+        if (l >= 1 && l <= 4) dudt = dudt + src[l][c];
       }
-      usv[l] = beta * u_stage[(long)l * n + c] + (1 - beta) * u_base[(long)l * n + c];
+      usv[l] = beta * u_stage[l][c] + (1 - beta) * u_base[l][c];
       cs[l] = usv[l] + bdt * dudt;
     }
 #pragma unroll
-    for (int l = 0; l < NC; ++l) u_stage[(long)l * n + c] = usv[l];
+    for (int l = 0; l < NC; ++l) u_stage[l][c] = usv[l];
     // U_{s+1} goes out before the inversion, which re-derives it only where it used a floor
     // or ceiling (con2prim_fields); no conserved variable is live across the solve
 #pragma unroll
-    for (int l = 0; l < NC; ++l) u_out[(long)l * n + c] = cs[l];
+    for (int l = 0; l < NC; ++l) u_out[l][c] = cs[l];
     if (!do_c2p) return DBL_MAX;  // the inversion runs in a separate (learned-bracket) pass
     // ConservedToPrimitive on U_{s+1}
     double sig[3] = {0.0, 0.0, 0.0};
@@ -77,7 +77,7 @@ __device__ __forceinline__ double update_con2prim_body(
     if (!(isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0)) {
       const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
       st = con2prim_fields<MHD, GEOM, CACHED>(p, b, (long)j * b.ni + i, X, cs, prim_out, temp, mu,
-                                              u_out, n, c, sig, it, solved);
+                                              u_out, c, sig, it, solved);
     }
     // a zone the inversion left (NaN input, unbracketed root) keeps its signal speeds
 #pragma unroll
@@ -112,9 +112,8 @@ __device__ __forceinline__ double update_con2prim_body(
 
 template <bool MHD, int GEOM, int NDIM>
 __global__ __launch_bounds__(256) void k_update_con2prim(
-    DevParams p, BlockGeom b, const double *__restrict__ flux, const double *__restrict__ src,
-    double *u_stage, const double *u_base, double beta, double bdt,
-    double *u_out, double *__restrict__ prim_out, double *__restrict__ temp,
+    DevParams p, BlockGeom b, FluxPtrs flux, ConsPtrs src, ConsPtrs u_stage, ConsPtrs u_base,
+    double beta, double bdt, ConsPtrs u_out, PrimPtrs prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
     double *__restrict__ fail, int32_t *__restrict__ iters, double *__restrict__ partial) {
   const double v = update_con2prim_body<MHD, GEOM, false, NDIM>(
@@ -139,9 +138,11 @@ __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_FMKS ? 4 : 1))) void k_pack_
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
   const double bdt = beta * (dt_dev ? dt_dev[0] : dt);
+  const long n = E.bg.ncell;
   const double val = update_con2prim_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(
-      p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, E.flux, E.src, E.u_stage, E.u_base, beta,
-      bdt, E.u_out, E.prim_out, E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
+      p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, n), contig(E.src, n),
+      contig(E.u_stage, n), contig(E.u_base, n), beta, bdt, contig(E.u_out, n),
+      contig(E.prim_out, n), E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
       partial != nullptr, do_c2p != 0);
   if (partial) {
     const double m = block_min(val);
@@ -163,48 +164,58 @@ void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long g
 
 using namespace phx;
 
-extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
-                               const phx_block_fields *f, const phx_stage_args *a,
-                               phx_stream stream) {
-  if (!ctx) return PHX_ERR_INVALID;
+// the stage's launches; partial = the CFL partial minima buffer (at least kDtPartials)
+static int fluid_stage_launches(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                                const phx_block_fields *f, const phx_stage_args *a,
+                                phx_stream stream, double *partial_buf) {
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = check_block(ctx, b, p->recon))) return rc;
-  if (!a || !f || !f->prim || !f->cons || !f->flux || !f->face_sig || !f->c2p_mu ||
-      !f->cell_sig || !f->fail || !a->u_base || !a->u_out || !a->prim_out)
+  if (!a || !f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const BlockGeom bg = make_block_geom(b);
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const ConsPtrs cons = cons_ptrs(f, bg.ncell, nc), src_c = src_ptrs(f, bg.ncell, nc);
+  const FluxPtrs flux = flux_ptrs(f, bg.ncell, nc);
+  const ConsPtrs u_base = comps<8>(a->u_base, bg.ncell, nc, a->base_vars ? a->base_vars->cons : nullptr);
+  const ConsPtrs u_out = comps<8>(a->u_out, bg.ncell, nc, a->out_vars ? a->out_vars->cons : nullptr);
+  const PrimPtrs prim_out =
+      comps<10>(a->prim_out, bg.ncell, np, a->out_vars ? a->out_vars->prim : nullptr);
+  if (!complete(prim, np) || !complete(cons, nc) || !complete_flux(flux, b->ndim, nc) ||
+      !f->face_sig || !f->c2p_mu || !f->cell_sig || !f->fail || !complete(u_base, nc) ||
+      !complete(u_out, nc) || !complete(prim_out, np))
     return ctx->fail(PHX_ERR_INVALID, "null field");
   if (p->mhd && b->ndim > 1 && (!f->emf || b->ng < 2)) return ctx->fail(PHX_ERR_INVALID, "CT needs emf");
   const bool curved = p->geom.kind != PHX_GEOM_MINKOWSKI;
-  if (curved && !f->src) return ctx->fail(PHX_ERR_INVALID, "curved metric needs src scratch");
+  if (curved && !complete(src_c, nc)) return ctx->fail(PHX_ERR_INVALID, "curved metric needs src scratch");
   const long n = (long)b->nx[0] * b->nx[1] * b->nx[2];
   const long nb = (n + 255) / 256;
   if (a->dt_out && nb > kDtPartials)  // refused before anything is launched
     return ctx->fail(PHX_ERR_INVALID, "block too large for the CFL reduction");
   const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
   static const char *kFluxName[3] = {"flux_x", "flux_y", "flux_z"};
   for (int d = 0; d < b->ndim; ++d) {
     ctx->mark_begin(kFluxName[d], s);
-    launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
-                  f->face_sig, s, ctx->tile_for(p, b, d));
+    launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, prim, flux,
+                  f->face_sig, s, ctx->tile_for(p, b, d), flux_fast_of(p));
     ctx->mark_end(s);
   }
   if (p->mhd && b->ndim > 1) {
     ctx->mark_begin("flux_ct", s);
-    launch_flux_ct(bg, f->flux, f->emf, s);
+    launch_flux_ct(bg, flux, f->emf, s);
     ctx->mark_end(s);
   }
   if (curved) {
     ctx->mark_begin("sources", s);
-    launch_sources(p->geom.kind, p->mhd, dp, bg, f->prim, f->src, s);
+    launch_sources(p->geom.kind, p->mhd, dp, bg, prim, src_c, s);
     ctx->mark_end(s);
   }
-  double *partial = a->dt_out ? ctx->dt_partial : nullptr;
-  const double *src = curved ? f->src : nullptr;
+  double *partial = a->dt_out ? partial_buf : nullptr;
+  const ConsPtrs src = curved ? src_c : ConsPtrs::contiguous(nullptr, 0);
   const double bdt = a->beta * a->dt;
   ctx->mark_begin("update_con2prim", s);
-#define ARGS dp, bg, f->flux, src, f->cons, a->u_base, a->beta, bdt, a->u_out, a->prim_out, \
+#define ARGS dp, bg, flux, src, cons, u_base, a->beta, bdt, u_out, prim_out, \
              f->temp, f->c2p_mu, f->cell_sig, f->face_sig, f->fail, a->iters, partial
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {
@@ -220,8 +231,15 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
   ctx->mark_end(s);
   if (a->dt_out) {
     ctx->mark_begin("dt_finish", s);
-    launch_dt_finish(ctx->dt_partial, (int)nb, a->cfl, a->dt_out, s);
+    launch_dt_finish(partial, (int)nb, a->cfl, a->dt_out, s);
     ctx->mark_end(s);
   }
   return ctx->check_launch();
+}
+
+extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
+                               const phx_block_fields *f, const phx_stage_args *a,
+                               phx_stream stream) {
+  if (!ctx) return PHX_ERR_INVALID;
+  return fluid_stage_launches(ctx, p, b, f, a, stream, ctx->dt_partial);
 }

@@ -29,10 +29,15 @@ int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = check_block(ctx, b, p->recon))) return rc;
-  if (!f || !f->prim || !f->flux || !f->face_sig) return ctx->fail(PHX_ERR_INVALID, "null field");
+  if (!f) return ctx->fail(PHX_ERR_INVALID, "null field");
+  const BlockGeom bg = make_block_geom(b);
+  const int np = p->mhd ? 10 : 7, nc = p->mhd ? 8 : 5;
+  const PrimPtrs prim = prim_ptrs(f, bg.ncell, np);
+  const FluxPtrs flux = flux_ptrs(f, bg.ncell, nc);
+  if (!complete(prim, np) || !complete_flux(flux, b->ndim, nc) || !f->face_sig)
+    return ctx->fail(PHX_ERR_INVALID, "null field");
   if (reps < 1) reps = 3;
   const DevParams dp = make_dev_params(p);
-  const BlockGeom bg = make_block_geom(b);
   hipStream_t s = ctx->stream_of(stream);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
@@ -70,12 +75,12 @@ int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
     // ms for n launches of candidate k after one warm-up launch; < 0 when a launch failed
     auto time_cand = [&](size_t k, int n) -> float {
       const TileShape &c = cand[k];
-      launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim, f->flux,
-                    f->face_sig, s, c);
+      launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, prim, flux,
+                    f->face_sig, s, c, flux_fast_of(p));
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < n; ++r)
-        launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, f->prim,
-                      f->flux, f->face_sig, s, c);
+        launch_fluxes(d, p->recon, p->riemann, p->mhd, p->geom.kind, dp, bg, prim, flux,
+                      f->face_sig, s, c, flux_fast_of(p));
       (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) return -1.f;
       float ms = 0.f;

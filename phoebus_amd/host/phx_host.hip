@@ -48,7 +48,7 @@ MeshBlockData::~MeshBlockData() {
 }
 
 phx_block_fields MeshBlockData::fields(int container) const {
-  phx_block_fields f;
+  phx_block_fields f{};  // contiguous fields: no per-component table
   f.prim = f_.prim;
   f.cons = cons_[container];
   f.temp = f_.temp;

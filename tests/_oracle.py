@@ -12,7 +12,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIBPATH = os.path.join(ORACLE_DIR, "_build", "libphx_oracle.so")
+# PHX_ORACLE_LIB: another build of the oracle's source (a planted-mutation test builds one)
+LIBPATH = os.environ.get("PHX_ORACLE_LIB") or os.path.join(ORACLE_DIR, "_build",
+                                                          "libphx_oracle.so")
 
 DBL_MAX = np.finfo(np.float64).max
 
@@ -354,6 +356,11 @@ def geom_gradlnalpha(params, X):
 def threads():
     """OpenMP threads the oracle uses."""
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    """Sets the OpenMP threads of the oracle's later parallel loops."""
+    lib().orc_set_num_threads(int(n))
 
 
 BC_OUTFLOW, BC_PERIODIC = 0, 1
