@@ -244,6 +244,25 @@ typedef struct {
 int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
                     const phx_block_fields *f, const phx_stage_args *a, phx_stream stream);
 
+/* ---- the same stage without a context, allocating nothing ---------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory or
+ * create a context on the calling path (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same launches and results as phx_fluid_stage,
+ * bit for bit, with the built-in default sweep shapes (no autotuned shape: there is no
+ * context).  The CFL partial minima go to a file-scope device array of 65536 doubles
+ * (512 KiB) that each device holds from its first use until the process ends; its address
+ * is looked up by phx_create, or by the first call on a device with that device current.
+ * Call with the stream's device current (stream NULL = that device's null stream).
+ * Limit: with a->dt_out set, at most 65536 * 256 = 16,777,216 interior cells per call;
+ * larger blocks are refused with PHX_ERR_INVALID before anything is launched.  The array
+ * keeps the last call's partials; every call writes the entries it reads before reading
+ * them.  Calls with dt_out set that overlap on one device (different streams, or threads)
+ * race on the one array: order them.  On failure phx_last_error(NULL) returns the calling
+ * thread's message. */
+int phx_fluid_stage_noalloc(const phx_fluid_params *p, const phx_block *b,
+                            const phx_block_fields *f, const phx_stage_args *a,
+                            phx_stream stream);
+
 /* ---- batched con2prim (independent states) ---------------------------------------
  * The per-zone solve of con2prim_robust.hpp:299-453 over n states at spherical
  * Kerr-Schild points coords = [r[n], theta[n]] (phi = 0), or Minkowski when

@@ -127,6 +127,7 @@ def lib():
         "phx_con2prim": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(i32), P(i32), vp, vp]),
         "phx_estimate_dt": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), i32, dbl, vp, vp]),
         "phx_fluid_stage": (i32, [vp, P(FluidParams), P(Block), P(BlockFields), P(StageArgs), vp]),
+        "phx_fluid_stage_noalloc": (i32, [P(FluidParams), P(Block), P(BlockFields), P(StageArgs), vp]),
         "phx_con2prim_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
         "phx_prim_to_con_batch": (i32, [vp, P(FluidParams), C.c_int64, vp, vp, vp, vp]),
         "phx_mlp_create": (i32, [vp, P(vp), C.c_uint64]),

@@ -229,6 +229,11 @@ long pack_groups_cells(const BlockGeom &bg, bool corners);
 // partial minima of the CFL reduction (one per 256-cell block of the fused stage kernel,
 // so one call covers up to 16.7 M cells; the separate reduction strides beyond that)
 constexpr int kDtPartials = 1 << 16;
+// the allocation-free stage's device scratch (kDtPartials doubles in a file-scope device
+// array, phx_stage.hip): its address on the current device, looked up once per device
+int static_scratch(double **out);
+// the message of the calling thread's last failed phx_fluid_stage_noalloc ("" if none)
+const char *noalloc_last_error();
 }  // namespace phx
 
 struct phx_ctx {

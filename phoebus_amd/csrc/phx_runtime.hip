@@ -22,6 +22,12 @@ int phx_create(phx_ctx **out, int device) {
   phx_ctx *c = new (std::nothrow) phx_ctx();
   if (!c) return PHX_ERR_NOMEM;
   c->device = device;
+  // the device's copy of the allocation-free stage's scratch, looked up at set-up
+  double *scratch_ptr = nullptr;
+  if (phx::static_scratch(&scratch_ptr) != PHX_OK) {
+    delete c;
+    return PHX_ERR_DEVICE;
+  }
   if (hipStreamCreate(&c->stream) != hipSuccess ||
       hipMalloc(&c->dt_partial, sizeof(double) * phx::kDtPartials) != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -43,7 +49,11 @@ int phx_destroy(phx_ctx *c) {
   return PHX_OK;
 }
 
-const char *phx_last_error(const phx_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+const char *phx_last_error(const phx_ctx *c) {
+  if (c) return c->last_error.c_str();
+  const char *m = phx::noalloc_last_error();  // phx_fluid_stage_noalloc has no context
+  return m[0] ? m : "null context";
+}
 
 int phx_rt_device_count(int *n) { return rt(hipGetDeviceCount(n)); }
 int phx_rt_set_device(int d) { return rt(hipSetDevice(d)); }

@@ -6,6 +6,10 @@
 // inversion.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <cstring>
+#include <mutex>
+
 #include "phx.h"
 #include "phx_device.hpp"
 #include "phx_internal.hpp"
@@ -164,6 +168,44 @@ void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long g
 
 using namespace phx;
 
+// Device scratch of the allocation-free stage (phx_fluid_stage_noalloc): the CFL partial
+// minima, one per 256 interior cells, in a file-scope device array.  Each device holds its
+// own copy for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name in host code is a
+// stand-in, not a device address).
+static __device__ __align__(256) double scratch[kDtPartials];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<double *> g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+thread_local char g_noalloc_err[256];
+}  // namespace
+
+namespace phx {
+// the address of the current device's scratch, looked up once per device
+int static_scratch(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return PHX_ERR_DEVICE;
+  if (dev < 0 || dev >= kMaxDevices) return PHX_ERR_UNSUPPORTED;
+  double *p = g_scratch[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    p = g_scratch[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void *q = nullptr;
+      if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q)
+        return PHX_ERR_DEVICE;
+      p = static_cast<double *>(q);
+      g_scratch[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return PHX_OK;
+}
+const char *noalloc_last_error() { return g_noalloc_err; }
+}  // namespace phx
+
 // the stage's launches; partial = the CFL partial minima buffer (at least kDtPartials)
 static int fluid_stage_launches(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
                                 const phx_block_fields *f, const phx_stage_args *a,
@@ -192,6 +234,10 @@ static int fluid_stage_launches(phx_ctx *ctx, const phx_fluid_params *p, const p
   const long nb = (n + 255) / 256;
   if (a->dt_out && nb > kDtPartials)  // refused before anything is launched
     return ctx->fail(PHX_ERR_INVALID, "block too large for the CFL reduction");
+  if (a->dt_out && !partial_buf) {  // the allocation-free form looks its scratch up here
+    if ((rc = static_scratch(&partial_buf)))
+      return ctx->fail(rc, "no device scratch on the current device");
+  }
   const DevParams dp = make_dev_params(p);
   hipStream_t s = ctx->stream_of(stream);
   static const char *kFluxName[3] = {"flux_x", "flux_y", "flux_z"};
@@ -242,4 +288,21 @@ extern "C" int phx_fluid_stage(phx_ctx *ctx, const phx_fluid_params *p, const ph
                                phx_stream stream) {
   if (!ctx) return PHX_ERR_INVALID;
   return fluid_stage_launches(ctx, p, b, f, a, stream, ctx->dt_partial);
+}
+
+extern "C" int phx_fluid_stage_noalloc(const phx_fluid_params *p, const phx_block *b,
+                                       const phx_block_fields *f, const phx_stage_args *a,
+                                       phx_stream stream) {
+  // a context on the stack for argument checks and the message only: it owns no device
+  // memory and no stream (stream NULL = the device's null stream); sweep shapes are the
+  // built-in defaults
+  phx_ctx local;
+  const int rc = fluid_stage_launches(&local, p, b, f, a, stream, nullptr);
+  if (rc) {
+    std::strncpy(g_noalloc_err, local.last_error.c_str(), sizeof(g_noalloc_err) - 1);
+    g_noalloc_err[sizeof(g_noalloc_err) - 1] = '\0';
+  } else {
+    g_noalloc_err[0] = '\0';
+  }
+  return rc;
 }
