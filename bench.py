@@ -40,15 +40,21 @@ CLOCK_HZ = 2.4e9             # peak engine clock
 B_ALG_STAGE = 52 * 8
 
 
+# the sources the single-block stage's kernels are compiled from (flux sweeps, FluxCT,
+# fused update + con2prim, dt, p2c) with every header they include
+STAGE_SOURCES = ["phoebus_amd/csrc/" + f for f in (
+    "phx_kernels.hip", "phx_flux.hip", "phx_flux_fast.hip", "phx_stage.hip", "phx_device.hpp",
+    "phx_internal.hpp", "phx_cells.hpp", "phx_flux_shapes.hpp", "phx_flux_sweeps.hpp")] + [
+    "include/phx.h", "include/phx_fixup.h"]
+
+
 def sources_digest():
-    """sha256 over the library's kernel sources and headers (the build a profile measured)."""
-    import glob
+    """sha256 over the stage kernels' sources and headers (the build a stage profile
+    measured)."""
     import hashlib
 
     h = hashlib.sha256()
-    files = sorted(glob.glob(os.path.join(ROOT, "phoebus_amd", "csrc", "*.hip")) +
-                   glob.glob(os.path.join(ROOT, "phoebus_amd", "csrc", "*.hpp")) +
-                   glob.glob(os.path.join(ROOT, "include", "*.h")))
+    files = sorted(os.path.join(ROOT, f) for f in STAGE_SOURCES)
     for f in files:
         h.update(os.path.basename(f).encode())
         h.update(open(f, "rb").read())
