@@ -180,6 +180,37 @@ def cpu_baseline(prim, cons, mu0, dt, args, log=lambda m: None):
                       f"median {t_one:.3f} s"}
 
 
+def bracket_centre_stats(net, feat, out, mu, ok, iters, fallback):
+    """The network's bracket against the plain root on one chunk of states: percentiles of
+    the relative centre error |c / mu - 1| and of the relative half-width, the share of
+    roots outside the bracket, and the learned solve's mean iterations per decade of
+    centre error (fallback solves included)."""
+    mean, std = net.norm
+    ws, wmin = net.width
+    f0 = feat[0].astype(np.float64) * std[0] + mean[0]
+    f2 = feat[2].astype(np.float64) * std[2] + mean[2]
+    z0 = np.maximum(1.0 + (np.exp(f0) - 1e-12) - 0.5 * (np.exp(f2) - 1e-12), 1.0)
+    c = np.exp(out[0].astype(np.float64)) / z0
+    half = np.maximum(ws * np.exp(out[1].astype(np.float64)), wmin)
+    sel = ok & (mu > 0.0)
+    err = np.abs(c[sel] / mu[sel] - 1.0)
+    outside = np.abs(c[sel] - mu[sel]) > half[sel] * c[sel]
+    half, it, fb = half[sel], iters[sel], fallback[sel]
+    q = [50, 90, 99]
+    decades = {}
+    for lo in range(-8, 1):
+        m = (err >= 10.0 ** (lo - 1)) & (err < 10.0 ** lo) if lo > -8 else err < 10.0 ** lo
+        if np.any(m):
+            decades[f"1e{lo - 1}..1e{lo}" if lo > -8 else f"<1e{lo}"] = {
+                "share": float(np.mean(m)), "mean_iters": float(np.mean(it[m])),
+                "fallback": float(np.mean(fb[m]))}
+    return {"states": int(sel.sum()),
+            "centre_rel_err_p50_p90_p99": [float(x) for x in np.percentile(err, q)],
+            "half_width_rel_p50_p90_p99": [float(x) for x in np.percentile(half, q)],
+            "root_outside_bracket": float(np.mean(outside)),
+            "iters_by_centre_err": decades}
+
+
 def c3_learned_vs_plain(phx, args, log):
     """Batched con2prim on spherical Kerr-Schild (a = 0.9375, Gamma = 4/3, tol 1e-10): the
     plain solver and the learned-bracket solver on the same 2^26 states (seed SEED, in
@@ -207,6 +238,7 @@ def c3_learned_vs_plain(phx, args, log):
     fb_count = ok_count = 0
     ms_plain = ms_learned = 0.0
     max_dmu = 0.0
+    centre = None
     fail_mismatch = 0
     sample = None
     d_coords = DeviceArray((2, chunk))
@@ -232,6 +264,11 @@ def c3_learned_vs_plain(phx, args, log):
             sample = (coords[:, :2048].copy(), d_cons.get()[:, :2048].copy())
         d_mu.zero()
         d_mu2.zero()
+        if c0 == 0:  # the network's bracket for the first chunk, before any solve
+            d_feat = DeviceArray((4, chunk), np.float32)
+            d_o = DeviceArray((2, chunk), np.float32)
+            net.features(p, chunk, d_coords, d_cons, d_feat)
+            net.forward(chunk, d_feat, d_o)
         synchronize()
         e0.record()
         phx.con2prim_batch(p, chunk, d_coords, d_cons, d_out, d_mu, d_fail, d_it)
@@ -250,6 +287,10 @@ def c3_learned_vs_plain(phx, args, log):
         fb_count += int(np.sum(d_fb.get()[ok]))
         ok_count += int(np.sum(ok))
         max_dmu = max(max_dmu, float(np.max(np.abs(d_mu.get()[ok] - d_mu2.get()[ok]))))
+        if c0 == 0:
+            centre = bracket_centre_stats(net, d_feat.get(), d_o.get(), d_mu.get(), ok,
+                                          d_it2.get(), d_fb.get())
+            del d_feat, d_o
     if sample is not None:
         mean_p = float(np.dot(np.arange(128), hp) / max(ok_count, 1))
     mean_l = float(np.dot(np.arange(128), hl) / max(ok_count, 1))
@@ -265,7 +306,8 @@ def c3_learned_vs_plain(phx, args, log):
            "hist_plain": np.trim_zeros(hp, "b").tolist(),
            "hist_learned": np.trim_zeros(hl, "b").tolist(),
            "train": {"pairs": args.c3_pairs, "steps": args.c3_train_steps,
-                     "seconds": t_train, "loss_first_last": [losses[0], losses[-1]]}}
+                     "seconds": t_train, "loss_first_last": [losses[0], losses[-1]]},
+           "bracket_first_chunk": centre}
     if not args.no_cpu and sample is not None:
         # FP64 oracle on a sample of the first chunk: its root against the learned one
         sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -253,6 +253,7 @@ class Mlp:
         return w
 
     def set_normalization(self, mean, std, width_scale=4.0, width_min=1e-9):
+        self.width = (float(width_scale), float(width_min))
         m = (C.c_float * 4)(*[float(x) for x in mean])
         s = (C.c_float * 4)(*[float(x) for x in std])
         self.phx._chk(self.L.phx_mlp_set_normalization(self.phx.h, self.h, m, s, width_scale,

@@ -58,13 +58,23 @@ static_assert(kNParam == PHX_MLP_NPARAM, "parameter count");
 // compact copy of the biases (b1, b2, b3, b4), rebuilt with the packed weights
 constexpr int kB1 = 0, kB2 = kHid, kB3 = 2 * kHid, kB4 = 3 * kHid, kNBias = 3 * kHid + kOut;
 
-// fragment counts: forward A = W (M = out, K = in); backward A = W^T (M = in, K = out)
-constexpr int kFwdFrags = 4 * 1 + 4 * 4 + 4 * 4 + 1 * 4;  // L1, L2, L3, L4
-constexpr int kBwdFrags = 4 * 4 + 4 * 4 + 4 * 1;          // L2, L3, L4 (W^T)
-constexpr int fwdBase[4] = {0, 4, 20, 36};
+// Forward fragments (A = W, M = out, K = in).  Layer 1 (K = 4 inputs) runs on
+// v_mfma_f32_16x16x16_bf16: 4 M-tiles, one bf16x4 (one uint64) per lane.  Layers 2-4
+// (K = 64) run on gfx950's v_mfma_f32_16x16x32_bf16: per M-tile two K-chunks of 32, one
+// bf16x8 (two uint64) per lane.  The K order inside chunk kc is the C layout of the
+// previous layer's tiles 2kc and 2kc+1: lane group q = l/16 holds slots 8q..8q+7 =
+// rows 4q..4q+3 of tile 2kc, then of tile 2kc+1, so a layer's output tiles are the next
+// layer's B operand with no data movement (the weights are packed in the same order).
+// Fragment f of layers 2-4: L2 f = 2 mt + kc (0..7), L3 8 + 2 mt + kc, L4 16 + kc.
+// Each of the hi and lo sets: [4][64] uint64 for layer 1, then [18][64][2].
+constexpr int kFwd32Frags = 8 + 8 + 2;
+constexpr int kFwdSet = 4 * 64 + kFwd32Frags * 64 * 2;  // uint64 per set
+constexpr int kFwdU64 = 2 * kFwdSet;                    // hi set, then lo set
+// backward fragments (A = W^T: M = in, K = out) on 16x16x16
+constexpr int kBwdFrags = 4 * 4 + 4 * 4 + 4 * 1;  // L2, L3, L4
 
 struct MlpDev {
-  const uint64_t *fwd;   // [2][kFwdFrags][64] packed bf16x4: hi parts, then lo parts
+  const uint64_t *fwd;   // [kFwdU64]: the hi set, then the lo set (layout above)
   const uint64_t *bwd;   // [kBwdFrags][64]
   const float *bias;     // [kNBias] compact biases
   const float *param;    // [kNParam] fp32 master
@@ -72,20 +82,22 @@ struct MlpDev {
   float width_scale, width_min;
 };
 
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// FP32 -> BF16 round to nearest even: gfx950's v_cvt_pk_bf16_f32 (two values per
+// instruction)
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t cvt2(float a, float b) {
+  const bf16x2_t h = {(__bf16)a, (__bf16)b};
+  uint32_t u;
+  memcpy(&u, &h, 4);
+  return u;
 }
+__device__ __forceinline__ uint16_t f2bf(float f) { return (uint16_t)cvt2(f, 0.f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+  const uint64_t u = (uint64_t)cvt2(a, b) | ((uint64_t)cvt2(c, d) << 32);
   bf16x4 v;
-  v[0] = (short)f2bf(a);
-  v[1] = (short)f2bf(b);
-  v[2] = (short)f2bf(c);
-  v[3] = (short)f2bf(d);
+  memcpy(&v, &u, 8);
   return v;
 }
 __device__ __forceinline__ bf16x4 as_bf16x4(uint64_t u) {
@@ -96,26 +108,65 @@ __device__ __forceinline__ bf16x4 as_bf16x4(uint64_t u) {
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 as_bf16x8(uint64_t lo, uint64_t hi) {
+  bf16x8 v;
+  const uint64_t u[2] = {lo, hi};
+  memcpy(&v, u, 16);
+  return v;
+}
+__device__ __forceinline__ bf16x8 cat8(bf16x4 a, bf16x4 b) {
+  uint64_t ua, ub;
+  memcpy(&ua, &a, 8);
+  memcpy(&ub, &b, 8);
+  return as_bf16x8(ua, ub);
+}
 
 // ---- forward for one tile of 16 zones -------------------------------------------------
 struct Split {
   bf16x4 hi, lo;
 };
 __device__ __forceinline__ Split split4(float a, float b, float c, float d) {
+  // hi = bf16(x), lo = bf16(x - hi); the low/high halves of a packed pair widen to FP32
+  // by a shift and a mask
+  const uint32_t h01 = cvt2(a, b), h23 = cvt2(c, d);
+  const float ra = a - __uint_as_float(h01 << 16), rb = b - __uint_as_float(h01 & 0xffff0000u);
+  const float rc = c - __uint_as_float(h23 << 16), rd = d - __uint_as_float(h23 & 0xffff0000u);
+  const uint64_t hi = (uint64_t)h01 | ((uint64_t)h23 << 32);
+  const uint64_t lo = (uint64_t)cvt2(ra, rb) | ((uint64_t)cvt2(rc, rd) << 32);
   Split s;
-  s.hi = pack4(a, b, c, d);
-  s.lo = pack4(a - bf2f((uint16_t)s.hi[0]), b - bf2f((uint16_t)s.hi[1]),
-               c - bf2f((uint16_t)s.hi[2]), d - bf2f((uint16_t)s.hi[3]));
+  memcpy(&s.hi, &hi, 8);
+  memcpy(&s.lo, &lo, 8);
   return s;
 }
-// split-BF16 product accumulate: W . a with W = (whi, wlo) fragments of one A tile
-__device__ __forceinline__ f32x4 mfma3(const MlpDev &m, int frag, int lane, const Split &a,
+// split-BF16 product accumulate: W . a with W = (whi, wlo) fragments of one A tile;
+// layer 1 (16x16x16, M-tile mt) and layers 2-4 (16x16x32, fragment f, B = two C tiles)
+__device__ __forceinline__ f32x4 mfma3(const MlpDev &m, int mt, int lane, const Split &a,
                                       f32x4 acc, int ofs) {
-  const bf16x4 wh = as_bf16x4(m.fwd[ofs + frag * 64 + lane]);
-  const bf16x4 wl = as_bf16x4(m.fwd[ofs + (kFwdFrags + frag) * 64 + lane]);
+  const bf16x4 wh = as_bf16x4(m.fwd[ofs + mt * 64 + lane]);
+  const bf16x4 wl = as_bf16x4(m.fwd[ofs + kFwdSet + mt * 64 + lane]);
   acc = mfma(wh, a.hi, acc);
   acc = mfma(wh, a.lo, acc);
   acc = mfma(wl, a.hi, acc);
+  return acc;
+}
+struct Split8 {
+  bf16x8 hi, lo;
+};
+__device__ __forceinline__ Split8 split8(const Split &a, const Split &b) {
+  return Split8{cat8(a.hi, b.hi), cat8(a.lo, b.lo)};
+}
+__device__ __forceinline__ f32x4 mfma3_32(const MlpDev &m, int f, int lane, const Split8 &a,
+                                         f32x4 acc, int ofs) {
+  const int q = ofs + 4 * 64 + (f * 64 + lane) * 2;
+  const bf16x8 wh = as_bf16x8(m.fwd[q], m.fwd[q + 1]);
+  const bf16x8 wl = as_bf16x8(m.fwd[kFwdSet + q], m.fwd[kFwdSet + q + 1]);
+  acc = mfma32(wh, a.hi, acc);
+  acc = mfma32(wh, a.lo, acc);
+  acc = mfma32(wl, a.hi, acc);
   return acc;
 }
 
@@ -133,25 +184,28 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma3(m, fwdBase[0] + mt, lane, x, acc, ofs);
+    acc = mfma3(m, mt, lane, x, acc, ofs);
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.bias[ofs + kB1 + mt * 16 + r0 + r], 0.f);
     a[mt] = acc;
     if (h1) h1[mt] = acc;
   }
-  // layers 2 and 3: 64 x 64 . 64 x 16
+  // layers 2 and 3: 64 x 64 . 64 x 16, two K-chunks of 32 per M-tile
 #pragma unroll
   for (int layer = 2; layer <= 3; ++layer) {
-    const int base = fwdBase[layer - 1];
+    const int base = layer == 2 ? 0 : 8;
     const int ob = layer == 2 ? kB2 : kB3;
-    Split bin[4];
+    Split8 bin[2];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) bin[ks] = split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]);
+    for (int kc = 0; kc < 2; ++kc)
+      bin[kc] = split8(split4(a[2 * kc][0], a[2 * kc][1], a[2 * kc][2], a[2 * kc][3]),
+                       split4(a[2 * kc + 1][0], a[2 * kc + 1][1], a[2 * kc + 1][2],
+                              a[2 * kc + 1][3]));
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) acc = mfma3(m, base + mt * 4 + ks, lane, bin[ks], acc, ofs);
+      for (int kc = 0; kc < 2; ++kc) acc = mfma3_32(m, base + 2 * mt + kc, lane, bin[kc], acc, ofs);
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = fmaxf(acc[r] + m.bias[ofs + ob + mt * 16 + r0 + r], 0.f);
       bcur[mt] = acc;
@@ -166,8 +220,12 @@ __device__ __forceinline__ f32x4 mlp_tile_forward(const MlpDev &m, const Split &
   // layer 4: 16(2) x 64 . 64 x 16
   f32x4 o = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks)
-    o = mfma3(m, fwdBase[3] + ks, lane, split4(a[ks][0], a[ks][1], a[ks][2], a[ks][3]), o, ofs);
+  for (int kc = 0; kc < 2; ++kc)
+    o = mfma3_32(m, 16 + kc, lane,
+                 split8(split4(a[2 * kc][0], a[2 * kc][1], a[2 * kc][2], a[2 * kc][3]),
+                        split4(a[2 * kc + 1][0], a[2 * kc + 1][1], a[2 * kc + 1][2],
+                               a[2 * kc + 1][3])),
+                 o, ofs);
   if (r0 == 0) {
     o[0] += m.bias[ofs + kB4 + 0];
     o[1] += m.bias[ofs + kB4 + 1];
@@ -222,16 +280,10 @@ __device__ __forceinline__ void mlp_predict_wave(const MlpDev &m, const float f[
 // ---- kernels: packing, training, Adam, batch inference ------------------------------------
 __global__ void k_mlp_pack(const float *__restrict__ param, uint64_t *__restrict__ fwd,
                            uint64_t *__restrict__ bwd, float *__restrict__ bias) {
+  // threads: layer-1 fragments (4 x 64), layers 2-4 (18 x 64), backward (kBwdFrags x 64),
+  // then the biases
+  constexpr int kT1 = 4 * 64, kT32 = kFwd32Frags * 64, kTB = kBwdFrags * 64;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (kFwdFrags + kBwdFrags) * 64) {
-    const int q = t - (kFwdFrags + kBwdFrags) * 64;
-    if (q < kNBias)
-      bias[q] = q < kB2 ? param[oB1 + q] : q < kB3 ? param[oB2 + q - kB2]
-                : q < kB4 ? param[oB3 + q - kB3] : param[oB4 + q - kB4];
-    return;
-  }
-  const int lane = t & 63, frag = t >> 6;
-  const int row = lane & 15, c0 = 4 * (lane >> 4);
   auto W = [&](int layer, int o, int i) -> float {
     const int nin = layer == 1 ? kIn : kHid, nout = layer == 4 ? kOut : kHid;
     if (o >= nout || i >= nin) return 0.f;
@@ -242,29 +294,46 @@ __global__ void k_mlp_pack(const float *__restrict__ param, uint64_t *__restrict
     return (uint64_t)f2bf(a) | ((uint64_t)f2bf(b) << 16) | ((uint64_t)f2bf(c) << 32) |
            ((uint64_t)f2bf(d) << 48);
   };
-  if (frag < kFwdFrags) {
-    int layer, mt, ks;
-    if (frag < 4) { layer = 1; mt = frag; ks = 0; }
-    else if (frag < 20) { layer = 2; mt = (frag - 4) / 4; ks = (frag - 4) % 4; }
-    else if (frag < 36) { layer = 3; mt = (frag - 20) / 4; ks = (frag - 20) % 4; }
-    else { layer = 4; mt = 0; ks = frag - 36; }
-    const int o = mt * 16 + row, i = ks * 16 + c0;
-    float w4[4], l4[4];
-    for (int e = 0; e < 4; ++e) {
-      w4[e] = W(layer, o, i + e);
-      l4[e] = w4[e] - bf2f(f2bf(w4[e]));
-    }
-    fwd[frag * 64 + lane] = pack(w4[0], w4[1], w4[2], w4[3]);
-    fwd[(kFwdFrags + frag) * 64 + lane] = pack(l4[0], l4[1], l4[2], l4[3]);
-  } else if (frag < kFwdFrags + kBwdFrags) {
-    const int bf = frag - kFwdFrags;
+  auto rest = [&](float w) { return w - bf2f(f2bf(w)); };  // the lo part of a split
+  const int lane = t & 63, row = lane & 15, qg = lane >> 4;
+  if (t < kT1) {  // layer 1: M-tile t / 64, K = inputs 4 qg .. 4 qg + 3 (only 0..3 exist)
+    const int o = (t >> 6) * 16 + row, i = 4 * qg;
+    float w4[4];
+    for (int e = 0; e < 4; ++e) w4[e] = W(1, o, i + e);
+    fwd[t] = pack(w4[0], w4[1], w4[2], w4[3]);
+    fwd[kFwdSet + t] = pack(rest(w4[0]), rest(w4[1]), rest(w4[2]), rest(w4[3]));
+    return;
+  }
+  if (t < kT1 + kT32) {  // layers 2-4: fragment f, K slot e -> input (2kc + e/4) 16 + 4 qg + e%4
+    const int f = (t - kT1) >> 6;
+    const int layer = f < 8 ? 2 : f < 16 ? 3 : 4;
+    const int g = f < 8 ? f : f < 16 ? f - 8 : f - 16;
+    const int mt = layer == 4 ? 0 : g >> 1, kc = layer == 4 ? g : g & 1;
+    const int o = mt * 16 + row;
+    float w8[8];
+    for (int e = 0; e < 8; ++e) w8[e] = W(layer, o, (2 * kc + (e >> 2)) * 16 + 4 * qg + (e & 3));
+    const int q = kT1 + (f * 64 + lane) * 2;
+    fwd[q] = pack(w8[0], w8[1], w8[2], w8[3]);
+    fwd[q + 1] = pack(w8[4], w8[5], w8[6], w8[7]);
+    fwd[kFwdSet + q] = pack(rest(w8[0]), rest(w8[1]), rest(w8[2]), rest(w8[3]));
+    fwd[kFwdSet + q + 1] = pack(rest(w8[4]), rest(w8[5]), rest(w8[6]), rest(w8[7]));
+    return;
+  }
+  if (t < kT1 + kT32 + kTB) {
+    const int bf = (t - kT1 - kT32) >> 6;
+    const int c0 = 4 * qg;
     int layer, mt, ks;  // A = W^T: M = in (mt), K = out (ks)
     if (bf < 16) { layer = 2; mt = bf / 4; ks = bf % 4; }
     else if (bf < 32) { layer = 3; mt = (bf - 16) / 4; ks = (bf - 16) % 4; }
     else { layer = 4; mt = bf - 32; ks = 0; }
     const int i = mt * 16 + row, o = ks * 16 + c0;
     bwd[bf * 64 + lane] = pack(W(layer, o, i), W(layer, o + 1, i), W(layer, o + 2, i), W(layer, o + 3, i));
+    return;
   }
+  const int qb = t - (kT1 + kT32 + kTB);
+  if (qb < kNBias)
+    bias[qb] = qb < kB2 ? param[oB1 + qb] : qb < kB3 ? param[oB2 + qb - kB2]
+               : qb < kB4 ? param[oB3 + qb - kB3] : param[oB4 + qb - kB4];
 }
 
 // transpose a 16x16 tile held in C layout (lane: column l%16, rows 4(l/16)+r) into the
@@ -283,15 +352,17 @@ __device__ __forceinline__ bf16x4 tile_transpose(uint16_t *lds, const f32x4 v) {
 }
 
 // One wavefront per block; each block walks `chunk` samples in tiles of 16, accumulates
-// the loss gradient of every parameter and adds it to grad (fp32 atomics) once.
+// the loss gradient of every parameter and writes it, with its loss, to its own row of
+// `partial` ([blocks][kNParam + 1]); k_mlp_reduce then sums the rows in block order, so a
+// gradient does not depend on the order the blocks ran in (no float atomics).
 // Loss per sample, with t = log(mu* z0): (o1 - t)^2 + lam (o2 - log(|1 - exp(t - o1)| +
 // 1e-12))^2 (the second target, the log relative miss of the centre, held fixed),
 // averaged over the n samples of the step.
 __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
                                                   const float *__restrict__ feat,
                                                   const float *__restrict__ target,
-                                                  float lam, float *__restrict__ grad,
-                                                  float *__restrict__ loss) {
+                                                  float lam, float *__restrict__ partial) {
+  float *__restrict__ grad = partial + (long)blockIdx.x * (kNParam + 1);
   __shared__ uint16_t tl[256];
   const int lane = threadIdx.x;
   const int col = lane & 15, r0 = 4 * (lane >> 4);
@@ -415,16 +486,16 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int o = mt * 16 + r0 + r;
-      if (col < kIn) atomicAdd(&grad[oW1 + o * kIn + col], gW1[mt][r]);
+      if (col < kIn) grad[oW1 + o * kIn + col] = gW1[mt][r];
 #pragma unroll
       for (int nt = 0; nt < 4; ++nt) {
-        atomicAdd(&grad[oW2 + o * kHid + nt * 16 + col], gW2[mt * 4 + nt][r]);
-        atomicAdd(&grad[oW3 + o * kHid + nt * 16 + col], gW3[mt * 4 + nt][r]);
+        grad[oW2 + o * kHid + nt * 16 + col] = gW2[mt * 4 + nt][r];
+        grad[oW3 + o * kHid + nt * 16 + col] = gW3[mt * 4 + nt][r];
       }
       if (col == 0) {
-        atomicAdd(&grad[oB1 + o], gb1[mt][r]);
-        atomicAdd(&grad[oB2 + o], gb2[mt][r]);
-        atomicAdd(&grad[oB3 + o], gb3[mt][r]);
+        grad[oB1 + o] = gb1[mt][r];
+        grad[oB2 + o] = gb2[mt][r];
+        grad[oB3 + o] = gb3[mt][r];
       }
     }
 #pragma unroll
@@ -432,13 +503,27 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev m, long n, int chunk,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int o = r0 + r;
-      if (o < kOut) atomicAdd(&grad[oW4 + o * kHid + nt * 16 + col], gW4[nt][r]);
+      if (o < kOut) grad[oW4 + o * kHid + nt * 16 + col] = gW4[nt][r];
     }
   if (lane == 0) {
-    atomicAdd(&grad[oB4 + 0], gb4[0]);
-    atomicAdd(&grad[oB4 + 1], gb4[1]);
-    atomicAdd(loss, lsum);
+    grad[oB4 + 0] = gb4[0];
+    grad[oB4 + 1] = gb4[1];
+    grad[kNParam] = lsum;
   }
+}
+
+// grad[t] += the sum of the blocks' partial rows in block order; the loss (slot kNParam)
+// goes to *loss the same way
+__global__ void k_mlp_reduce(const float *__restrict__ partial, int blocks,
+                             float *__restrict__ grad, float *__restrict__ loss) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > kNParam) return;
+  float acc = 0.f;
+  for (int b = 0; b < blocks; ++b) acc += partial[(long)b * (kNParam + 1) + t];
+  if (t < kNParam)
+    grad[t] += acc;
+  else
+    *loss = acc;
 }
 
 __global__ void k_mlp_adam(float *__restrict__ param, float *__restrict__ grad,
@@ -476,6 +561,8 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev m, long n,
 // was tried: the loop makes the compiler keep the kernel arguments in scratch and hoist
 // loop invariants into ~270 VGPRs, which costs more than the per-block copy.)
 constexpr int kLearnedThreads = 256;
+// training launches at most this many blocks (their partial gradients are summed in order)
+constexpr int kMaxTrainBlocks = 1024;
 
 // three waves per SIMD (the packed weights in LDS allow no more): the MFMA accumulators then
 // live in VGPRs and nothing spills
@@ -487,9 +574,9 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_con2prim_batch_learned(
     int32_t *__restrict__ iters, int32_t *__restrict__ evals, int32_t *__restrict__ fallback) {
   using M = typename MetricOf<GEOM>::type;
   using L = Layout<MHD>;
-  __shared__ uint64_t wfwd[2 * kFwdFrags * 64];
+  __shared__ uint64_t wfwd[kFwdU64];
   __shared__ float wbias[kNBias];
-  for (int q = threadIdx.x; q < 2 * kFwdFrags * 64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
+  for (int q = threadIdx.x; q < kFwdU64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
   for (int q = threadIdx.x; q < kNBias; q += kLearnedThreads) wbias[q] = m.bias[q];
   __syncthreads();
   MlpDev ml = m;
@@ -587,9 +674,9 @@ k_pack_con2prim_learned(
   using M = typename MetricOf<GEOM>::type;
   constexpr bool CACHED = GEOM == PHX_GEOM_FMKS;  // every FMKS block of a pack has its geometry
   using L = Layout<true>;
-  __shared__ uint64_t wfwd[2 * kFwdFrags * 64];
+  __shared__ uint64_t wfwd[kFwdU64];
   __shared__ float wbias[kNBias];
-  for (int q = threadIdx.x; q < 2 * kFwdFrags * 64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
+  for (int q = threadIdx.x; q < kFwdU64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
   for (int q = threadIdx.x; q < kNBias; q += kLearnedThreads) wbias[q] = m.bias[q];
   __syncthreads();
   MlpDev ml = m;
@@ -714,6 +801,7 @@ using namespace phx;
 struct phx_mlp {
   int device = 0;
   float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
+  float *partial = nullptr;  // [kMaxTrainBlocks][kNParam + 1] per-block gradients
   uint64_t *fwd = nullptr, *bwd = nullptr;
   float *bias = nullptr;
   float fmean[4] = {0, 0, 0, 0}, fstd[4] = {1, 1, 1, 1};
@@ -752,7 +840,7 @@ void phx::launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *
 }
 
 static void repack(phx_mlp *m, hipStream_t s) {
-  const int threads = (kFwdFrags + kBwdFrags) * 64 + kNBias;
+  const int threads = (4 + kFwd32Frags + kBwdFrags) * 64 + kNBias;
   hipLaunchKernelGGL(k_mlp_pack, dim3((threads + 255) / 256), dim3(256), 0, s, m->param, m->fwd,
                      m->bwd, m->bias);
 }
@@ -769,7 +857,8 @@ int phx_mlp_create(phx_ctx *ctx, phx_mlp **out, uint64_t seed) {
              hipMalloc(&m->m1, sizeof(float) * kNParam) == hipSuccess &&
              hipMalloc(&m->m2, sizeof(float) * kNParam) == hipSuccess &&
              hipMalloc(&m->loss, sizeof(float)) == hipSuccess &&
-             hipMalloc(&m->fwd, sizeof(uint64_t) * 2 * kFwdFrags * 64) == hipSuccess &&
+             hipMalloc(&m->partial, sizeof(float) * kMaxTrainBlocks * (kNParam + 1)) == hipSuccess &&
+             hipMalloc(&m->fwd, sizeof(uint64_t) * kFwdU64) == hipSuccess &&
              hipMalloc(&m->bwd, sizeof(uint64_t) * kBwdFrags * 64) == hipSuccess &&
              hipMalloc(&m->bias, sizeof(float) * kNBias) == hipSuccess;
   if (!okm) {
@@ -803,7 +892,8 @@ int phx_mlp_destroy(phx_mlp *m) {
   if (!m) return PHX_OK;
   (void)hipSetDevice(m->device);
   for (void *p : {(void *)m->param, (void *)m->grad, (void *)m->m1, (void *)m->m2,
-                  (void *)m->loss, (void *)m->fwd, (void *)m->bwd, (void *)m->bias})
+                  (void *)m->loss, (void *)m->partial, (void *)m->fwd, (void *)m->bwd,
+                  (void *)m->bias})
     if (p) (void)hipFree(p);
   delete m;
   return PHX_OK;
@@ -848,11 +938,15 @@ int phx_mlp_gradient(phx_ctx *ctx, phx_mlp *m, int64_t n, const float *features,
                      const float *target_mu, float lam, float *loss_out, phx_stream stream) {
   if (!ctx || !m || n <= 0 || !features || !target_mu) return PHX_ERR_INVALID;
   hipStream_t s = ctx->stream_of(stream);
-  constexpr int chunk = 256;
+  // at least 256 samples per block, at most kMaxTrainBlocks blocks (chunks of whole tiles)
+  long chunk = 256;
+  if ((n + chunk - 1) / chunk > kMaxTrainBlocks)
+    chunk = ((n + kMaxTrainBlocks - 1) / kMaxTrainBlocks + 15) / 16 * 16;
   const long blocks = (n + chunk - 1) / chunk;
-  (void)hipMemsetAsync(m->loss, 0, sizeof(float), s);
   hipLaunchKernelGGL(k_mlp_train, dim3((unsigned)blocks), dim3(64), 0, s, dev_view(m), (long)n,
-                     chunk, features, target_mu, lam, m->grad, m->loss);
+                     (int)chunk, features, target_mu, lam, m->partial);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNParam + 256) / 256), dim3(256), 0, s, m->partial,
+                     (int)blocks, m->grad, m->loss);
   if (loss_out) (void)hipMemcpyAsync(loss_out, m->loss, sizeof(float), hipMemcpyDeviceToDevice, s);
   return ctx->check_launch();
 }
