@@ -214,15 +214,19 @@ int phx_estimate_dt(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
 
 /* ---- one RK stage --------------------------------------------------------------------
  * The fluid tasks of one stage of PhoebusDriver::RungeKuttaStage (phoebus_driver.cpp:
- * 330-722) for one block, fused into five launches: CalculateFluxes (:344), FluxCT (:345),
- * CalculateFluidSourceTerms (:322, curved metrics only), then FluxDivergence + SumData +
- * AverageIndependentData + UpdateIndependentData (:560-576) fused with the robust
- * ConservedToPrimitive (FillDerived, :588) and the CFL reduction (:720-722) in one kernel.
+ * 330-722) for one block, fused into five launches: CalculateFluxes (:344), FluxCT's
+ * corner EMFs (:345), CalculateFluidSourceTerms (:322, curved metrics only), then
+ * FluxCT's rewrite of the B fluxes, FluxDivergence + SumData + AverageIndependentData +
+ * UpdateIndependentData (:560-576) fused with the robust ConservedToPrimitive
+ * (FillDerived, :588) and the CFL reduction (:720-722) in one kernel.
  * con2prim runs on the interior cells: the ghost zones are refilled by the boundary
  * exchange and re-inverted there (:646-668), which this call does not cover.
  *
  * Reads f->prim (the stage's primitives) and f->c2p_mu (guess); f->flux, f->face_sig,
  * f->emf and f->src (curved metrics) are written as scratch and keep this stage's values.
+ * With CT (MHD, 2-D and 3-D) the update forms FluxCT's B fluxes from the corner EMFs in
+ * f->emf (the same bits as the rewritten arrays), so f->flux keeps the B fluxes of
+ * CalculateFluxes; phx_flux_ct on f afterwards gives the rewritten ones.
  * f->cons is U_s (averaged in place); writes a->u_out (U_{s+1}, after any con2prim
  * re-synchronisation), a->prim_out (may alias f->prim), f->temp, f->c2p_mu, f->cell_sig,
  * f->fail and, when non-NULL, a->iters and a->dt_out (device; cfl * min over the block,

@@ -10,7 +10,9 @@
  * the number of blocks.  A stage, or a sequence of stages, can be captured into a HIP
  * graph and replayed; the time step can be read from device memory so that replays need
  * no host value.  Blocks must be 3-D and MHD (the torus configurations); results per block
- * are those of phx_fluid_stage on that block.
+ * are those of phx_fluid_stage on that block (FluxCT's B fluxes are formed in the update
+ * from the corner EMFs there too, so each block's flux keeps the B fluxes of
+ * CalculateFluxes).
  *
  * Containers follow parthenon's rk2 LowStorageIntegrator (phoebus_driver.cpp:289-299,
  * 560-576): stage 1 reads U_s = U^n = f.cons and writes U_1 = cons1; stage 2 averages U_1

@@ -144,7 +144,6 @@ __device__ __forceinline__ bool floors_needed(const DevParams &p, const DevFixup
                                               const BlockGeom &b, int i, int j, int k,
                                               const PP &prim) {
   using M = typename MetricOf<GEOM>::type;
-  const long n = b.ncell;
   const long c = ((long)k * b.nj + j) * b.ni + i;
   const double X[3] = {xc(b, 0, i), xc(b, 1, j), xc(b, 2, k)};
   double rho_floor, sie_floor;

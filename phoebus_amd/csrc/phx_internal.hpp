@@ -137,6 +137,8 @@ int check_params(phx_ctx *ctx, const phx_fluid_params *p);
 int check_block(phx_ctx *ctx, const phx_block *b, int recon);
 
 void launch_flux_ct(const BlockGeom &bg, const FluxPtrs &flux, double *emf, hipStream_t s);
+// FluxCT's corner EMFs alone (the fused updates form the rewritten B fluxes from them)
+void launch_ct_emf(const BlockGeom &bg, const FluxPtrs &flux, double *emf, hipStream_t s);
 void launch_sources(int geom, int mhd, const DevParams &dp, const BlockGeom &bg,
                     const PrimPtrs &prim, const ConsPtrs &src, hipStream_t s);
 void launch_dt_finish(const double *partial, int np, double cfl, double *dt, hipStream_t s);
@@ -190,7 +192,7 @@ void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevPa
 void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
                       const PackView &v, long groups, TileShape sh, hipStream_t s);
 }  // namespace flux_fast
-void launch_pack_ct(const PackView &v, long groups, hipStream_t s);
+void launch_pack_ct_emf(const PackView &v, long groups, hipStream_t s);
 void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
                          hipStream_t s);
 // do_c2p = false: U_{s+1} only (the inversion then runs in a learned-bracket pass)

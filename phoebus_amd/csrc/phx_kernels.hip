@@ -408,7 +408,6 @@ __device__ __forceinline__ void sources_body(const DevParams &p, const BlockGeom
   const int j = (int)((t / ni) % nj) + b.is[1];
   const int k = (int)(t / ((long)ni * nj)) + b.is[2];
   const long c = ((long)k * b.nj + j) * b.ni + i;
-  const long n = b.ncell;
   const double X1 = xc(b, 0, i), X2 = xc(b, 1, j), X3 = xc(b, 2, k);
   const long q = (long)j * b.ni + i;
   const M g = metric_at<GEOM, CACHED>(p, b, 0, q, X1, X2, X3);
@@ -503,15 +502,6 @@ __global__ __launch_bounds__(256) void k_pack_ct_emf(PackView v) {
   const PackEntry &E = v.e[b];
   ct_emf_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, E.bg.ncell),
                      E.emf);
-}
-
-template <int NC>
-__global__ __launch_bounds__(256) void k_pack_ct_flux(PackView v) {
-  const long g = xcd_remap(blockIdx.x, gridDim.x);
-  const int b = pack_find(v.start, v.n, g);
-  const PackEntry &E = v.e[b];
-  ct_flux_3d_body<NC>(E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, E.bg.ncell),
-                      E.emf);
 }
 
 template <int GEOM>
@@ -677,12 +667,6 @@ long pack_groups_cells(const BlockGeom &bg, bool corners) {
   return (n + 255) / 256;
 }
 
-void launch_pack_ct(const PackView &v, long groups, hipStream_t s) {
-  if (groups <= 0) return;
-  hipLaunchKernelGGL(k_pack_ct_emf<8>, dim3((unsigned)groups), dim3(256), 0, s, v);
-  hipLaunchKernelGGL(k_pack_ct_flux<8>, dim3((unsigned)groups), dim3(256), 0, s, v);
-}
-
 void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
                          hipStream_t s) {
   if (groups <= 0) return;
@@ -691,6 +675,23 @@ void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long 
     if constexpr (GEOM != PHX_GEOM_MINKOWSKI)
       hipLaunchKernelGGL((k_pack_sources<GEOM>), dim3((unsigned)groups), dim3(128), 0, s, dp, v);
   });
+}
+
+void launch_pack_ct_emf(const PackView &v, long groups, hipStream_t s) {
+  if (groups <= 0) return;
+  hipLaunchKernelGGL(k_pack_ct_emf<8>, dim3((unsigned)groups), dim3(256), 0, s, v);
+}
+
+void launch_ct_emf(const BlockGeom &bg, const FluxPtrs &flux, double *emf, hipStream_t s) {
+  const int nx0 = bg.ie[0] - bg.is[0] + 1, nx1 = bg.ie[1] - bg.is[1] + 1;
+  const int nx2 = bg.ie[2] - bg.is[2] + 1;
+  if (bg.ndim == 3) {
+    const long n = (long)(nx0 + 1) * (nx1 + 1) * (nx2 + 1);
+    hipLaunchKernelGGL(k_ct_emf_3d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, emf);
+  } else {
+    const long n = (long)(nx0 + 1) * (nx1 + 1) * nx2;
+    hipLaunchKernelGGL(k_ct_emf_2d<8>, dim3(nblocks(n, 256)), dim3(256), 0, s, bg, flux, emf);
+  }
 }
 
 void launch_flux_ct(const BlockGeom &bg, const FluxPtrs &flux, double *emf, hipStream_t s) {

@@ -72,8 +72,9 @@ void launch_stage_fluxes(phx_ctx *ctx, phx_pack *k, int st, hipStream_t s) {
                      k->groups[d], k->tiles[d], s, flux_fast_of(&k->p));
     ctx->mark_end(s);
   }
+  // FluxCT: the corner EMFs here; the update forms the rewritten B fluxes from them
   ctx->mark_begin("flux_ct", s);
-  launch_pack_ct(view(k, st, 3), k->groups[3], s);
+  launch_pack_ct_emf(view(k, st, 3), k->groups[3], s);
   ctx->mark_end(s);
   if (k->p.geom.kind != PHX_GEOM_MINKOWSKI) {
     ctx->mark_begin("sources", s);

@@ -16,12 +16,71 @@
 
 namespace phx {
 
+// The face fluxes of conserved component l at cell c: lower (m) and upper (p) face in each
+// direction.  With CT (MHD, 2-D or 3-D) the magnetic components are FluxCT's rewrite
+// (fluid.cpp:795-837) formed here from the corner EMFs, with the expressions and operation
+// order of k_ct_flux_3d / k_ct_flux_2d, so the update gets the same bits as from the
+// rewritten flux arrays without that pass; emf holds the 3 (3-D) or 1 (2-D, the z
+// component) corner EMF fields of k_ct_emf_*.
+template <int NDIM, bool CT, class FP>
+__device__ __forceinline__ void face_fluxes(const FP &flux, const double *__restrict__ emf,
+                                            int l, long c, long n, long sj, long sk,
+                                            double f[3][2]) {
+  if (CT && l >= 5 && NDIM == 3) {
+    const double *e0 = emf, *e1 = emf + n, *e2 = emf + 2 * n;
+    if (l == 5) {
+      f[0][0] = f[0][1] = 0.0;
+      f[1][0] = -0.5 * (e2[c] + e2[c + 1]);
+      f[1][1] = -0.5 * (e2[c + sj] + e2[c + sj + 1]);
+      f[2][0] = 0.5 * (e1[c] + e1[c + 1]);
+      f[2][1] = 0.5 * (e1[c + sk] + e1[c + sk + 1]);
+    } else if (l == 6) {
+      f[0][0] = 0.5 * (e2[c] + e2[c + sj]);
+      f[0][1] = 0.5 * (e2[c + 1] + e2[c + 1 + sj]);
+      f[1][0] = f[1][1] = 0.0;
+      f[2][0] = -0.5 * (e0[c] + e0[c + sj]);
+      f[2][1] = -0.5 * (e0[c + sk] + e0[c + sk + sj]);
+    } else {
+      f[0][0] = -0.5 * (e1[c] + e1[c + sk]);
+      f[0][1] = -0.5 * (e1[c + 1] + e1[c + 1 + sk]);
+      f[1][0] = 0.5 * (e0[c] + e0[c + sk]);
+      f[1][1] = 0.5 * (e0[c + sj] + e0[c + sj + sk]);
+      f[2][0] = f[2][1] = 0.0;
+    }
+    return;
+  }
+  if (CT && NDIM == 2 && (l == 5 || l == 6)) {
+    if (l == 5) {
+      f[0][0] = f[0][1] = 0.0;
+      f[1][0] = -0.5 * (emf[c] + emf[c + 1]);
+      f[1][1] = -0.5 * (emf[c + sj] + emf[c + sj + 1]);
+    } else {
+      f[0][0] = 0.5 * (emf[c] + emf[c + sj]);
+      f[0][1] = 0.5 * (emf[c + 1] + emf[c + 1 + sj]);
+      f[1][0] = f[1][1] = 0.0;
+    }
+    return;
+  }
+  f[0][0] = flux[l][c];
+  f[0][1] = flux[l][c + 1];
+  if constexpr (NDIM >= 2) {
+    f[1][0] = flux[8 + l][c];
+    f[1][1] = flux[8 + l][c + sj];
+  }
+  if constexpr (NDIM == 3) {
+    f[2][0] = flux[16 + l][c];
+    f[2][1] = flux[16 + l][c + sk];
+  }
+}
+
 // interior cell t of block b: returns the cell's CFL time (DBL_MAX for t beyond the block
-// or when with_dt is false)
-template <bool MHD, int GEOM, bool CACHED = false, int NDIM = 3, class FP, class CP, class PP>
+// or when with_dt is false).  CT: take the magnetic fluxes from the corner EMFs in emf
+// (face_fluxes) instead of flux.
+template <bool MHD, int GEOM, bool CACHED = false, int NDIM = 3, bool CT = false, class FP,
+          class CP, class PP>
 __device__ __forceinline__ double update_con2prim_body(
-    const DevParams &p, const BlockGeom &b, long t, const FP &flux, const CP &src,
-    const CP &u_stage, const CP &u_base, double beta, double bdt,
+    const DevParams &p, const BlockGeom &b, long t, const FP &flux, const double *emf,
+    const CP &src, const CP &u_stage, const CP &u_base, double beta, double bdt,
     const CP &u_out, const PP &prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
     double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt, bool do_c2p = true) {
@@ -49,16 +108,11 @@ __device__ __forceinline__ double update_con2prim_body(
     double cs[NC], usv[NC];
 #pragma unroll
     for (int l = 0; l < NC; ++l) {
-      const double *F1 = flux[l];
-      double du = a1 * F1[c + 1] - a1 * F1[c];
-      if constexpr (NDIM >= 2) {
-        const double *F2 = flux[8 + l];
-        du += a2 * F2[c + sj] - a2 * F2[c];
-      }
-      if constexpr (NDIM == 3) {
-        const double *F3 = flux[16 + l];
-        du += a3 * F3[c + sk] - a3 * F3[c];
-      }
+      double f[3][2];
+      face_fluxes<NDIM, CT && MHD>(flux, emf, l, c, n, sj, sk, f);
+      double du = a1 * f[0][1] - a1 * f[0][0];
+      if constexpr (NDIM >= 2) du += a2 * f[1][1] - a2 * f[1][0];
+      if constexpr (NDIM == 3) du += a3 * f[2][1] - a3 * f[2][0];
       double dudt = -du / vol;
       if constexpr (SRC) {
         if (l >= 1 && l <= 4) dudt = dudt + src[l][c];
@@ -116,12 +170,14 @@ __device__ __forceinline__ double update_con2prim_body(
 
 template <bool MHD, int GEOM, int NDIM>
 __global__ __launch_bounds__(256) void k_update_con2prim(
-    DevParams p, BlockGeom b, FluxPtrs flux, ConsPtrs src, ConsPtrs u_stage, ConsPtrs u_base,
+    DevParams p, BlockGeom b, FluxPtrs flux, const double *__restrict__ emf, ConsPtrs src,
+    ConsPtrs u_stage, ConsPtrs u_base,
     double beta, double bdt, ConsPtrs u_out, PrimPtrs prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
     double *__restrict__ fail, int32_t *__restrict__ iters, double *__restrict__ partial) {
-  const double v = update_con2prim_body<MHD, GEOM, false, NDIM>(
-      p, b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, src, u_stage, u_base, beta, bdt,
+  const double v = update_con2prim_body<MHD, GEOM, false, NDIM, MHD && NDIM >= 2>(
+      p, b, (long)blockIdx.x * blockDim.x + threadIdx.x, flux, emf, src, u_stage, u_base, beta,
+      bdt,
       u_out, prim_out, temp, mu, csig, fsig, fail, iters, partial != nullptr);
   if (partial) {
     const double m = block_min(v);
@@ -143,8 +199,8 @@ __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_FMKS ? 4 : 1))) void k_pack_
   const PackEntry &E = v.e[b];
   const double bdt = beta * (dt_dev ? dt_dev[0] : dt);
   const long n = E.bg.ncell;
-  const double val = update_con2prim_body<true, GEOM, GEOM == PHX_GEOM_FMKS>(
-      p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, n), contig(E.src, n),
+  const double val = update_con2prim_body<true, GEOM, GEOM == PHX_GEOM_FMKS, 3, true>(
+      p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, n), E.emf, contig(E.src, n),
       contig(E.u_stage, n), contig(E.u_base, n), beta, bdt, contig(E.u_out, n),
       contig(E.prim_out, n), E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
       partial != nullptr, do_c2p != 0);
@@ -247,9 +303,9 @@ static int fluid_stage_launches(phx_ctx *ctx, const phx_fluid_params *p, const p
                   f->face_sig, s, ctx->tile_for(p, b, d), flux_fast_of(p));
     ctx->mark_end(s);
   }
-  if (p->mhd && b->ndim > 1) {
+  if (p->mhd && b->ndim > 1) {  // FluxCT: the EMFs here, the B-flux rewrite in the update
     ctx->mark_begin("flux_ct", s);
-    launch_flux_ct(bg, flux, f->emf, s);
+    launch_ct_emf(bg, flux, f->emf, s);
     ctx->mark_end(s);
   }
   if (curved) {
@@ -261,7 +317,7 @@ static int fluid_stage_launches(phx_ctx *ctx, const phx_fluid_params *p, const p
   const ConsPtrs src = curved ? src_c : ConsPtrs::contiguous(nullptr, 0);
   const double bdt = a->beta * a->dt;
   ctx->mark_begin("update_con2prim", s);
-#define ARGS dp, bg, flux, src, cons, u_base, a->beta, bdt, u_out, prim_out, \
+#define ARGS dp, bg, flux, f->emf, src, cons, u_base, a->beta, bdt, u_out, prim_out, \
              f->temp, f->c2p_mu, f->cell_sig, f->face_sig, f->fail, a->iters, partial
   with_geom(p->geom.kind, [&](auto G) {
     with_mhd(p->mhd, [&](auto H) {

@@ -207,6 +207,9 @@ def test_full_torus_pack_stage_properties(phx, torus):
                            DeviceArray(st.cell_sig.shape))
         u_div, u_tmp, scratch_sig = scratch[sh]
         u_tmp.fill_from(st.cons)
+        # the pack forms FluxCT's B fluxes from the corner EMFs and leaves the Riemann B
+        # fluxes in flux: apply FluxCT to them first
+        phx.flux_ct(p, b, st.fields())
         phx.flux_divergence_update(p, b, st.flux, st.src, u_tmp, st.cons, 1.0, dt, u_div)
         synchronize()
         U0, U1, Ud = st.cons.get(), c1.get(), u_div.get()

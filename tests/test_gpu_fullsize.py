@@ -2,8 +2,8 @@
 do not depend on the size (the oracle comparisons run on small blocks in test_gpu_parity):
 the flux divergence telescopes (the sum of U_{s+1} - U_s over the block is the net flux
 through its boundary; checked on the update before the inversion, phx_flux_divergence_update
-on the stage's own fluxes, which the fused stage's U_{s+1} equals bit for bit wherever the
-inversion did not re-derive it), the inverted primitives give U_{s+1} back through p2c to
+on the stage's own fluxes after phx_flux_ct, which the fused stage's U_{s+1} equals bit for
+bit wherever the inversion did not re-derive it), the inverted primitives give U_{s+1} back through p2c to
 the solver's tolerance, and the CFL time step is the minimum over cells recomputed from the
 stage's own signal speeds, bit for bit."""
 import sys
@@ -56,7 +56,12 @@ def test_fullsize_stage_properties(phx):
     u1, flux = u_out.get(), st.flux.get()
     I = (slice(NG, NG + N),) * 3
     lo, hi = NG, NG + N  # first interior index, first index past the interior
-    # the update alone on the stage's fluxes (after CT): U_{s+1} before the inversion
+    # the update alone on the stage's fluxes after FluxCT (the fused stage forms CT's B
+    # fluxes from the corner EMFs and leaves the Riemann B fluxes in flux): U_{s+1} before
+    # the inversion
+    phx.flux_ct(p, b, st.fields())
+    synchronize()
+    flux = st.flux.get()
     d_us, d_ub = DeviceArray.from_host(u0), DeviceArray.from_host(u0)
     u_sep = DeviceArray(st.cons.shape)
     phx.flux_divergence_update(p, b, st.flux, None, d_us, d_ub, 1.0, dt, u_sep)
