@@ -464,6 +464,13 @@ def c4_uneven_pack(phx, args, log):
     ms_graph = timed(lambda: g.launch(s))
     fail_graph = interior_fails()
 
+    def interior_iters():
+        it_h = it_arena.get()
+        return np.concatenate([it_h[int(offs[i]):int(offs[i + 1])].reshape(shapes[i])[NG:-NG, NG:-NG, NG:-NG].ravel()
+                               for i in range(len(blocks))])
+
+    its = interior_iters()  # those of the graph steps (the learned run repeats them)
+
     def eager():
         pk.stage(1, dt_dev=dt_dev, cfl=0.4, stream=s)
         pk.stage(2, dt_dev=dt_dev, cfl=0.4, dt_out=dt_dev, stream=s)
@@ -486,13 +493,6 @@ def c4_uneven_pack(phx, args, log):
 
     ms_blocks = timed(per_block) if args.c4_per_block else None
     g.close()
-
-    def interior_iters():
-        it_h = it_arena.get()
-        return np.concatenate([it_h[int(offs[i]):int(offs[i + 1])].reshape(shapes[i])[NG:-NG, NG:-NG, NG:-NG].ravel()
-                               for i in range(len(blocks))])
-
-    its = interior_iters()
     learned = None
     if not args.no_c4_learned:
         # the learned bracket: pairs (conserved state, plain root, (X1, X2)) from the
@@ -528,6 +528,27 @@ def c4_uneven_pack(phx, args, log):
         misses, solves = pk.learned_stats()
         its_l = interior_iters()
         gl.close()
+
+        def cold_stage(with_net):
+            # stage 1 from the same state with no previous roots (mu = 0, as after the
+            # problem generator): the plain cold solve or the network's bracket; the second
+            # of two runs is timed
+            pk.set_learned(net if with_net else None)
+            for _ in range(2):
+                restore()
+                arenas["mu"][0].zero()
+                dt_dev.upload(dt0)
+                pk.learned_stats(reset=True)
+                e0.record(s)
+                pk.stage(1, dt_dev=dt_dev, cfl=0.4, stream=s)
+                e1.record(s)
+                phx.L.phx_rt_stream_synchronize(s)
+            cm, cs_ = pk.learned_stats()
+            ci = interior_iters()
+            return {"ms": e0.elapsed_ms(e1), "con2prim_mean_iters": float(np.mean(np.maximum(ci, 0))),
+                    "network_solves": cs_, "bracket_misses": cm}
+
+        cold = {"plain": cold_stage(False), "learned": cold_stage(True)}
         pk.set_learned(None)
         learned = {"pairs": int(data["mu"].size), "train_s": train_s,
                    "loss_first_last": [losses[0], losses[-1]],
@@ -535,9 +556,14 @@ def c4_uneven_pack(phx, args, log):
                    "zone_updates_per_s_graph": 2 * interior / (ms_learned * 1e-3),
                    "con2prim_mean_iters": float(np.mean(np.maximum(its_l, 0))),
                    "con2prim_hist": np.bincount(np.maximum(its_l, 0)).tolist(),
-                   "bracket_miss_rate": misses / max(solves, 1)}
+                   # every cell of the timed steps has a previous root: the update inverts
+                   # it from there as the plain stage does and the network brackets nothing
+                   "network_solves": solves, "bracket_misses": misses,
+                   "cold_first_stage": cold}
         log(f"c4: learned graph {ms_learned:.3f} ms/step, iters {learned['con2prim_mean_iters']:.2f}"
-            f" vs plain {float(np.mean(np.maximum(its, 0))):.2f}, misses {misses}/{solves}")
+            f" vs plain {float(np.mean(np.maximum(its, 0))):.2f}, misses {misses}/{solves}; "
+            f"cold stage 1: plain {cold['plain']['ms']:.3f} ms {cold['plain']['con2prim_mean_iters']:.2f} "
+            f"iters, learned {cold['learned']['ms']:.3f} ms {cold['learned']['con2prim_mean_iters']:.2f} iters")
     phx.L.phx_rt_stream_destroy(s)
     pk.close()
     zu = 2 * interior
@@ -975,6 +1001,11 @@ def c2_stage(phx, args, recon_name, prim_h, step, full=True, arith=None):
         "magnetization": {"sigma_percentiles_1_10_50_90_99":
                           np.percentile(sig, [1, 10, 50, 90, 99]).tolist()},
         "con2prim": {"mean_iters_warm": float(np.mean(warm_iters)),
+                     # a wavefront runs its slowest lane's iterations: the mean over the
+                     # update's waves (64 consecutive interior cells) of the largest count,
+                     # the headroom a straggler compaction could recover
+                     "mean_wave_max_iters_warm": float(np.mean(np.max(
+                         np.maximum(warm_iters, 0).reshape(-1, 64), axis=1))),
                      "mean_iters_cold": float(np.mean(cold_iters)),
                      "hist_warm": np.bincount(np.maximum(warm_iters, 0).ravel()).tolist(),
                      "hist_cold": np.bincount(np.maximum(cold_iters, 0).ravel()).tolist(),

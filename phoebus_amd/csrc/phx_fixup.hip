@@ -393,8 +393,15 @@ __global__ __launch_bounds__(256) void k_pack_floors_apply(DevParams p, DevFixup
   }
 }
 
-__global__ void k_zero_count(unsigned int *count) {
-  if (threadIdx.x == 0) *count = 0u;
+__global__ void k_zero_count(unsigned int *count, double *extra) {
+  if (threadIdx.x == 0) {
+    *count = 0u;
+    if (extra) *extra = DBL_MAX;
+  }
+}
+
+void launch_zero_count(unsigned int *count, hipStream_t s, double *extra) {
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, count, extra);
 }
 
 void launch_pack_c2p_fixup(int geom, const DevParams &dp, const DevFixup &fx, const PackView &v,
@@ -412,7 +419,7 @@ void launch_pack_floors(int geom, const DevParams &dp, const DevFixup &fx, const
   if (groups <= 0 || (!fx.floors && !partial)) return;
   const FloorQueue qq = fx.floors && q.cell && q.cap >= groups * 256 ? q : FloorQueue{};
   if (fx.floors && !qq.cell) return;  // the owner sizes the queue: never reached
-  if (qq.cell) hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, qq.count);
+  if (qq.cell) launch_zero_count(qq.count, s);
   with_geom(geom, [&](auto G) {
     hipLaunchKernelGGL((k_pack_floors<decltype(G)::value>), dim3((unsigned)groups), dim3(256),
                        0, s, dp, fx, v, entire ? 1 : 0, partial, qq);

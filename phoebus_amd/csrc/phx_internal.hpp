@@ -195,15 +195,52 @@ void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &
 void launch_pack_ct_emf(const PackView &v, long groups, hipStream_t s);
 void launch_pack_sources(int geom, const DevParams &dp, const PackView &v, long groups,
                          hipStream_t s);
-// do_c2p = false: U_{s+1} only (the inversion then runs in a learned-bracket pass)
+// The cells a plain con2prim pass leaves to the learned bracket: those with no previous
+// root (mu <= 0: initial data, fresh cells).  Entries are (block << 32) | (keep << 31) |
+// cell, keep = 1 for a cell whose iteration count is not to be written (an interior cell
+// of an entire-block re-inversion); count is cleared (launch_zero_count) before the
+// producing launch.  A cell that finds the queue full is inverted by the plain pass.
+struct ColdQueue {
+  unsigned long long *cell = nullptr;
+  unsigned int *count = nullptr;
+  long cap = 0;
+};
+// called by every active lane of a wave: queues this lane's cell when cold; true when the
+// cell is queued (the learned pass inverts it)
+__device__ __forceinline__ bool enqueue_cold(const ColdQueue &q, bool cold, int blk, long c,
+                                             bool keep) {
+  const unsigned long long mask = __ballot(cold);
+  if (!mask) return false;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned int base = 0;
+  if (lane == leader) base = atomicAdd(q.count, (unsigned int)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+  if (!cold || slot >= q.cap) return false;
+  q.cell[slot] = ((unsigned long long)blk << 32) | ((unsigned long long)keep << 31) |
+                 (unsigned long long)c;
+  return true;
+}
+// clears a queue's count and, when extra is set, sets *extra = DBL_MAX
+void launch_zero_count(unsigned int *count, hipStream_t s, double *extra = nullptr);
+// which cells the fused update inverts: all (the plain stage), or those with a previous
+// root (mu > 0) to start from, queueing the rest for the learned bracket (cq)
+enum { kC2pAll = 1, kC2pWarm = 2 };
+// c2p_mode == kC2pWarm: the queue is cleared first (and *extra set to DBL_MAX, the
+// partial the learned pass folds its cells' CFL times into); the update's partials then
+// cover the cells it inverts
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
-                        hipStream_t s, bool do_c2p = true);
-// con2prim with the learned bracket over the cells of a pack work list of kind 4
-// (interior), 5 (entire blocks) or 6 (face-adjacent ghost layer) (phx_mlp.hip); counters
-// (nullable) accumulate [bracket misses, solves]
-void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
-                             long groups, int kind, unsigned long long *counters, hipStream_t s);
+                        hipStream_t s, int c2p_mode = kC2pAll, const ColdQueue &cq = ColdQueue{},
+                        double *extra = nullptr);
+// con2prim with the learned bracket over the queued cells of a pack (phx_mlp.hip), in a
+// grid-stride loop over the count the producing pass left; counters (nullable)
+// accumulate [bracket misses, network-bracketed solves]; extra (nullable, interior cells
+// only): the minimum of the cells' CFL times, by an atomic minimum on the bits
+void launch_pack_c2p_cold(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
+                          const ColdQueue &cq, unsigned long long *counters, double *extra,
+                          hipStream_t s);
 // the fixups over a pack (phx_fixup.hip): con2prim failure repair on the interior cells,
 // then floors and ceilings (interior cells, or the entire blocks over an entire-cell work
 // list) folded with the interior CFL partial minima (partial may be null)

@@ -198,8 +198,10 @@ __global__ __launch_bounds__(256) void k_mesh_fix_fluxes(MeshInfo m, PackView v,
 // cells of work-list kind 5 (entire blocks) or 6 (the face-adjacent ghost layer, what the
 // c2p fixup reads); a re-inversion of the entire blocks keeps the interior's iteration
 // counts (those of the stage's inversion of U_{s+1})
+// (cq.cell set: the learned bracket is on, and a cell with no previous root goes to its
+// queue instead)
 template <int GEOM, int KIND>
-__global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v) {
+__global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v, ColdQueue cq) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
@@ -207,6 +209,10 @@ __global__ __launch_bounds__(256) void k_pack_con2prim(DevParams p, PackView v) 
   int idx[3];
   if (!cell_of_kind<KIND>(bg, (g - v.start[b]) * 256 + threadIdx.x, idx)) return;
   int32_t *it = (KIND == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
+  if (cq.cell) {
+    const long c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
+    if (enqueue_cold(cq, !(E.mu[c] > 0.0), b, c, it == nullptr)) return;
+  }
   con2prim_cell<true, GEOM, GEOM == PHX_GEOM_FMKS>(p, bg, idx[0], idx[1], idx[2],
                                                    contig(E.u_out, bg.ncell),
                                                    contig(E.prim_out, bg.ncell),
@@ -302,21 +308,26 @@ void launch_c2p(phx_mesh *m, int st, int kind, hipStream_t s) {
   static const char *kPlain[7] = {"", "", "", "", "con2prim", "con2prim", "con2prim_ghost"};
   static const char *kLearned[7] = {"", "", "", "", "con2prim_learned_first", "con2prim_learned",
                                     "con2prim_learned_ghost"};
-  if (k->mlp) {
-    m->ctx->mark_begin(kLearned[kind], s);
-    launch_pack_c2p_learned(m->p.geom.kind, k->dp, k->mlp, pv(m, st, kind), groups(m, kind),
-                            kind, k->counters, s);
+  // with the learned bracket: the plain pass over the cells with a previous root, queueing
+  // the rest, then the learned pass over the queue (kind 4: the update queued them)
+  const ColdQueue cq = k->mlp ? k->cq : ColdQueue{};
+  if (kind != 4) {
+    m->ctx->mark_begin(kPlain[kind], s);
+    if (cq.cell) launch_zero_count(cq.count, s);
+    with_geom(m->p.geom.kind, [&](auto G) {
+      constexpr int GM = decltype(G)::value;
+      auto kernel = kind == 5 ? k_pack_con2prim<GM, 5> : k_pack_con2prim<GM, 6>;
+      hipLaunchKernelGGL(kernel, dim3((unsigned)groups(m, kind)), dim3(256), 0, s, k->dp,
+                         pv(m, st, kind), cq);
+    });
     m->ctx->mark_end(s);
-    return;
   }
-  m->ctx->mark_begin(kPlain[kind], s);
-  with_geom(m->p.geom.kind, [&](auto G) {
-    constexpr int GM = decltype(G)::value;
-    auto kernel = kind == 5 ? k_pack_con2prim<GM, 5> : k_pack_con2prim<GM, 6>;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)groups(m, kind)), dim3(256), 0, s, k->dp,
-                       pv(m, st, kind));
-  });
-  m->ctx->mark_end(s);
+  if (cq.cell) {
+    m->ctx->mark_begin(kLearned[kind], s);
+    launch_pack_c2p_cold(m->p.geom.kind, k->dp, k->mlp, pv(m, st, kind), cq, k->counters,
+                         nullptr, s);
+    m->ctx->mark_end(s);
+  }
 }
 
 // FillDerived + ConservedToPrimitiveFixup + ApplyFloors after the boundary exchange;
@@ -346,7 +357,7 @@ int run_stage(phx_mesh *m, int st, bool last, hipStream_t s) {
   const bool learned = k->mlp != nullptr;
   ctx->mark_begin(learned ? "update" : "update_con2prim", s);
   launch_pack_update(k->p.geom.kind, k->dp, pv(m, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
-                     m->dt, nullptr, nullptr, s, !learned);
+                     m->dt, nullptr, nullptr, s, learned ? kC2pWarm : kC2pAll, k->cq);
   ctx->mark_end(s);
   // the first FillDerived: the interior (inverted by the update unless the learned bracket
   // is on) and the face-adjacent ghost layer the fixup reads

@@ -659,114 +659,132 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_con2prim_batch_learned(
   }
 }
 
-// con2prim with the learned bracket over the cells of a pack's work list of kind 4
-// (interior), 5 (entire blocks) or 6 (the face-adjacent ghost layer): the cell-field
-// semantics of the plain pass (con2prim_cell: NaN input -> fail, every other field kept;
-// an unbracketed root -> fail with iters -1), the bracket and fallback of
-// k_con2prim_batch_learned
+// con2prim with the learned bracket over the cells a plain pass of a pack queued (those
+// with no previous root: initial data, fresh cells), with the cell-field semantics of the
+// plain pass (con2prim_cell: NaN input -> fail, every other field kept; an unbracketed
+// root -> fail with iters -1) and the bracket and fallback of k_con2prim_batch_learned.
+// A grid-stride loop over the queue's count, so a stage whose cells all have a previous
+// root (a time-dependent run) costs one launch that reads the count and ends; a workgroup
+// with no queued cell loads no weights.
 // (three waves per SIMD fit without spilling on Minkowski blocks only; the curved metrics
 // would spill 22-34 VGPRs)
-template <int GEOM, int KIND>
+constexpr int kColdGrid = 768;  // workgroups: 3 per CU (the LDS of the weights allows 3)
+template <int GEOM>
 __global__ __launch_bounds__(kLearnedThreads)
 __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 3 : 1))) void
-k_pack_con2prim_learned(
-    DevParams p, MlpDev m, PackView v, unsigned long long *counters) {
+k_pack_con2prim_cold(DevParams p, MlpDev m, PackView v, ColdQueue q,
+                     unsigned long long *counters, double *extra) {
   using M = typename MetricOf<GEOM>::type;
   constexpr bool CACHED = GEOM == PHX_GEOM_FMKS;  // every FMKS block of a pack has its geometry
   using L = Layout<true>;
   __shared__ uint64_t wfwd[kFwdU64];
   __shared__ float wbias[kNBias];
-  for (int q = threadIdx.x; q < kFwdU64; q += kLearnedThreads) wfwd[q] = m.fwd[q];
-  for (int q = threadIdx.x; q < kNBias; q += kLearnedThreads) wbias[q] = m.bias[q];
+  const long nq = (long)min((unsigned long long)*q.count, (unsigned long long)q.cap);
+  if ((long)blockIdx.x * kLearnedThreads >= nq) return;  // the whole workgroup: no cell
+  for (int t = threadIdx.x; t < kFwdU64; t += kLearnedThreads) wfwd[t] = m.fwd[t];
+  for (int t = threadIdx.x; t < kNBias; t += kLearnedThreads) wbias[t] = m.bias[t];
   __syncthreads();
   MlpDev ml = m;
   ml.fwd = wfwd;
   ml.bias = wbias;
-  const long g = xcd_remap(blockIdx.x, gridDim.x);
-  const int b = pack_find(v.start, v.n, g);
-  const PackEntry &E = v.e[b];
-  const BlockGeom &bg = E.bg;
-  int idx[3] = {bg.is[0], bg.is[1], bg.is[2]};
-  const bool have = cell_of_kind<KIND>(bg, (g - v.start[b]) * kLearnedThreads + threadIdx.x, idx);
-  // a re-inversion of the entire blocks keeps the interior's iteration counts
-  int32_t *iters = (KIND == 5 && is_interior(bg, idx)) ? nullptr : E.iters;
-  const long n = bg.ncell;
-  const long c = ((long)idx[2] * bg.nj + idx[1]) * bg.ni + idx[0];
-  // phase 1: features of this lane's cell
-  float f[4] = {0.f, 0.f, 0.f, 0.f};
-  if (have) {
+  // the loop's trip count is the same for every lane of the workgroup: the network is
+  // evaluated by whole wavefronts
+  for (long base = (long)blockIdx.x * kLearnedThreads; base < nq;
+       base += (long)gridDim.x * kLearnedThreads) {
+    const long e = base + threadIdx.x;
+    const bool have = e < nq;
+    const unsigned long long w = have ? q.cell[e] : 0ull;
+    const PackEntry &E = v.e[(int)(w >> 32)];
+    const BlockGeom &bg = E.bg;
+    const int ci = (int)(w & 0x7fffffffull);  // cell index (< 2^31)
+    const long c = ci;
+    const bool keep = (w >> 31) & 1ull;
+    const int idx[3] = {ci % bg.ni, (ci / bg.ni) % bg.nj, ci / (bg.ni * bg.nj)};
+    int32_t *iters = keep ? nullptr : E.iters;
+    const long n = bg.ncell;
+    // phase 1: features of this lane's cell
+    float f[4] = {0.f, 0.f, 0.f, 0.f};
+    if (have) {
+      const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
+      const M gm = metric_at<GEOM, CACHED>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
+      double cs[L::ncons], pr[L::nprim];
+#pragma unroll
+      for (int t = 0; t < L::ncons; ++t) cs[t] = E.u_out[t * n + c];
+      C2PState z;
+      if (c2p_setup<true>(p, gm, X, cs, pr, z)) zone_features(ml, z.res, f);
+    }
+    // phase 2: the network for the wave's 64 cells
+    float o1, o2;
+    mlp_predict_wave(ml, f, o1, o2);
+    if (!have) continue;
+    // phase 3: the set-up again, then the bracketed solve
+    double *cons = E.u_out, *prim = E.prim_out;
+    double cs[L::ncons], pr[L::nprim], sig[3];
+#pragma unroll
+    for (int t = 0; t < L::ncons; ++t) cs[t] = cons[t * n + c];
+    if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
+      E.fail[c] = 0.0;
+      if (iters) iters[c] = 0;
+      continue;
+    }
     const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
     const M gm = metric_at<GEOM, CACHED>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
-    double cs[L::ncons], pr[L::nprim];
 #pragma unroll
-    for (int q = 0; q < L::ncons; ++q) cs[q] = E.u_out[q * n + c];
+    for (int t = 0; t < L::nprim; ++t) pr[t] = prim[t * n + c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sig[d] = d < bg.ndim ? E.csig[d * n + c] : 0.0;
+    double tt = E.temp ? E.temp[c] : 0.0, root = E.mu[c];
     C2PState z;
-    if (c2p_setup<true>(p, gm, X, cs, pr, z)) zone_features(ml, z.res, f);
-  }
-  // phase 2: the network for the wave's 64 cells
-  float o1, o2;
-  mlp_predict_wave(ml, f, o1, o2);
-  if (!have) return;
-  // phase 3: the set-up again, then the bracketed solve
-  double *cons = E.u_out, *prim = E.prim_out;
-  double cs[L::ncons], pr[L::nprim], sig[3];
+    c2p_setup<true>(p, gm, X, cs, pr, z);
+    const double cc = exp((double)o1) / (double)prior_z0(ml, f[0], f[2]);
+    const double wd = cc * fmax((double)ml.width_scale * exp((double)o2), (double)ml.width_min);
+    const double lo = fmax(cc - wd, 0.0), hi = fmin(cc + wd, 1.0);
+    const double ylo = z.res(lo), yhi = z.res(hi);
+    int it = 0;
+    bool good = true, fb = false;
+    double mu_new = root;
+    if (ylo * yhi <= 0.0) {
+      mu_new = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
+    } else {  // the plain solve on [0, 1], split at the network's centre
+      fb = true;
+      good = regula_falsi(z.res, 0.0, 1.0, p.tol, cc, p.max_iter, mu_new, it);
+    }
+    double st = 0.0;
+    if (good) {
+      root = mu_new;
+      st = c2p_unwrap<true>(p, gm, z, root, cs, pr, tt, sig);
+    } else {
+      it = -1;
+    }
+    if (counters) {  // [0] bracket misses, [1] network-bracketed solves
+      if (fb) atomicAdd(&counters[0], 1ull);
+      const unsigned long long act = __ballot(1);
+      if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1)
+        atomicAdd(&counters[1], (unsigned long long)__popcll(act));
+    }
 #pragma unroll
-  for (int q = 0; q < L::ncons; ++q) cs[q] = cons[q * n + c];
-  if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
-    E.fail[c] = 0.0;
-    if (iters) iters[c] = 0;
-    return;
-  }
-  const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
-  const M gm = metric_at<GEOM, CACHED>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
+    for (int t = 0; t < 5; ++t) cons[t * n + c] = cs[t];
 #pragma unroll
-  for (int q = 0; q < L::nprim; ++q) pr[q] = prim[q * n + c];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) sig[d] = d < bg.ndim ? E.csig[d * n + c] : 0.0;
-  double tt = E.temp ? E.temp[c] : 0.0, root = E.mu[c];
-  C2PState z;
-  c2p_setup<true>(p, gm, X, cs, pr, z);
-  const double cc = exp((double)o1) / (double)prior_z0(ml, f[0], f[2]);
-  const double w = cc * fmax((double)ml.width_scale * exp((double)o2), (double)ml.width_min);
-  const double lo = fmax(cc - w, 0.0), hi = fmin(cc + w, 1.0);
-  const double ylo = z.res(lo), yhi = z.res(hi);
-  int it = 0;
-  bool good = true, fb = false;
-  double mu_new = root;
-  if (ylo * yhi <= 0.0) {
-    mu_new = regula_falsi_bracketed(z.res, lo, hi, ylo, yhi, p.tol, p.max_iter, it);
-  } else {
-    fb = true;
-    // the plain solve with the plain solver's own guess, the cell's previous root, when it
-    // has one (a time-dependent run), else the network's centre
-    good = regula_falsi(z.res, 0.0, 1.0, p.tol, root > 0.0 ? root : cc, p.max_iter, mu_new, it);
+    for (int t = 0; t < L::nprim; ++t) prim[t * n + c] = pr[t];
+    for (int d = 0; d < bg.ndim; ++d) E.csig[d * n + c] = sig[d];
+    if (E.temp) E.temp[c] = tt;
+    E.mu[c] = root;
+    E.fail[c] = st;
+    if (iters) iters[c] = it;
+    if (extra) {  // the cell's CFL time with face speeds, as the update forms it
+      const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
+      double ldt = 0.0;
+      for (int d = 0; d < bg.ndim; ++d) {
+        const double s = smax(sig[d], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
+        ldt += s / bg.dx[d];
+      }
+      const double t = 1.0 / (ldt + 1.e-50);
+      if (!isnan(t))  // positive doubles order as their bit patterns
+        atomicMin(reinterpret_cast<unsigned long long *>(extra),
+                  (unsigned long long)__double_as_longlong(t));
+    }
   }
-  double st = 0.0;
-  if (good) {
-    root = mu_new;
-    st = c2p_unwrap<true>(p, gm, z, root, cs, pr, tt, sig);
-  } else {
-    it = -1;
-  }
-  if (counters) {
-    if (fb) atomicAdd(&counters[0], 1ull);
-    const unsigned long long act = __ballot(1);
-    if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1)
-      atomicAdd(&counters[1], (unsigned long long)__popcll(act));
-  }
-#pragma unroll
-  for (int q = 0; q < 5; ++q) cons[q * n + c] = cs[q];
-#pragma unroll
-  for (int q = 0; q < L::nprim; ++q) prim[q * n + c] = pr[q];
-  for (int d = 0; d < bg.ndim; ++d) E.csig[d * n + c] = sig[d];
-  if (E.temp) E.temp[c] = tt;
-  E.mu[c] = root;
-  E.fail[c] = st;
-  if (iters) iters[c] = it;
 }
-
-void launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m, const PackView &v,
-                             long groups, int kind, unsigned long long *counters, hipStream_t s);
 
 // features of batched states (training data and tests)
 template <bool MHD, int GEOM>
@@ -824,18 +842,14 @@ static MlpDev dev_view(const phx_mlp *m) {
   return d;
 }
 
-void phx::launch_pack_c2p_learned(int geom, const DevParams &dp, const phx_mlp *m,
-                                  const PackView &v, long groups, int kind,
-                                  unsigned long long *counters, hipStream_t s) {
-  if (groups <= 0) return;
+void phx::launch_pack_c2p_cold(int geom, const DevParams &dp, const phx_mlp *m,
+                               const PackView &v, const ColdQueue &cq,
+                               unsigned long long *counters, double *extra, hipStream_t s) {
+  if (!cq.cell) return;
   const MlpDev md = dev_view(m);
   with_geom(geom, [&](auto G) {
-    constexpr int GM = decltype(G)::value;
-    auto kernel = kind == 4   ? k_pack_con2prim_learned<GM, 4>
-                  : kind == 5 ? k_pack_con2prim_learned<GM, 5>
-                              : k_pack_con2prim_learned<GM, 6>;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)groups), dim3(kLearnedThreads), 0, s, dp, md, v,
-                       counters);
+    hipLaunchKernelGGL((k_pack_con2prim_cold<decltype(G)::value>), dim3(kColdGrid),
+                       dim3(kLearnedThreads), 0, s, dp, md, v, cq, counters, extra);
   });
 }
 

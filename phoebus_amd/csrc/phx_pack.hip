@@ -86,22 +86,26 @@ void launch_stage_fluxes(phx_ctx *ctx, phx_pack *k, int st, hipStream_t s) {
 void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipStream_t s) {
   const int st = a->stage;
   launch_stage_fluxes(ctx, k, st, s);
-  // with fixups on or the learned inversion, the CFL partials are taken after them (the
-  // repaired and floored cells' signal speeds), in the floors launch
+  // with fixups on, the CFL partials are taken after them (the repaired and floored cells'
+  // signal speeds), in the floors launch; with the learned inversion and no fixups, the
+  // update takes those of the cells it inverts and the learned pass folds its cells into
+  // one more partial (partial[groups[4]])
   const bool learned = k->mlp != nullptr;
   const bool fixups = k->fx.c2p_fixup || k->fx.floors;
   double *partial = a->dt_out ? k->partial : nullptr;
+  double *extra = (learned && partial && !fixups) ? partial + k->groups[4] : nullptr;
   ctx->mark_begin(learned ? "update" : "update_con2prim", s);
   launch_pack_update(k->p.geom.kind, k->dp, view(k, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
-                     a->dt, a->dt_dev, (fixups || learned) ? nullptr : partial, s, !learned);
+                     a->dt, a->dt_dev, fixups ? nullptr : partial, s,
+                     learned ? kC2pWarm : kC2pAll, k->cq, extra);
   ctx->mark_end(s);
-  if (learned) {
+  if (learned) {  // the cells the update queued (no previous root)
     ctx->mark_begin("con2prim_learned", s);
-    launch_pack_c2p_learned(k->p.geom.kind, k->dp, k->mlp, view(k, st, 4), k->groups[4], 4,
-                            k->counters, s);
+    launch_pack_c2p_cold(k->p.geom.kind, k->dp, k->mlp, view(k, st, 4), k->cq, k->counters,
+                         extra, s);
     ctx->mark_end(s);
   }
-  if (fixups || learned) {
+  if (fixups) {
     if (k->fx.c2p_fixup) {
       ctx->mark_begin("c2p_fixup", s);
       launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, view(k, st, 4), k->groups[4], s);
@@ -114,7 +118,7 @@ void launch_stage(phx_ctx *ctx, phx_pack *k, const phx_pack_stage_args *a, hipSt
   }
   if (a->dt_out) {
     ctx->mark_begin("dt_finish", s);
-    launch_dt_finish(k->partial, (int)k->groups[4], a->cfl, a->dt_out, s);
+    launch_dt_finish(k->partial, (int)k->groups[4] + (extra ? 1 : 0), a->cfl, a->dt_out, s);
     ctx->mark_end(s);
   }
 }
@@ -208,7 +212,7 @@ int phx_pack_create(phx_ctx *ctx, const phx_fluid_params *p, int nblocks,
          hipMemcpy(k->entries[st], e[st].data(), sizeof(PackEntry) * nblocks,
                    hipMemcpyHostToDevice) == hipSuccess;
   for (int kind = 0; kind < 8 && ok; ++kind) ok = build_start(k, kind) == PHX_OK;
-  ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max({k->groups[4], k->groups[5], 1L})) ==
+  ok = ok && hipMalloc(&k->partial, sizeof(double) * std::max({k->groups[4] + 1, k->groups[5], 1L})) ==
                  hipSuccess;
   ok = ok && hipMalloc(&k->counters, 2 * sizeof(unsigned long long)) == hipSuccess &&
        hipMemset(k->counters, 0, 2 * sizeof(unsigned long long)) == hipSuccess;
@@ -234,6 +238,8 @@ int phx_pack_destroy(phx_pack *k) {
   if (k->fq.cell) (void)hipFree(k->fq.cell);
   if (k->fq.grp) (void)hipFree(k->fq.grp);
   if (k->fq.count) (void)hipFree(k->fq.count);
+  if (k->cq.cell) (void)hipFree(k->cq.cell);
+  if (k->cq.count) (void)hipFree(k->cq.count);
   for (double *g : k->geo) (void)hipFree(g);
   delete k;
   return PHX_OK;
@@ -277,6 +283,18 @@ int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *k, const phx_fixup_params *fx) {
 
 int phx_pack_set_learned(phx_ctx *ctx, phx_pack *k, const phx_mlp *mlp) {
   if (!ctx || !k) return PHX_ERR_INVALID;
+  if (mlp && !k->cq.cell) {  // the queue of cells to bracket: room for every cell of the pack
+    long cells = 0;
+    for (const BlockGeom &g : k->geoms) cells += g.ncell;
+    ColdQueue q;
+    q.cap = cells;
+    if (hipMalloc(&q.cell, sizeof(unsigned long long) * cells) != hipSuccess ||
+        hipMalloc(&q.count, sizeof(unsigned int)) != hipSuccess) {
+      if (q.cell) (void)hipFree(q.cell);
+      return ctx->fail(PHX_ERR_NOMEM, "learned-bracket queue");
+    }
+    k->cq = q;
+  }
   k->mlp = mlp;
   return PHX_OK;
 }

@@ -33,6 +33,7 @@ struct phx_pack {
   const phx_mlp *mlp = nullptr;          // learned con2prim bracket (null: plain solver)
   std::vector<double *> geo;             // per block: cached geometry plane (FMKS) or empty
   unsigned long long *counters = nullptr;  // device [2]: bracket misses, learned solves
+  phx::ColdQueue cq;  // cells left to the learned bracket, allocated by the first set_learned
   phx::FloorQueue fq;  // the floors' repair queue, allocated when floors are set on
   // live graphs captured from this pack: they hold its work-list pointers, grid sizes and
   // tile shapes, so autotuning (which rewrites the work lists) and destruction wait for

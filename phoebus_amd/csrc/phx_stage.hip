@@ -83,7 +83,8 @@ __device__ __forceinline__ double update_con2prim_body(
     const CP &src, const CP &u_stage, const CP &u_base, double beta, double bdt,
     const CP &u_out, const PP &prim_out, double *__restrict__ temp,
     double *__restrict__ mu, double *__restrict__ csig, const double *__restrict__ fsig,
-    double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt, bool do_c2p = true) {
+    double *__restrict__ fail, int32_t *__restrict__ iters, bool with_dt,
+    int c2p_mode = kC2pAll, const ColdQueue &cq = ColdQueue{}, int blk = 0) {
   constexpr int NC = Layout<MHD>::ncons;
   const int ni = b.ie[0] - b.is[0] + 1, nj = b.ie[1] - b.is[1] + 1, nk = b.ie[2] - b.is[2] + 1;
   const bool active = t < (long)ni * nj * nk;
@@ -126,7 +127,8 @@ __device__ __forceinline__ double update_con2prim_body(
     // or ceiling (con2prim_fields); no conserved variable is live across the solve
 #pragma unroll
     for (int l = 0; l < NC; ++l) u_out[l][c] = cs[l];
-    if (!do_c2p) return DBL_MAX;  // the inversion runs in a separate (learned-bracket) pass
+    // kC2pWarm: a cell with no previous root to start from goes to the learned bracket
+    if (c2p_mode == kC2pWarm && enqueue_cold(cq, !(mu[c] > 0.0), blk, c, false)) return DBL_MAX;
     // ConservedToPrimitive on U_{s+1}
     double sig[3] = {0.0, 0.0, 0.0};
     int it = 0;
@@ -193,7 +195,8 @@ template <int GEOM>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_FMKS ? 4 : 1))) void k_pack_update(DevParams p, PackView v, double beta,
                                                      double dt, const double *dt_dev,
-                                                     double *__restrict__ partial, int do_c2p) {
+                                                     double *__restrict__ partial, int c2p_mode,
+                                                     ColdQueue cq) {
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
@@ -203,7 +206,7 @@ __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_FMKS ? 4 : 1))) void k_pack_
       p, E.bg, (g - v.start[b]) * 256 + threadIdx.x, contig(E.flux, n), E.emf, contig(E.src, n),
       contig(E.u_stage, n), contig(E.u_base, n), beta, bdt, contig(E.u_out, n),
       contig(E.prim_out, n), E.temp, E.mu, E.csig, E.fsig, E.fail, E.iters,
-      partial != nullptr, do_c2p != 0);
+      partial != nullptr, c2p_mode, cq, b);
   if (partial) {
     const double m = block_min(val);
     if (threadIdx.x == 0) partial[g] = m;
@@ -212,11 +215,12 @@ __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_FMKS ? 4 : 1))) void k_pack_
 
 void launch_pack_update(int geom, const DevParams &dp, const PackView &v, long groups,
                         double beta, double dt, const double *dt_dev, double *partial,
-                        hipStream_t s, bool do_c2p) {
+                        hipStream_t s, int c2p_mode, const ColdQueue &cq, double *extra) {
   if (groups <= 0) return;
+  if (c2p_mode == kC2pWarm) launch_zero_count(cq.count, s, extra);
   with_geom(geom, [&](auto G) {
     hipLaunchKernelGGL((k_pack_update<decltype(G)::value>), dim3((unsigned)groups), dim3(256), 0,
-                       s, dp, v, beta, dt, dt_dev, do_c2p ? partial : nullptr, do_c2p ? 1 : 0);
+                       s, dp, v, beta, dt, dt_dev, partial, c2p_mode, cq);
   });
 }
 

@@ -1,9 +1,10 @@
 """The learned con2prim bracket inside the stage (phx_pack_set_learned, phx_mesh_set_learned):
-the update and a fused network + bracketed-solve pass replace the fused update + plain
-inversion.  The solver still runs to the same tolerance, so primitives agree with the
-plain path to what that tolerance allows and the fail flags are identical, whether the
-bracket holds the root (a bracket scale of 1e6 on Minkowski states) or misses it (a
-scale of 1e-9: the plain [0, 1] solve runs)."""
+the update inverts the cells that have a previous root (as the plain stage does) and a
+network + bracketed-solve pass inverts the rest.  The solver still runs to the same
+tolerance, so primitives agree with the plain path to what that tolerance allows and the
+fail flags are identical, whether the bracket holds the root (a bracket scale of 1e6 on
+Minkowski states) or misses it (a scale of 1e-9: the plain [0, 1] solve runs); cells with a
+previous root get the plain path's results bit for bit."""
 import sys
 
 import numpy as np
@@ -52,14 +53,16 @@ def test_pack_learned_stage_matches_plain(phx, width):
     S = Setup(phx, p, blocks, rng, geom)
     pk = Pack(phx, p, [(b, st, c1, it) for b, st, c1, it in zip(blocks, S.states, S.cons1, S.iters)])
     S.reset()
-    pk.stage(1, dt=1e-3)
+    d_dtp = DeviceArray((1,))
+    pk.stage(1, dt=1e-3, cfl=0.4, dt_out=d_dtp)
     synchronize()
     plain = S.snapshot()
     net = _net(phx, 1e6, 1e-9) if width == "wide" else _net(phx, 1e-9, 1e-12)
     pk.set_learned(net)
     pk.learned_stats(reset=True)
     S.reset()
-    pk.stage(1, dt=1e-3)
+    d_dtl = DeviceArray((1,))
+    pk.stage(1, dt=1e-3, cfl=0.4, dt_out=d_dtl)
     synchronize()
     learned = S.snapshot()
     misses, solves = pk.learned_stats()
@@ -75,6 +78,33 @@ def test_pack_learned_stage_matches_plain(phx, width):
         ok, err = _close(prim_l, prim_p, 1e-6)
         assert ok, f"block {n}: primitives differ by {err}"
         assert np.array_equal(a[4], b[4]), f"block {n}: fail flags"
+    # the time step from the signal speeds of the learned pass's cells: those of roots
+    # within the tolerance of the plain ones
+    dtl, dtp = float(d_dtl.get()[0]), float(d_dtp.get()[0])
+    assert abs(dtl - dtp) <= 1e-6 * dtp, (dtl, dtp)
+    # stage 2 starts from stage 1's roots: every interior cell has one, the update inverts
+    # it as the plain stage does and the network brackets nothing
+    d_dt = DeviceArray((1,))
+    S.reset()
+    pk.set_learned(None)
+    pk.stage(1, dt=1e-3)
+    pk.stage(2, dt=1e-3, cfl=0.4, dt_out=d_dt)
+    synchronize()
+    plain2, dt_plain = S.snapshot(), float(d_dt.get()[0])
+    S.reset()
+    pk.set_learned(None)
+    pk.stage(1, dt=1e-3)
+    pk.set_learned(net)
+    pk.learned_stats(reset=True)
+    d_dt.zero()
+    pk.stage(2, dt=1e-3, cfl=0.4, dt_out=d_dt)
+    synchronize()
+    misses, solves = pk.learned_stats()
+    assert solves == 0 and misses == 0
+    assert float(d_dt.get()[0]) == dt_plain
+    for n, (a, b) in enumerate(zip(S.snapshot(), plain2)):
+        for q, name in enumerate(("prim", "cons", "cons1", "mu", "fail", "iters")):
+            assert np.array_equal(a[q], b[q], equal_nan=True), f"block {n}: {name}"
     pk.close()
     net.close()
 
@@ -100,9 +130,9 @@ def test_mesh_learned_matches_plain(phx):
             lx = [int(round((b.xmin[d] - xmin[d]) / dx[d])) for d in range(3)]
             m.set_prim(g, np.ascontiguousarray(prim_g[:, lx[2]:lx[2] + 16, lx[1]:lx[1] + 16,
                                                       lx[0]:lx[0] + 24]))
-        m.initialize()
-        if learned:
+        if learned:  # the initial inversion (no previous roots) uses the network
             m.set_learned(net)
+        m.initialize()
         for _ in range(2):
             m.step()
         out.append([m.get(g, A.DRIVER_PRIM) for g in range(m.nblocks)])
