@@ -75,11 +75,12 @@ def kernel_bytes(nx):
     flux_dir = faces * (10 + 8 + 1) * 8  # 10 primitives in, 8 fluxes + 1 speed out
     return {
         "flux_x": flux_dir, "flux_y": flux_dir, "flux_z": flux_dir,
-        # 6 transverse B fluxes in, 3 EMFs out and back in, 9 B fluxes out
-        "flux_ct": (nx + 1) ** 3 * (6 + 3 + 3 + 9) * 8,
-        # 24 face fluxes, U_s, U^n, mu, 3 face speeds in; U_s, U_{s+1}, 10 prims, T, mu,
-        # fail, 3 cell speeds, int32 iteration count out
-        "update_con2prim": cells * ((24 + 8 + 8 + 1 + 3) + (8 + 8 + 10 + 1 + 1 + 1 + 3)) * 8
+        # FluxCT's EMF pass: 6 transverse B fluxes in, 3 corner EMFs out
+        "flux_ct": (nx + 1) ** 3 * (6 + 3) * 8,
+        # 18 face fluxes (the B fluxes come from the EMFs), 3 EMFs, U_s, U^n, mu, 3 face
+        # speeds in; U_s, U_{s+1}, 10 prims, T, mu, fail, 3 cell speeds, int32 iteration
+        # count out
+        "update_con2prim": cells * ((18 + 3 + 8 + 8 + 1 + 3) + (8 + 8 + 10 + 1 + 1 + 1 + 3)) * 8
                            + cells * 4,
         "dt_finish": 8 * ((cells + 255) // 256),
     }

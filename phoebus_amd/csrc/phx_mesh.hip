@@ -300,34 +300,33 @@ void launch_exchange_and_boundaries(phx_mesh *m, int st, hipStream_t s) {
   ctx->mark_end(s);
 }
 
-// the robust con2prim of a FillDerived over the stage's output container, on the cells of
-// work-list kind 4 (interior; learned bracket only: the plain one is fused into the
-// update), 5 (entire blocks) or 6 (the face-adjacent ghost layer)
-void launch_c2p(phx_mesh *m, int st, int kind, hipStream_t s) {
+// the learned-bracket pass over the cells queued since the queue was last cleared
+void launch_c2p_queued(phx_mesh *m, int st, const char *name, hipStream_t s) {
   phx_pack *k = m->pack;
-  static const char *kPlain[7] = {"", "", "", "", "con2prim", "con2prim", "con2prim_ghost"};
-  static const char *kLearned[7] = {"", "", "", "", "con2prim_learned_first", "con2prim_learned",
-                                    "con2prim_learned_ghost"};
-  // with the learned bracket: the plain pass over the cells with a previous root, queueing
-  // the rest, then the learned pass over the queue (kind 4: the update queued them)
+  m->ctx->mark_begin(name, s);
+  launch_pack_c2p_cold(m->p.geom.kind, k->dp, k->mlp, pv(m, st, 5), k->cq, k->counters, nullptr,
+                       s);
+  m->ctx->mark_end(s);
+}
+
+// the robust con2prim of a FillDerived over the stage's output container, on the cells of
+// work-list kind 5 (entire blocks) or 6 (the face-adjacent ghost layer).  With the learned
+// bracket the cells with no previous root are queued instead and the learned pass runs
+// over the queue; append: the queue already holds this FillDerived's cells of the update
+// (it is not cleared, and the caller runs the learned pass)
+void launch_c2p(phx_mesh *m, int st, int kind, hipStream_t s, bool append = false) {
+  phx_pack *k = m->pack;
   const ColdQueue cq = k->mlp ? k->cq : ColdQueue{};
-  if (kind != 4) {
-    m->ctx->mark_begin(kPlain[kind], s);
-    if (cq.cell) launch_zero_count(cq.count, s);
-    with_geom(m->p.geom.kind, [&](auto G) {
-      constexpr int GM = decltype(G)::value;
-      auto kernel = kind == 5 ? k_pack_con2prim<GM, 5> : k_pack_con2prim<GM, 6>;
-      hipLaunchKernelGGL(kernel, dim3((unsigned)groups(m, kind)), dim3(256), 0, s, k->dp,
-                         pv(m, st, kind), cq);
-    });
-    m->ctx->mark_end(s);
-  }
-  if (cq.cell) {
-    m->ctx->mark_begin(kLearned[kind], s);
-    launch_pack_c2p_cold(m->p.geom.kind, k->dp, k->mlp, pv(m, st, kind), cq, k->counters,
-                         nullptr, s);
-    m->ctx->mark_end(s);
-  }
+  m->ctx->mark_begin(kind == 5 ? "con2prim" : "con2prim_ghost", s);
+  if (cq.cell && !append) launch_zero_count(cq.count, s);
+  with_geom(m->p.geom.kind, [&](auto G) {
+    constexpr int GM = decltype(G)::value;
+    auto kernel = kind == 5 ? k_pack_con2prim<GM, 5> : k_pack_con2prim<GM, 6>;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)groups(m, kind)), dim3(256), 0, s, k->dp,
+                       pv(m, st, kind), cq);
+  });
+  m->ctx->mark_end(s);
+  if (cq.cell && !append) launch_c2p_queued(m, st, "con2prim_learned", s);
 }
 
 // FillDerived + ConservedToPrimitiveFixup + ApplyFloors after the boundary exchange;
@@ -359,10 +358,11 @@ int run_stage(phx_mesh *m, int st, bool last, hipStream_t s) {
   launch_pack_update(k->p.geom.kind, k->dp, pv(m, st, 4), k->groups[4], st == 1 ? 1.0 : 0.5,
                      m->dt, nullptr, nullptr, s, learned ? kC2pWarm : kC2pAll, k->cq);
   ctx->mark_end(s);
-  // the first FillDerived: the interior (inverted by the update unless the learned bracket
-  // is on) and the face-adjacent ghost layer the fixup reads
-  if (learned) launch_c2p(m, st, 4, s);
-  launch_c2p(m, st, 6, s);
+  // the first FillDerived: the interior (inverted by the update) and the face-adjacent
+  // ghost layer the fixup reads; with the learned bracket, the cells of both with no
+  // previous root go through one learned pass
+  launch_c2p(m, st, 6, s, learned);
+  if (learned) launch_c2p_queued(m, st, "con2prim_learned_first", s);
   ctx->mark_begin("c2p_fixup", s);
   launch_pack_c2p_fixup(k->p.geom.kind, k->dp, k->fx, pv(m, st, 4), k->groups[4], s);
   ctx->mark_end(s);
