@@ -107,6 +107,13 @@ __device__ __forceinline__ double update_con2prim_body(
     // source terms exist exactly for the curved metrics.)
     constexpr bool SRC = GEOM != PHX_GEOM_MINKOWSKI;
     double cs[NC], usv[NC];
+    // with beta = 1 and U_s the same container as U^n (stage 1 of the RK2 cycle) the
+    // average 1 U_s + 0 U_s is U_s itself, bit for bit, for every finite or NaN component
+    // (an infinite one gives NaN): such a cell's U_s is not stored back
+    bool identity = beta == 1.0;
+#pragma unroll
+    for (int l = 0; l < NC; ++l) identity = identity && u_stage[l] == u_base[l];
+    bool inf_in = false;
 #pragma unroll
     for (int l = 0; l < NC; ++l) {
       double f[3][2];
@@ -118,11 +125,15 @@ __device__ __forceinline__ double update_con2prim_body(
       if constexpr (SRC) {
         if (l >= 1 && l <= 4) dudt = dudt + src[l][c];
       }
-      usv[l] = beta * u_stage[l][c] + (1 - beta) * u_base[l][c];
+      const double us = u_stage[l][c];
+      usv[l] = beta * us + (1 - beta) * u_base[l][c];
+      inf_in = inf_in || isinf(us);
       cs[l] = usv[l] + bdt * dudt;
     }
+    if (!identity || inf_in) {
 #pragma unroll
-    for (int l = 0; l < NC; ++l) u_stage[l][c] = usv[l];
+      for (int l = 0; l < NC; ++l) u_stage[l][c] = usv[l];
+    }
     // U_{s+1} goes out before the inversion, which re-derives it only where it used a floor
     // or ceiling (con2prim_fields); no conserved variable is live across the solve
 #pragma unroll
