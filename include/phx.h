@@ -95,7 +95,7 @@ typedef struct {
  * EXACT: IEEE division and square root without contraction, rounding as the reference
  * (fluxes bit-identical to the reference's operation order).  FAST: fused multiply-adds
  * and Newton-refined hardware reciprocals and reciprocal square roots in the sweeps only
- * (fluxes within about 1e-15 relative of EXACT, well inside a 1e-12 flux tolerance); every
+ * (fluxes within about 1e-14 relative of EXACT, well inside a 1e-12 flux tolerance); every
  * other kernel is the same in both. */
 enum { PHX_FLUX_ARITH_EXACT = 0, PHX_FLUX_ARITH_FAST = 1 };
 

@@ -2,7 +2,7 @@
 // arithmetic (phx_fluid_params::flux_arith = PHX_FLUX_ARITH_FAST): built with contraction
 // on (-ffp-contract=fast), and the reconstruction, prim_to_flux and Riemann divisions and
 // square roots through Newton-refined hardware reciprocals (phx_device.hpp, frcp / frsqrt /
-// fsqrt).  Fluxes agree with the exact build to about 1e-15 relative; the sweeps issue
+// fsqrt).  Fluxes agree with the exact build to about 1e-14 relative; the sweeps issue
 // fewer VALU instructions.
 #define PHX_FLUX_FAST 1
 #include <hip/hip_runtime.h>

@@ -2,7 +2,7 @@
 // launchers, compiled once per flux arithmetic: phx_flux.hip includes this with
 // PHX_FLUX_NS = flux_exact (IEEE division and square root, contraction off: bit-identical
 // to the reference's rounding), phx_flux_fast.hip with PHX_FLUX_NS = flux_fast and
-// PHX_FLUX_FAST = 1 (Newton-refined reciprocals and contraction; within a few ulp).  The
+// PHX_FLUX_FAST = 1 (Newton-refined reciprocals and contraction; within ~1e-14).  The
 // namespace keeps the two builds' kernels and launchers apart.
 #pragma once
 

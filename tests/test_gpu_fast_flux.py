@@ -3,7 +3,8 @@ reciprocals in the sweeps) against the oracle and against the exact arithmetic.
 
 Bar: fluxes and face signal speeds within 1e-12 of the oracle relative to the largest
 magnitude of each flux component over the block (the flux tolerance of the reference
-comparison; the differences are a few ulp of the operands); every y/z sweep shape gives the
+comparison; the differences are below 5e-14: one Newton step leaves reciprocals and square
+roots within 2.2e-15 and 4.3e-15 of IEEE); every y/z sweep shape gives the
 same bits as every other (the same fast arithmetic per face); the fused stage and a block
 pack in the fast arithmetic keep the fail flags of the exact arithmetic and agree with it
 to the con2prim tolerance; the pack equals the single-block fast stage bit for bit.
