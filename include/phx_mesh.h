@@ -79,10 +79,12 @@ int phx_mesh_get(const phx_mesh *m, int gid, int what, double *host);
 /* con2prim iteration counts [cells] of block gid (int32): on the interior those of the
  * last stage's inversion of U_{s+1}, on the ghosts those of the last FillDerived */
 int phx_mesh_get_iters(const phx_mesh *m, int gid, int32_t *host);
-/* every con2prim of the cycle with the learned bracket (mlp non-NULL; it must outlive its
- * use) or the plain solver with the c2p_mu warm start (NULL, the default) */
+/* the learned bracket for every con2prim of the cycle of the cells with no previous root
+ * (c2p_mu <= 0; the others start from it, as in the plain solver) (mlp non-NULL; it must
+ * outlive its use), or the plain solver with the c2p_mu warm start everywhere (NULL, the
+ * default); as phx_pack_set_learned */
 int phx_mesh_set_learned(phx_mesh *m, const phx_mlp *mlp);
-/* bracket misses and learned solves since the last reset */
+/* bracket misses and network-bracketed solves since the last reset */
 int phx_mesh_learned_stats(phx_mesh *m, int reset, int64_t *misses, int64_t *solves);
 /* tunes the y/z flux tile shapes on the mesh's own fields (overwriting fluxes) */
 int phx_mesh_autotune(phx_mesh *m, int reps);

@@ -63,10 +63,15 @@ int phx_pack_nblocks(const phx_pack *pack, int *n, int64_t *interior_cells);
 int phx_pack_set_fixup(phx_ctx *ctx, phx_pack *pack, const phx_fixup_params *fx);
 /* Selects the learned con2prim bracket (include/phx_mlp.h) for the stage's inversion
  * (mlp non-NULL; the network must outlive its use) or the plain solver (NULL, the
- * default).  With the network the update and the inversion are two launches, the second
- * one fusing the MFMA forward pass with the bracketed solve. */
+ * default).  With the network the fused update inverts the cells that have a previous
+ * root (c2p_mu > 0) from it, as the plain stage does, and queues the others; one more
+ * launch brackets the queued cells with the network (its MFMA forward pass fused with the
+ * bracketed solve) and ends at once when the queue is empty.  The first call with a
+ * network allocates the queue (8 bytes per cell of the pack; PHX_ERR_NOMEM if that
+ * fails), kept until phx_pack_destroy. */
 int phx_pack_set_learned(phx_ctx *ctx, phx_pack *pack, const phx_mlp *mlp);
-/* bracket misses (the plain [0, 1] solve ran) and learned solves since the last reset */
+/* bracket misses (the plain [0, 1] solve ran) and network-bracketed solves since the last
+ * reset */
 int phx_pack_learned_stats(phx_pack *pack, int reset, int64_t *misses, int64_t *solves);
 /* One RK stage over the pack (4-10 launches). */
 int phx_pack_stage(phx_ctx *ctx, phx_pack *pack, const phx_pack_stage_args *a,
