@@ -25,6 +25,7 @@
 #include "phx_internal.hpp"
 
 #define PHX_FLUX_NS flux_exact
+#define PHX_FLUX_DISPATCH
 #include "phx_flux_sweeps.hpp"
 
 namespace phx {

@@ -12,4 +12,5 @@
 #include "phx_internal.hpp"
 
 #define PHX_FLUX_NS flux_fast
+#define PHX_FLUX_DISPATCH
 #include "phx_flux_sweeps.hpp"

@@ -171,16 +171,20 @@ __device__ __forceinline__ void flux_tile_body(const DevParams &p, const BlockGe
                                        (long)(DIR == 1 ? o : t) * b.ni + i, flux, fsig);
 }
 
-// the DIR sweep for logical workgroup bx of block b; lds as in march_lds_bytes
-template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM, bool CACHED = false, class PP,
-          class FP>
+// the DIR sweep for logical workgroup bx of block b; lds as in march_lds_bytes.  TI > 0:
+// the column count ti is that constant (the launchers pass it for the common widths), so
+// the ring's row and variable strides fold into the LDS instructions' offsets instead of
+// run-time multiplies; TI = 0: ti as passed
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM, bool CACHED = false, int TI = 0,
+          class PP, class FP>
 __device__ __forceinline__ void flux_march_body(const DevParams &p, const BlockGeom &b, long bx,
-                                                int ti, int nseg, const PP &prim,
+                                                int ti_run, int nseg, const PP &prim,
                                                 const FP &flux,
                                                 double *__restrict__ fsig, double *lds) {
   constexpr int NP = Layout<MHD>::nprim;
   constexpr int H = ReconWidth<RECON>::H;
   constexpr int T = DIR == 1 ? 2 : 1;  // the transverse dimension
+  const int ti = TI > 0 ? TI : ti_run;
   const int MR = kMarchLanes / ti, Q = MR + 2 * H, lg = __builtin_ctz((unsigned)ti);
   const int lane = threadIdx.x;
   const int dt_ = b.ndim > T;
@@ -273,15 +277,15 @@ __device__ __forceinline__ void flux_march_body(const DevParams &p, const BlockG
 }
 
 // Minkowski fits four waves per SIMD (128 VGPRs) without spilling
-template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
+template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM, int TI>
 __global__ __launch_bounds__(kMarchLanes)
 __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 4 : 1))) void k_flux_march(DevParams p, BlockGeom b, int ti,
                                                             int nseg, PrimPtrs prim,
                                                             FluxPtrs flux,
                                                             double *__restrict__ fsig) {
   extern __shared__ double lds[];
-  flux_march_body<DIR, RECON, RIEMANN, MHD, GEOM>(p, b, xcd_remap(blockIdx.x, gridDim.x), ti,
-                                                  nseg, prim, flux, fsig, lds);
+  flux_march_body<DIR, RECON, RIEMANN, MHD, GEOM, false, TI>(
+      p, b, xcd_remap(blockIdx.x, gridDim.x), ti, nseg, prim, flux, fsig, lds);
 }
 
 template <int DIR, int RECON, int RIEMANN, bool MHD, int GEOM>
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(512) void k_pack_flux_tile(DevParams p, PackView v,
       contig(E.flux, E.bg.ncell), E.fsig, lds);
 }
 
-template <int DIR, int RECON, int RIEMANN, int GEOM>
+template <int DIR, int RECON, int RIEMANN, int GEOM, int TI>
 __global__ __launch_bounds__(kMarchLanes)
 __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 4 : 1))) void k_pack_flux_march(DevParams p, PackView v, int ti,
                                                                  int nseg) {
@@ -325,12 +329,25 @@ __attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 4 : 1))) void k_
   const long g = xcd_remap(blockIdx.x, gridDim.x);
   const int b = pack_find(v.start, v.n, g);
   const PackEntry &E = v.e[b];
-  flux_march_body<DIR, RECON, RIEMANN, true, GEOM, GEOM == PHX_GEOM_FMKS>(
+  flux_march_body<DIR, RECON, RIEMANN, true, GEOM, GEOM == PHX_GEOM_FMKS, TI>(
       p, E.bg, g - v.start[b], ti, nseg, contig(E.prim, E.bg.ncell), contig(E.flux, E.bg.ncell),
       E.fsig, lds);
 }
 
 // ---- dispatch --------------------------------------------------------------------------
+// In the fast arithmetic the MHD marching kernels of the Minkowski and FMKS metrics (the
+// stage's and the torus packs') are built for a compile-time column count of 8 and 16, the
+// widths the autotuner keeps; every other case, and the other widths, take the run-time
+// column count (fewer kernels to build)
+template <bool MHD, int GEOM, class F>
+static void with_march_width(int ti, F &&f) {
+  if constexpr (kFluxFast && MHD && GEOM != PHX_GEOM_SPHERICAL_KS) {
+    if (ti == 8) return f(std::integral_constant<int, 8>{});
+    if (ti == 16) return f(std::integral_constant<int, 16>{});
+  }
+  f(std::integral_constant<int, 0>{});
+}
+
 template <int RECON, int RIEMANN, bool MHD, int GEOM>
 static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const PrimPtrs &prim,
                        const FluxPtrs &flux, double *fsig, hipStream_t s, TileShape shape) {
@@ -353,12 +370,15 @@ static void launch_dir(int d, const DevParams &dp, const BlockGeom &bg, const Pr
   if (ts.ts < 0) {
     const unsigned blocks = (unsigned)flux_march_groups(bg, d, ts.ti, -ts.ts);
     const size_t lds = (size_t)march_lds_bytes(NP, H, ts.ti);
-    if (d == 1)
-      hipLaunchKernelGGL((k_flux_march<1, RECON, RIEMANN, MHD, GEOM>), dim3(blocks),
-                         dim3(kMarchLanes), lds, s, dp, bg, ts.ti, -ts.ts, prim, flux, fsig);
-    else
-      hipLaunchKernelGGL((k_flux_march<2, RECON, RIEMANN, MHD, GEOM>), dim3(blocks),
-                         dim3(kMarchLanes), lds, s, dp, bg, ts.ti, -ts.ts, prim, flux, fsig);
+    with_march_width<MHD, GEOM>(ts.ti, [&](auto W) {
+      constexpr int TI = decltype(W)::value;
+      if (d == 1)
+        hipLaunchKernelGGL((k_flux_march<1, RECON, RIEMANN, MHD, GEOM, TI>), dim3(blocks),
+                           dim3(kMarchLanes), lds, s, dp, bg, ts.ti, -ts.ts, prim, flux, fsig);
+      else
+        hipLaunchKernelGGL((k_flux_march<2, RECON, RIEMANN, MHD, GEOM, TI>), dim3(blocks),
+                           dim3(kMarchLanes), lds, s, dp, bg, ts.ti, -ts.ts, prim, flux, fsig);
+    });
     return;
   }
   const int ts_n = ts.ts, ti_n = ts.ti;
@@ -406,24 +426,46 @@ static void launch_riemann(int riemann, int mhd, int geom, int d, const DevParam
     launch_mhd<RECON, PHX_RIEMANN_LLF>(mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
 }
 
+// The sweeps of one reconstruction, block and pack launches.  The WENO5-Z ones are
+// instantiated in a translation unit of their own (phx_flux_weno.hip,
+// phx_flux_fast_weno.hip) so that the builds of the kernels run in parallel; the including
+// unit defines PHX_FLUX_DISPATCH for the rest and the dispatch below.
+template <int RECON>
+void launch_recon(int riemann, int mhd, int geom, int d, const DevParams &dp,
+                  const BlockGeom &bg, const PrimPtrs &prim, const FluxPtrs &flux, double *fsig,
+                  hipStream_t s, TileShape sh);
+template <int RECON>
+void launch_pack_recon(int d, int riemann, int geom, const DevParams &dp, const PackView &v,
+                       long groups, TileShape sh, hipStream_t s);
+
+#ifdef PHX_FLUX_DISPATCH
+extern template void launch_recon<PHX_RECON_WENO5Z>(int, int, int, int, const DevParams &,
+                                                    const BlockGeom &, const PrimPtrs &,
+                                                    const FluxPtrs &, double *, hipStream_t,
+                                                    TileShape);
+extern template void launch_pack_recon<PHX_RECON_WENO5Z>(int, int, int, const DevParams &,
+                                                         const PackView &, long, TileShape,
+                                                         hipStream_t);
+
 void launch_fluxes(int d, int recon, int riemann, int mhd, int geom, const DevParams &dp,
                    const BlockGeom &bg, const PrimPtrs &prim, const FluxPtrs &flux, double *fsig,
                    hipStream_t s, TileShape sh) {
   switch (recon) {
   case PHX_RECON_CONSTANT:
-    launch_riemann<PHX_RECON_CONSTANT>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
+    launch_recon<PHX_RECON_CONSTANT>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   case PHX_RECON_LINEAR:
-    launch_riemann<PHX_RECON_LINEAR>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
+    launch_recon<PHX_RECON_LINEAR>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   case PHX_RECON_WENO5Z:
-    launch_riemann<PHX_RECON_WENO5Z>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
+    launch_recon<PHX_RECON_WENO5Z>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   default:
-    launch_riemann<PHX_RECON_MP5>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
+    launch_recon<PHX_RECON_MP5>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
     break;
   }
 }
+#endif
 
 template <int RECON, int RIEMANN, int GEOM>
 static void pack_dir(int d, const DevParams &dp, const PackView &v, long groups, TileShape sh,
@@ -437,12 +479,17 @@ static void pack_dir(int d, const DevParams &dp, const PackView &v, long groups,
   if (sh.ts < 0) {
     const size_t lds =
         (size_t)march_lds_bytes(Layout<true>::nprim, ReconWidth<RECON>::H, sh.ti);
-    if (d == 1)
-      hipLaunchKernelGGL((k_pack_flux_march<1, RECON, RIEMANN, GEOM>), dim3((unsigned)groups),
-                         dim3(kMarchLanes), lds, s, dp, v, sh.ti, -sh.ts);
-    else
-      hipLaunchKernelGGL((k_pack_flux_march<2, RECON, RIEMANN, GEOM>), dim3((unsigned)groups),
-                         dim3(kMarchLanes), lds, s, dp, v, sh.ti, -sh.ts);
+    with_march_width<true, GEOM>(sh.ti, [&](auto W) {
+      constexpr int TI = decltype(W)::value;
+      if (d == 1)
+        hipLaunchKernelGGL((k_pack_flux_march<1, RECON, RIEMANN, GEOM, TI>),
+                           dim3((unsigned)groups), dim3(kMarchLanes), lds, s, dp, v, sh.ti,
+                           -sh.ts);
+      else
+        hipLaunchKernelGGL((k_pack_flux_march<2, RECON, RIEMANN, GEOM, TI>),
+                           dim3((unsigned)groups), dim3(kMarchLanes), lds, s, dp, v, sh.ti,
+                           -sh.ts);
+    });
     return;
   }
   const int threads = ((sh.ti * sh.ts + 63) / 64) * 64;
@@ -466,15 +513,36 @@ static void pack_recon(int d, int riemann, int geom, const DevParams &dp, const 
   });
 }
 
+template <int RECON>
+void launch_recon(int riemann, int mhd, int geom, int d, const DevParams &dp,
+                  const BlockGeom &bg, const PrimPtrs &prim, const FluxPtrs &flux, double *fsig,
+                  hipStream_t s, TileShape sh) {
+  launch_riemann<RECON>(riemann, mhd, geom, d, dp, bg, prim, flux, fsig, s, sh);
+}
+template <int RECON>
+void launch_pack_recon(int d, int riemann, int geom, const DevParams &dp, const PackView &v,
+                       long groups, TileShape sh, hipStream_t s) {
+  pack_recon<RECON>(d, riemann, geom, dp, v, groups, sh, s);
+}
+
+#ifdef PHX_FLUX_DISPATCH
 void launch_pack_flux(int d, int recon, int riemann, int geom, const DevParams &dp,
                       const PackView &v, long groups, TileShape sh, hipStream_t s) {
   switch (recon) {
-  case PHX_RECON_CONSTANT: pack_recon<PHX_RECON_CONSTANT>(d, riemann, geom, dp, v, groups, sh, s); break;
-  case PHX_RECON_LINEAR: pack_recon<PHX_RECON_LINEAR>(d, riemann, geom, dp, v, groups, sh, s); break;
-  case PHX_RECON_WENO5Z: pack_recon<PHX_RECON_WENO5Z>(d, riemann, geom, dp, v, groups, sh, s); break;
-  default: pack_recon<PHX_RECON_MP5>(d, riemann, geom, dp, v, groups, sh, s); break;
+  case PHX_RECON_CONSTANT: launch_pack_recon<PHX_RECON_CONSTANT>(d, riemann, geom, dp, v, groups, sh, s); break;
+  case PHX_RECON_LINEAR: launch_pack_recon<PHX_RECON_LINEAR>(d, riemann, geom, dp, v, groups, sh, s); break;
+  case PHX_RECON_WENO5Z: launch_pack_recon<PHX_RECON_WENO5Z>(d, riemann, geom, dp, v, groups, sh, s); break;
+  default: launch_pack_recon<PHX_RECON_MP5>(d, riemann, geom, dp, v, groups, sh, s); break;
   }
 }
+#else
+template void launch_recon<PHX_RECON_WENO5Z>(int, int, int, int, const DevParams &,
+                                             const BlockGeom &, const PrimPtrs &,
+                                             const FluxPtrs &, double *, hipStream_t, TileShape);
+template void launch_pack_recon<PHX_RECON_WENO5Z>(int, int, int, const DevParams &,
+                                                  const PackView &, long, TileShape,
+                                                  hipStream_t);
+#endif
 
 }  // namespace PHX_FLUX_NS
 }  // namespace phx
