@@ -557,3 +557,47 @@ def test_sweep_shapes_agree_on_curved_metrics(phx, geom):
     assert np.isfinite(first[0]).any()
     for d in (1, 2):
         phx.set_tile_shape(p, b, d, 0, 0)
+
+
+def test_stage1_same_container_average_bits(phx):
+    """Stage 1 of the RK2 cycle: beta = 1 with U_s the same container as U^n.  The update
+    does not store U_s back where the average 1 U_s + 0 U_s is U_s itself; the container
+    must hold exactly what the oracle's AverageIndependentData leaves, including a NaN where
+    a component is infinite (inf + 0 inf), and U_{s+1} must equal the oracle's update."""
+    say("[gpu] stage 1 in one container: U_s bits after the average")
+    rng = np.random.default_rng(17)
+    b = geom_block(3, (16, 12, 10), A.GEOM_MINKOWSKI)
+    p = geom_params(A.GEOM_MINKOWSKI, gamma=13.0 / 9.0, mhd=True, recon=A.RECON_WENO5Z,
+                    riemann=A.RIEMANN_HLL)
+    op, ob = oparams(p), oblock(b)
+    prim = random_block(rng, b, True, 13.0 / 9.0, A.GEOM_MINKOWSKI)
+    u0, _ = O.prim_to_con_block(op, ob, prim)
+    I = O.interior(ob)
+    # an infinite component in two interior cells, a negative zero in another
+    cells = [tuple(s.start + 2 for s in I), tuple(s.start + 5 for s in I)]
+    u0[4][cells[0]] = np.inf
+    u0[1][cells[1]] = -np.inf
+    u0[2][tuple(s.start + 3 for s in I)] = -0.0
+    flux, _ = O.calculate_fluxes(op, ob, prim)
+    O.flux_ct(op, ob, flux)
+    stage_ref = u0.copy()
+    out_ref = O.flux_divergence_update(op, ob, flux, None, stage_ref, u0.copy(), 1.0, 1e-3)
+    # the inversion re-derives U_{s+1} where it used a bound
+    lo = [s.start for s in I[::-1]]
+    hi = [s.stop - 1 for s in I[::-1]]
+    O.con2prim_block(op, ob, out_ref, prim.copy(), np.zeros(O.block_shape(ob)), lo=lo, hi=hi)
+    st = BlockState(p, b)
+    st.prim.upload(prim)
+    st.cons.upload(u0)
+    d_out = DeviceArray(st.cons.shape)
+    d_out.zero()
+    d_pout = DeviceArray(st.prim.shape)
+    d_pout.zero()
+    phx.fluid_stage(p, b, st.fields(), 1.0, 1e-3, st.cons, d_out, d_pout)
+    synchronize()
+    us = st.cons.get()
+    sl = (slice(None),) + I
+    assert np.array_equal(np.signbit(us[sl]), np.signbit(stage_ref[sl])), "U_s signs"
+    assert np.array_equal(us[sl], stage_ref[sl], equal_nan=True), "U_s after the average"
+    assert np.isnan(us[4][cells[0]]) and np.isnan(us[1][cells[1]])
+    assert np.array_equal(d_out.get()[sl], out_ref[sl], equal_nan=True), "U_{s+1}"
