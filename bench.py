@@ -43,7 +43,8 @@ B_ALG_STAGE = 52 * 8
 # the sources the single-block stage's kernels are compiled from (flux sweeps, FluxCT,
 # fused update + con2prim, dt, p2c) with every header they include
 STAGE_SOURCES = ["phoebus_amd/csrc/" + f for f in (
-    "phx_kernels.hip", "phx_flux.hip", "phx_flux_fast.hip", "phx_stage.hip", "phx_device.hpp",
+    "phx_kernels.hip", "phx_flux.hip", "phx_flux_fast.hip", "phx_flux_weno.hip",
+    "phx_flux_fast_weno.hip", "phx_stage.hip", "phx_device.hpp",
     "phx_internal.hpp", "phx_cells.hpp", "phx_flux_shapes.hpp", "phx_flux_sweeps.hpp")] + [
     "include/phx.h", "include/phx_fixup.h"]
 

@@ -63,8 +63,11 @@ int phx_autotune(phx_ctx *ctx, const phx_fluid_params *p, const phx_block *b,
       for (const auto &c : cand) dup = dup || (c.ti == r.ti && c.ts == r.ts);
       if (!dup) cand.push_back(r);
     }
-    // marching waves: ti columns by 64 / ti rows, in nseg segments of the faces
-    static const int kSeg[] = {1, 2, 3, 4, 6, 8};
+    // marching waves: ti columns by 64 / ti rows, in nseg segments of the faces.  Every
+    // count up to 10 and a few above: the wave count is nseg times the column chunks, and
+    // which count fills the chip's last round of waves best (four waves per SIMD, 4096 at
+    // once) depends on the block's cross-section
+    static const int kSeg[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16};
     for (int ti : {1, 2, 4, 8, 16})
       for (int nseg : kSeg) {
         const TileShape r = resolve_tile_shape(d, p->recon, p->mhd, bg, TileShape{ti, -nseg});

@@ -27,3 +27,27 @@ def test_help_lists_the_stage_choices():
         assert opt in r.stdout
     for choice in ("weno5z", "plm", "mp5", "constant", "hll", "llf"):
         assert choice in r.stdout
+
+
+def test_stage_profile_digest_covers_every_sweep_unit():
+    """The digest that dates the committed stage profiles covers every translation unit
+    the stage's kernels are compiled from: each .hip file that includes the sweeps header
+    or defines the stage's update, CT or dt kernels."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    listed = set(bench.STAGE_SOURCES)
+    csrc = os.path.join(ROOT, "phoebus_amd", "csrc")
+    units = {"phx_stage.hip", "phx_kernels.hip"}  # the update, EMF, dt and p2c kernels
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith(".hip") and \
+                '#include "phx_flux_sweeps.hpp"' in open(os.path.join(csrc, name)).read():
+            units.add(name)
+    assert len(units) >= 6
+    for name in sorted(units):
+        assert "phoebus_amd/csrc/" + name in listed, name
+    for f in listed:
+        assert os.path.exists(os.path.join(ROOT, f)), f
+    assert len(bench.sources_digest()) == 64
