@@ -227,11 +227,32 @@ def c3_learned_vs_plain(phx, args, log):
 
     p = fluid_params(gamma=4.0 / 3.0, mhd=True, geom=A.GEOM_SPHERICAL_KS, a=0.9375,
                      c2p_tol=1e-10, c2p_max_iter=100)
-    t0 = time.perf_counter()
-    net, losses = train_bracket(phx, p, np.random.default_rng(SEED + 1), args.c3_pairs,
-                                args.c3_train_steps)
-    t_train = time.perf_counter() - t0
-    log(f"c3: trained {args.c3_train_steps} steps on {args.c3_pairs} pairs in {t_train:.1f} s")
+    loaded = bool(args.c3_net) and os.path.exists(args.c3_net)
+    if loaded:
+        # a network an earlier run of the same configuration trained and saved (profiling
+        # passes reuse it instead of repeating the training launches)
+        from phoebus_amd.api import Mlp
+
+        z = np.load(args.c3_net)
+        net = Mlp(phx)
+        net.set_params(z["params"])
+        net.set_normalization(z["mean"], z["std"], float(z["width"][0]), float(z["width"][1]))
+        net.norm = (z["mean"], z["std"])
+        losses, t_train = [float(x) for x in z["losses"]], float(z["seconds"])
+        train_steps = int(z["steps"])
+        log(f"c3: network of {args.c3_net} (trained {train_steps} steps in {t_train:.1f} s)")
+    else:
+        t0 = time.perf_counter()
+        train_steps = args.c3_train_steps
+        net, losses = train_bracket(phx, p, np.random.default_rng(SEED + 1), args.c3_pairs,
+                                    train_steps)
+        t_train = time.perf_counter() - t0
+        log(f"c3: trained {args.c3_train_steps} steps on {args.c3_pairs} pairs in "
+            f"{t_train:.1f} s")
+        if args.c3_net:
+            np.savez(args.c3_net, params=net.get_params(), mean=net.norm[0], std=net.norm[1],
+                     width=np.array(net.width), losses=np.array(losses), seconds=t_train,
+                     steps=args.c3_train_steps)
     rng = np.random.default_rng(SEED)
     chunk = min(args.c3_states, 1 << 22)
     n_total = (args.c3_states // chunk) * chunk
@@ -307,8 +328,9 @@ def c3_learned_vs_plain(phx, args, log):
            "states_per_s_learned": n_total / (ms_learned * 1e-3),
            "hist_plain": np.trim_zeros(hp, "b").tolist(),
            "hist_learned": np.trim_zeros(hl, "b").tolist(),
-           "train": {"pairs": args.c3_pairs, "steps": args.c3_train_steps,
-                     "seconds": t_train, "loss_first_last": [losses[0], losses[-1]]},
+           "train": {"pairs": args.c3_pairs, "steps": train_steps,
+                     "seconds": t_train, "loss_first_last": [losses[0], losses[-1]],
+                     "loaded_from": args.c3_net if loaded else None},
            "bracket_first_chunk": centre}
     if not args.no_cpu and sample is not None:
         # FP64 oracle on a sample of the first chunk: its root against the learned one
@@ -1043,7 +1065,10 @@ def main():
                          "waves of TI columns in -TS segments) instead of autotuning it")
     ap.add_argument("--c3-states", type=int, default=1 << 26)
     ap.add_argument("--c3-pairs", type=int, default=1 << 22)
-    ap.add_argument("--c3-train-steps", type=int, default=30000)
+    ap.add_argument("--c3-train-steps", type=int, default=200000)
+    ap.add_argument("--c3-net", default="", metavar="FILE.npz",
+                    help="C3's trained network: loaded from this file when it exists, else "
+                         "trained and saved there")
     ap.add_argument("--no-c4", action="store_true", help="skip the uneven block pack leg")
     ap.add_argument("--c4-blocks", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=5)

@@ -88,7 +88,7 @@ def test_training_cuts_iterations_and_keeps_results(phx):
     p = fluid_params(gamma=4 / 3, mhd=True, geom=A.GEOM_SPHERICAL_KS, a=0.9375, c2p_tol=1e-10,
                      c2p_max_iter=100)
     net, losses = train_bracket(phx, p, np.random.default_rng(777), n_pairs=1 << 20,
-                                steps=8000)
+                                steps=20000)
     assert losses[-1] < 0.2 * losses[0], losses
     # disjoint evaluation set
     n = 1 << 15
@@ -107,8 +107,9 @@ def test_training_cuts_iterations_and_keeps_results(phx):
     assert np.array_equal(fail, ev["fail"])
     plain, learned = ev["iters"][ok].mean(), it[ok].mean()
     say(f"    plain {plain:.3f} learned {learned:.3f} fallback {fb[ok].mean():.4f}")
-    # training is deterministic (gradients summed in a fixed order)
-    assert learned * 1.4 < plain
+    # training is deterministic (gradients summed in a fixed order); 20,000 steps on 2^20
+    # pairs give 1.82x fewer iterations
+    assert learned * 1.6 < plain
     assert fb[ok].mean() < 0.05
     # same root to the solver tolerance: both stop inside [root - tol, root + tol]
     assert np.max(np.abs(mu[ok] - ev["mu"][ok])) <= 4 * p.c2p_tol
