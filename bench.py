@@ -39,6 +39,12 @@ CLOCK_HZ = 2.4e9             # peak engine clock
 # mu, T guess = 28 doubles; writes U_{s+1}, 11 prims incl. T, mu, fail, 3 cell speeds = 24)
 B_ALG_STAGE = 52 * 8
 
+# the bracket network on torus cells (C4, C5): their centre errors differ by decades between
+# the floored atmosphere and the disc, so the half-width needs the shared layers shaped by
+# its own loss (weight 0.05; the C3 default 1e-5 leaves them to the centre, and 67 % of
+# C4's cold cells then fall outside the bracket, against 4 %)
+TORUS_BRACKET_TRAINING = {"lam": 0.05, "width_scale": 4.0}
+
 
 # the sources the single-block stage's kernels are compiled from (flux sweeps, FluxCT,
 # fused update + con2prim, dt, p2c) with every header they include
@@ -543,7 +549,8 @@ def c4_uneven_pack(phx, args, log):
             data = {k: np.ascontiguousarray(v[..., sel]) for k, v in data.items()}
         data["d_coords"] = DeviceArray.from_host(data["coords"])
         t0 = time.perf_counter()
-        net, losses = train_bracket_on(phx, p, np.random.default_rng(SEED + 7), data, steps=6000)
+        net, losses = train_bracket_on(phx, p, np.random.default_rng(SEED + 7), data, steps=6000,
+                                       **TORUS_BRACKET_TRAINING)
         train_s = time.perf_counter() - t0
         pk.set_learned(net)
         gl = pk.graph(stages, s)
@@ -801,7 +808,7 @@ def c5_torus_mesh(phx, args, log):
     data["d_coords"] = DeviceArray.from_host(data["coords"])
     t0 = time.perf_counter()
     net, losses = train_bracket_on(phx, p, np.random.default_rng(SEED + 5), data,
-                                   steps=args.c5_train_steps)
+                                   steps=args.c5_train_steps, **TORUS_BRACKET_TRAINING)
     train_s = time.perf_counter() - t0
     log(f"[c5] trained on {data['cons'].shape[1]} pairs from steps 1-{L} in {train_s:.1f} s, "
         f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
