@@ -52,7 +52,9 @@ def _compare(m, om, exact, what=""):
             for a, b, name in ((gp, op, "prim"), (gc, oc, "cons")):
                 scale = np.max(np.abs(b), axis=(1, 2, 3), keepdims=True) + 1e-300
                 err = np.max(np.abs(a - b) / scale)
-                assert err < 1e-8, f"{what} block {g} {name}: {err}"
+                say(f"    block {g} {name}: largest difference {err:.2e} of the component scale")
+                # measured: up to 1.3e-10 on the FMKS torus mesh after two steps
+                assert err < 1e-9, f"{what} block {g} {name}: {err}"
 
 
 CASES = {

@@ -311,8 +311,9 @@ def test_block_stage_parity(phx, case):
         I = O.interior(ob)
         gsrc = st.src.get()
         # FMKS derivatives are finite differences with step ~6e-6: ulp-level metric
-        # differences grow by 1/step
-        rtol_src = 1e-8 if geom == A.GEOM_FMKS else 1e-11
+        # differences grow by 1/step (measured: <= 5.1e-12 of the scale for FMKS, 7e-16
+        # for Kerr-Schild's analytic derivatives)
+        rtol_src = 1e-10 if geom == A.GEOM_FMKS else 1e-13
         for l in range(1, 5):
             assert_match(gsrc[l][I], src_ref[l][I], False, f"src {l}", rtol=rtol_src,
                          atol_scale=np.max(np.abs(src_ref[1:5][(slice(None),) + I])))
@@ -468,11 +469,15 @@ def test_fused_stage_matches_oracle_sequence(phx, case):
         assert np.array_equal(d_it.get()[I], r["iters"][I])
         assert d_dt.get()[0] == dt_ref
     else:
+        # U_{s+1} carries the sources' differences (FMKS, above) and, in zones where the
+        # inversion re-derived it from bounded primitives, the root's (measured: 2.2e-12
+        # of the scale for FMKS, 2.3e-18 for Kerr-Schild; dt equal in both)
+        rtol_u = 1e-11 if geom == A.GEOM_FMKS else 1e-13
         for m in range(gout.shape[0]):
-            assert_match(gout[m][I], out_ref[m][I], False, f"U_out m={m}", rtol=1e-10,
+            assert_match(gout[m][I], out_ref[m][I], False, f"U_out m={m}", rtol=rtol_u,
                          atol_scale=np.max(np.abs(out_ref[m][I])))
         assert np.max(np.abs(st.c2p_mu.get()[I] - mu_ref[I])) <= 4 * p.c2p_tol
-        assert abs(d_dt.get()[0] - dt_ref) <= 1e-8 * dt_ref
+        assert abs(d_dt.get()[0] - dt_ref) <= 1e-10 * dt_ref
 
 
 @pytest.mark.parametrize("recon,riemann", [
