@@ -39,10 +39,11 @@ CLOCK_HZ = 2.4e9             # peak engine clock
 # mu, T guess = 28 doubles; writes U_{s+1}, 11 prims incl. T, mu, fail, 3 cell speeds = 24)
 B_ALG_STAGE = 52 * 8
 
-# the bracket network on torus cells (C4, C5): their centre errors differ by decades between
-# the floored atmosphere and the disc, so the half-width needs the shared layers shaped by
-# its own loss (weight 0.05; the C3 default 1e-5 leaves them to the centre, and 67 % of
-# C4's cold cells then fall outside the bracket, against 4 %)
+# the bracket network on torus cells (the uneven pack and the torus mesh): their centre
+# errors differ by decades between the floored atmosphere and the disc, so the half-width
+# needs the shared layers shaped by its own loss (weight 0.05; the batched states' default
+# 1e-5 leaves them to the centre, and 61-67 % of the pack's cold cells then fall outside
+# the bracket, against 4-25 %)
 TORUS_BRACKET_TRAINING = {"lam": 0.05, "width_scale": 4.0}
 
 

@@ -666,12 +666,14 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_con2prim_batch_learned(
 // A grid-stride loop over the queue's count, so a stage whose cells all have a previous
 // root (a time-dependent run) costs one launch that reads the count and ends; a workgroup
 // with no queued cell loads no weights.
-// (three waves per SIMD fit without spilling on Minkowski blocks only; the curved metrics
-// would spill 22-34 VGPRs)
+// Two waves per SIMD: 243 VGPRs on Minkowski blocks, 255-256 with 14-18 spilled on the
+// curved metrics, which left to themselves take 274-280 and run one wave per SIMD (the
+// first stage of the 512-block torus pack: 25.4 ms against 27.4); three waves would spill
+// 82 on Minkowski.
 constexpr int kColdGrid = 768;  // workgroups: 3 per CU (the LDS of the weights allows 3)
 template <int GEOM>
 __global__ __launch_bounds__(kLearnedThreads)
-__attribute__((amdgpu_waves_per_eu(GEOM == PHX_GEOM_MINKOWSKI ? 3 : 1))) void
+__attribute__((amdgpu_waves_per_eu(2))) void
 k_pack_con2prim_cold(DevParams p, MlpDev m, PackView v, ColdQueue q,
                      unsigned long long *counters, double *extra) {
   using M = typename MetricOf<GEOM>::type;
@@ -722,17 +724,33 @@ k_pack_con2prim_cold(DevParams p, MlpDev m, PackView v, ColdQueue q,
     double cs[L::ncons], pr[L::nprim], sig[3];
 #pragma unroll
     for (int t = 0; t < L::ncons; ++t) cs[t] = cons[t * n + c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sig[d] = d < bg.ndim ? E.csig[d * n + c] : 0.0;
+    // the cell's CFL time with face speeds, as the update forms it (a cell the inversion
+    // leaves keeps its signal speeds and still counts)
+    auto fold_dt = [&]() {
+      if (!extra) return;
+      const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
+      double ldt = 0.0;
+      for (int d = 0; d < bg.ndim; ++d) {
+        const double s = smax(sig[d], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
+        ldt += s / bg.dx[d];
+      }
+      const double t = 1.0 / (ldt + 1.e-50);
+      if (!isnan(t))  // positive doubles order as their bit patterns
+        atomicMin(reinterpret_cast<unsigned long long *>(extra),
+                  (unsigned long long)__double_as_longlong(t));
+    };
     if (isnan(cs[0]) + isnan(cs[1]) + isnan(cs[4]) > 0) {
       E.fail[c] = 0.0;
       if (iters) iters[c] = 0;
+      fold_dt();
       continue;
     }
     const double X[3] = {xc(bg, 0, idx[0]), xc(bg, 1, idx[1]), xc(bg, 2, idx[2])};
     const M gm = metric_at<GEOM, CACHED>(p, bg, 0, (long)idx[1] * bg.ni + idx[0], X[0], X[1], X[2]);
 #pragma unroll
     for (int t = 0; t < L::nprim; ++t) pr[t] = prim[t * n + c];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) sig[d] = d < bg.ndim ? E.csig[d * n + c] : 0.0;
     double tt = E.temp ? E.temp[c] : 0.0, root = E.mu[c];
     C2PState z;
     c2p_setup<true>(p, gm, X, cs, pr, z);
@@ -771,18 +789,7 @@ k_pack_con2prim_cold(DevParams p, MlpDev m, PackView v, ColdQueue q,
     E.mu[c] = root;
     E.fail[c] = st;
     if (iters) iters[c] = it;
-    if (extra) {  // the cell's CFL time with face speeds, as the update forms it
-      const long off[3] = {1, bg.ni, (long)bg.ni * bg.nj};
-      double ldt = 0.0;
-      for (int d = 0; d < bg.ndim; ++d) {
-        const double s = smax(sig[d], smax(E.fsig[d * n + c], E.fsig[d * n + c + off[d]]));
-        ldt += s / bg.dx[d];
-      }
-      const double t = 1.0 / (ldt + 1.e-50);
-      if (!isnan(t))  // positive doubles order as their bit patterns
-        atomicMin(reinterpret_cast<unsigned long long *>(extra),
-                  (unsigned long long)__double_as_longlong(t));
-    }
+    fold_dt();
   }
 }
 
