@@ -246,12 +246,12 @@ def c3_learned_vs_plain(phx, args, log):
         net.set_normalization(z["mean"], z["std"], float(z["width"][0]), float(z["width"][1]))
         net.norm = (z["mean"], z["std"])
         losses, t_train = [float(x) for x in z["losses"]], float(z["seconds"])
-        train_steps = int(z["steps"])
+        train_steps, train_pairs = int(z["steps"]), int(z["pairs"])
         log(f"c3: network of {args.c3_net} (trained {train_steps} steps in {t_train:.1f} s)")
     else:
         t0 = time.perf_counter()
-        train_steps = args.c3_train_steps
-        net, losses = train_bracket(phx, p, np.random.default_rng(SEED + 1), args.c3_pairs,
+        train_steps, train_pairs = args.c3_train_steps, args.c3_pairs
+        net, losses = train_bracket(phx, p, np.random.default_rng(SEED + 1), train_pairs,
                                     train_steps)
         t_train = time.perf_counter() - t0
         log(f"c3: trained {args.c3_train_steps} steps on {args.c3_pairs} pairs in "
@@ -259,7 +259,7 @@ def c3_learned_vs_plain(phx, args, log):
         if args.c3_net:
             np.savez(args.c3_net, params=net.get_params(), mean=net.norm[0], std=net.norm[1],
                      width=np.array(net.width), losses=np.array(losses), seconds=t_train,
-                     steps=args.c3_train_steps)
+                     steps=train_steps, pairs=train_pairs)
     rng = np.random.default_rng(SEED)
     chunk = min(args.c3_states, 1 << 22)
     n_total = (args.c3_states // chunk) * chunk
@@ -335,7 +335,7 @@ def c3_learned_vs_plain(phx, args, log):
            "states_per_s_learned": n_total / (ms_learned * 1e-3),
            "hist_plain": np.trim_zeros(hp, "b").tolist(),
            "hist_learned": np.trim_zeros(hl, "b").tolist(),
-           "train": {"pairs": args.c3_pairs, "steps": train_steps,
+           "train": {"pairs": train_pairs, "steps": train_steps,
                      "seconds": t_train, "loss_first_last": [losses[0], losses[-1]],
                      "loaded_from": args.c3_net if loaded else None},
            "bracket_first_chunk": centre}
