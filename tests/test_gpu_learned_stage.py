@@ -43,6 +43,44 @@ def _close(a, b, rtol):
     return ok.all(), float(np.nanmax(np.abs(a - b) / np.maximum(den, 1e-300)))
 
 
+def test_pack_learned_nan_cell_keeps_its_cfl_time(phx):
+    """A cell with no previous root whose conserved state is NaN is left by both inversions
+    (fail flag 0) with its signal speeds, and its CFL time still counts: given the largest
+    speeds of the pack it sets dt_out, the same in the plain and the learned stage."""
+    say("[gpu] pack stage with a NaN cell: learned vs plain time step")
+    geom = A.GEOM_MINKOWSKI
+    rng = np.random.default_rng(43)
+    p = _params(geom)
+    blocks = _blocks(geom)
+    S = Setup(phx, p, blocks, rng, geom)
+    pk = Pack(phx, p, [(b, st, c1, it) for b, st, c1, it in zip(blocks, S.states, S.cons1, S.iters)])
+    net = _net(phx, 1e6, 1e-9)
+    st0 = S.states[0]  # an 8^3 block with 4 ghosts: (6, 6, 6) is interior
+
+    def stage(learned):
+        S.reset()
+        cons = st0.cons.get()
+        cons[0, 6, 6, 6] = np.nan  # U^n, hence U_1, of the cell
+        st0.cons.upload(cons)
+        sig = st0.cell_sig.get()
+        sig[:, 6, 6, 6] = 50.0  # its speeds before the stage
+        st0.cell_sig.upload(sig)
+        pk.set_learned(net if learned else None)
+        d_dt = DeviceArray((1,))
+        pk.stage(1, dt=1e-3, cfl=0.4, dt_out=d_dt)
+        synchronize()
+        return float(d_dt.get()[0]), float(st0.fail.get()[6, 6, 6])
+
+    dt_plain, fail_plain = stage(False)
+    dt_learned, fail_learned = stage(True)
+    assert fail_plain == 0.0 and fail_learned == 0.0
+    assert dt_learned == dt_plain, (dt_learned, dt_plain)
+    want = 0.4 / (50.0 / 0.1 + 50.0 / 0.2 + 50.0 / 0.3)  # dx = (0.1, 0.2, 0.3)
+    assert abs(dt_plain - want) <= 1e-14 * want, (dt_plain, want)
+    pk.close()
+    net.close()
+
+
 @pytest.mark.parametrize("width", ["wide", "narrow"])
 def test_pack_learned_stage_matches_plain(phx, width):
     say(f"[gpu] pack stage, learned bracket ({width}) vs plain")
